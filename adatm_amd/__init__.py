@@ -1,0 +1,409 @@
+"""MI355X AdaTM: Python mirror of libsplatt_adatm's C API (ctypes).
+
+The compute path is the C library (host C + HIP kernels for gfx950); this
+module only marshals arrays.  Loading fails loudly when the library has not
+been built -- there is no Python or CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libsplatt_adatm.so"
+CLI_PATH = PKG / "bin" / "splatt"
+
+MAX_NMODES = 16
+SPLATT_SUCCESS = 1
+SPLATT_ERROR_BADINPUT = 2
+SPLATT_ERROR_NOMEMORY = 3
+SPLATT_ERROR_DEVICE = 4
+
+FP64, FP32, BF16ROWS = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16rows": BF16ROWS}
+
+DIST_UNIFORM, DIST_ZIPF, DIST_CLUSTER = 0, 1, 2
+VALS_U01, VALS_LOGNORMAL, VALS_PM1 = 0, 1, 2
+
+# SPLATT option indices (include/splatt.h)
+OPT_NTHREADS, OPT_TOLERANCE, OPT_NITER, OPT_VERBOSITY, OPT_RANDSEED = 0, 1, 2, 3, 4
+OPT_CSF_ALLOC, OPT_TILE, OPT_TILEDEPTH, OPT_DECOMP, OPT_COMM = 5, 6, 7, 8, 9
+OPT_PRECISION, OPT_PLANNER, OPT_GRAPH, OPT_REFQUIRKS = 10, 11, 12, 13
+OPT_SPACE_LIMIT, OPT_DEVICE, OPT_NOPTIONS = 14, 15, 16
+CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
+ADAPTIVE, ADAPTIVE_TIME_EFFICIENT, ADAPTIVE_SPACE_EFFICIENT = 3, 4, 5
+
+
+class AdatmError(RuntimeError):
+    """A libsplatt_adatm entry point returned an error code."""
+
+
+class Coo(C.Structure):
+    _fields_ = [
+        ("nmodes", C.c_uint32),
+        ("nnz", C.c_uint64),
+        ("dims", C.c_uint64 * MAX_NMODES),
+        ("ind", C.POINTER(C.c_uint32) * MAX_NMODES),
+        ("vals", C.POINTER(C.c_double)),
+    ]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [
+        ("nmodes", C.c_uint32),
+        ("dims", C.c_uint64 * MAX_NMODES),
+        ("ndraws", C.c_uint64),
+        ("index_dist", C.c_int),
+        ("zipf_s", C.c_double * MAX_NMODES),
+        ("nclusters", C.c_uint64),
+        ("cluster_sigma", C.c_double),
+        ("value_dist", C.c_int),
+        ("seed", C.c_uint64),
+    ]
+
+
+class Group(C.Structure):
+    _fields_ = [
+        ("memo_mode", C.c_uint32),
+        ("begin_imten", C.c_uint32),
+        ("n_imten", C.c_uint32),
+        ("o_opt", C.c_uint32 * MAX_NMODES),
+    ]
+
+
+class Plan(C.Structure):
+    _fields_ = [
+        ("n_grp", C.c_uint32),
+        ("grp", Group * MAX_NMODES),
+        ("pspace", C.c_uint64),
+        ("pops", C.c_uint64),
+    ]
+
+    def groups(self, nmodes: int) -> list[tuple[int, int, int, list[int]]]:
+        return [
+            (g.memo_mode, g.begin_imten, g.n_imten, list(g.o_opt[:nmodes]))
+            for g in self.grp[: self.n_grp]
+        ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import "
+            "__graft_entry__ as g; g.build()')")
+    L = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    sig = {
+        "adatm_last_error": (C.c_char_p, []),
+        "adatm_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+        "adatm_synth_named": (C.c_int, [C.c_int, C.c_uint64, P(SynthSpec)]),
+        "adatm_synth": (C.c_int, [P(SynthSpec), P(Coo)]),
+        "adatm_coo_canonicalize": (C.c_int, [P(Coo), P(P(C.c_uint32))]),
+        "adatm_coo_free": (None, [P(Coo)]),
+        "adatm_coo_alloc": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_uint64), P(Coo)]),
+        "adatm_rand_matrix": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, P(C.c_double)]),
+        "adatm_plan_candidates": (C.c_int, [C.c_uint32, P(C.c_uint64), P(Plan), C.c_int]),
+        "adatm_count_fibers": (C.c_int, [P(Coo), P(C.c_uint32), P(C.c_uint64), C.c_int]),
+        "adatm_plan_analytic": (C.c_int, [P(Coo), C.c_uint64, C.c_int, C.c_double, C.c_int,
+                                          P(Plan), C.c_int, P(C.c_int), P(C.c_int)]),
+        "adatm_plan_use": (None, [P(Plan), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
+        "adatm_engine_create": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
+                                          P(C.c_void_p)]),
+        "adatm_engine_destroy": (None, [C.c_void_p]),
+        "adatm_engine_set_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "adatm_engine_get_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "adatm_engine_get_lambda": (C.c_int, [C.c_void_p, P(C.c_double)]),
+        "adatm_engine_init_factors": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "adatm_engine_mttkrp": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "adatm_engine_mttkrp_seq": (C.c_int, [C.c_void_p]),
+        "adatm_engine_cpd": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_int,
+                                       P(C.c_double), P(C.c_uint32)]),
+        "adatm_engine_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        P(C.c_double), P(C.c_double)]),
+        "adatm_engine_stats": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_double),
+                                         P(C.c_double), P(C.c_uint64)]),
+        "adatm_engine_get_csf": (C.c_int, [C.c_void_p, C.c_uint32, P(P(C.c_uint32)),
+                                           P(P(C.c_uint32)), P(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != SPLATT_SUCCESS:
+        msg = lib().adatm_last_error().decode(errors="replace")
+        raise AdatmError(f"{what} failed with code {rc}: {msg}")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _u64ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+# --------------------------------------------------------------------------
+class Tensor:
+    """A coordinate tensor owned by the C library (u32 indices, f64 values)."""
+
+    def __init__(self, coo: Coo):
+        self.c = coo
+
+    @classmethod
+    def from_arrays(cls, inds: np.ndarray, vals: np.ndarray, dims) -> "Tensor":
+        inds = np.asarray(inds)
+        N, nnz = inds.shape
+        coo = Coo()
+        d = np.asarray(dims, dtype=np.uint64)
+        check(lib().adatm_coo_alloc(N, nnz, _u64ptr(d), C.byref(coo)), "coo_alloc")
+        t = cls(coo)
+        for m in range(N):
+            t.ind(m)[:] = inds[m]
+        t.vals[:] = vals
+        return t
+
+    @classmethod
+    def synth(cls, spec: SynthSpec) -> "Tensor":
+        coo = Coo()
+        check(lib().adatm_synth(C.byref(spec), C.byref(coo)), "synth")
+        return cls(coo)
+
+    @property
+    def nmodes(self) -> int:
+        return int(self.c.nmodes)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.c.nnz)
+
+    @property
+    def dims(self) -> list[int]:
+        return [int(x) for x in self.c.dims[: self.nmodes]]
+
+    def ind(self, m: int) -> np.ndarray:
+        return np.ctypeslib.as_array(self.c.ind[m], shape=(self.nnz,))
+
+    @property
+    def vals(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.c.vals, shape=(self.nnz,))
+
+    def inds(self) -> np.ndarray:
+        return np.stack([self.ind(m) for m in range(self.nmodes)])
+
+    def canonicalize(self) -> list[np.ndarray]:
+        """Merge duplicates, compact empty indices; returns indmap per mode."""
+        maps = (C.POINTER(C.c_uint32) * MAX_NMODES)()
+        old_dims = self.dims
+        check(lib().adatm_coo_canonicalize(C.byref(self.c), maps), "canonicalize")
+        out = []
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for m in range(self.nmodes):
+            n = self.dims[m]
+            arr = np.ctypeslib.as_array(maps[m], shape=(max(n, 1),))[:n].copy()
+            libc.free(C.cast(maps[m], C.c_void_p))
+            out.append(arr)
+        self.orig_dims = old_dims
+        return out
+
+    def free(self) -> None:
+        if self.c is not None and self.c.nmodes:
+            lib().adatm_coo_free(C.byref(self.c))
+            self.c.nmodes = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def named_spec(config: int, ndraws: int = 0) -> SynthSpec:
+    spec = SynthSpec()
+    check(lib().adatm_synth_named(config, ndraws, C.byref(spec)), "synth_named")
+    return spec
+
+
+def make_spec(dims, ndraws, dist=DIST_UNIFORM, zipf_s=1.1, nclusters=0, sigma=0.0,
+              values=VALS_U01, seed=1) -> SynthSpec:
+    spec = SynthSpec()
+    spec.nmodes = len(dims)
+    for m, d in enumerate(dims):
+        spec.dims[m] = int(d)
+        spec.zipf_s[m] = float(zipf_s[m] if np.ndim(zipf_s) else zipf_s)
+    spec.ndraws = int(ndraws)
+    spec.index_dist = dist
+    spec.nclusters = int(nclusters)
+    spec.cluster_sigma = float(sigma)
+    spec.value_dist = values
+    spec.seed = int(seed)
+    return spec
+
+
+def synth_tensor(config: int, ndraws: int = 0, canonical: bool = True) -> Tensor:
+    t = Tensor.synth(named_spec(config, ndraws))
+    if canonical:
+        t.canonicalize()
+    return t
+
+
+def rand_matrix(seed: int, mode: int, rows: int, rank: int) -> np.ndarray:
+    out = np.empty((rows, rank), dtype=np.float64)
+    lib().adatm_rand_matrix(seed, mode, rows, rank, _dptr(out))
+    return out
+
+
+def philox(ctr, key) -> list[int]:
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().adatm_philox4x32_10(_uptr(c), _uptr(k), _uptr(o))
+    return [int(x) for x in o]
+
+
+# --------------------------------------------------------------------------
+def plan_candidates(nmodes: int, dims) -> list[Plan]:
+    d = np.asarray(list(dims) + [0] * (MAX_NMODES - len(dims)), dtype=np.uint64)
+    arr = (Plan * 256)()
+    n = lib().adatm_plan_candidates(nmodes, _u64ptr(d), arr, 256)
+    return [arr[i] for i in range(n)]
+
+
+def count_fibers(t: Tensor, order, device: int = -1) -> list[int]:
+    o = np.asarray(list(order), dtype=np.uint32)
+    F = np.zeros(MAX_NMODES, dtype=np.uint64)
+    check(lib().adatm_count_fibers(C.byref(t.c), _uptr(o), _u64ptr(F), device),
+          "count_fibers")
+    return [int(x) for x in F[: t.nmodes]]
+
+
+def plan_analytic(t: Tensor, rank: int, ref_quirks: bool = True,
+                  space_limit: float = 90e9, device: int = -1):
+    """(candidates, chosen index) of the analytic planner."""
+    arr = (Plan * 256)()
+    nc = C.c_int()
+    ch = C.c_int()
+    check(lib().adatm_plan_analytic(C.byref(t.c), rank, int(ref_quirks), space_limit,
+                                    device, arr, 256, C.byref(nc), C.byref(ch)),
+          "plan_analytic")
+    return [arr[i] for i in range(nc.value)], ch.value
+
+
+def plan_use(plan: Plan, nmodes: int):
+    g = np.zeros(MAX_NMODES, dtype=np.uint32)
+    t = np.zeros(MAX_NMODES, dtype=np.uint32)
+    lib().adatm_plan_use(C.byref(plan), nmodes, _uptr(g), _uptr(t))
+    return [int(x) for x in g[:nmodes]], [int(x) for x in t[:nmodes]]
+
+
+# --------------------------------------------------------------------------
+class Engine:
+    """A memoization plan resident on one GPU (wraps adatm_engine)."""
+
+    def __init__(self, t: Tensor, rank: int, plan: Plan, precision: int = FP64,
+                 device: int = 0):
+        self.h = C.c_void_p()
+        self.N = t.nmodes
+        self.R = int(rank)
+        self.dims = t.dims
+        self.plan = plan
+        self.precision = precision
+        check(lib().adatm_engine_create(C.byref(t.c), rank, C.byref(plan), precision,
+                                        device, C.byref(self.h)), "engine_create")
+
+    def close(self) -> None:
+        if self.h:
+            lib().adatm_engine_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_factor(self, m: int, A: np.ndarray) -> None:
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        assert A.shape == (self.dims[m], self.R)
+        check(lib().adatm_engine_set_factor(self.h, m, _dptr(A)), "set_factor")
+
+    def get_factor(self, m: int) -> np.ndarray:
+        A = np.empty((self.dims[m], self.R), dtype=np.float64)
+        check(lib().adatm_engine_get_factor(self.h, m, _dptr(A)), "get_factor")
+        return A
+
+    def get_lambda(self) -> np.ndarray:
+        lam = np.empty(self.R, dtype=np.float64)
+        check(lib().adatm_engine_get_lambda(self.h, _dptr(lam)), "get_lambda")
+        return lam
+
+    def init_factors(self, seed: int) -> None:
+        check(lib().adatm_engine_init_factors(self.h, seed), "init_factors")
+
+    def mttkrp(self, m: int, out: bool = True):
+        M = np.empty((self.dims[m], self.R), dtype=np.float64) if out else None
+        check(lib().adatm_engine_mttkrp(self.h, m, _dptr(M) if out else None), "mttkrp")
+        return M
+
+    def mttkrp_seq(self) -> None:
+        check(lib().adatm_engine_mttkrp_seq(self.h), "mttkrp_seq")
+
+    def cpd(self, niters: int, tol: float = 0.0, graph: bool = False):
+        fits = np.zeros(niters, dtype=np.float64)
+        done = C.c_uint32()
+        check(lib().adatm_engine_cpd(self.h, niters, tol, int(graph), _dptr(fits),
+                                     C.byref(done)), "cpd")
+        return fits[: done.value]
+
+    def time(self, what: int, nwarm: int, nrep: int, graph: bool = False):
+        ms = C.c_double()
+        pk = np.zeros(24, dtype=np.float64)
+        check(lib().adatm_engine_time(self.h, what, nwarm, nrep, int(graph),
+                                      C.byref(ms), _dptr(pk)), "engine_time")
+        return ms.value, pk
+
+    def stats(self):
+        F = np.zeros(self.plan.n_grp * self.N, dtype=np.uint64)
+        b = C.c_double()
+        f = C.c_double()
+        db = C.c_uint64()
+        check(lib().adatm_engine_stats(self.h, _u64ptr(F), C.byref(b), C.byref(f),
+                                       C.byref(db)), "engine_stats")
+        return F.reshape(self.plan.n_grp, self.N), b.value, f.value, db.value
+
+    def get_csf(self, g: int, nnz: int):
+        F, _, _, _ = self.stats()
+        F = [int(x) for x in F[g]]
+        N = self.N
+        fptr = [np.zeros(F[l] + 1, dtype=np.uint32) if l < N - 1 else None
+                for l in range(N)]
+        fids = [np.zeros(F[l], dtype=np.uint32) if l >= 1 else None for l in range(N)]
+        pf = (C.POINTER(C.c_uint32) * MAX_NMODES)()
+        pi = (C.POINTER(C.c_uint32) * MAX_NMODES)()
+        for l in range(N):
+            if fptr[l] is not None:
+                pf[l] = _uptr(fptr[l])
+            if fids[l] is not None:
+                pi[l] = _uptr(fids[l])
+        vals = np.zeros(nnz, dtype=np.float64)
+        check(lib().adatm_engine_get_csf(self.h, g, pf, pi, _dptr(vals)), "get_csf")
+        return F, fptr, fids, vals

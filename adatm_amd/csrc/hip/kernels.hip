@@ -1,0 +1,925 @@
+/*
+ * HIP kernels of the memoized MTTKRP / CP-ALS path for MI355X (gfx950).
+ *
+ * Wave64 throughout.  A "lane group" of G = min(R, 64) lanes owns one work
+ * item; lane j of the group holds columns j, j+64, ... of the R-wide rows,
+ * so every factor-row gather and every memo-row read of a group is one
+ * contiguous R*sizeof(T)-byte access.
+ *
+ *   k_seg_reduce  -- leaf SpTTM (p_spttm, mttkrp.c:891-928) and level
+ *                    Hadamard-reduce (p_spt2t_add_hada / p_spt2m_add_hada,
+ *                    mttkrp.c:828-888) as one segmented reduction over the
+ *                    children of a CSF level.  Work items cover whole
+ *                    parents, so outputs are stored, never accumulated; a
+ *                    parent with more children than one item holds is cut
+ *                    into pieces whose partial rows go to carry slots and
+ *                    are summed in piece order by k_fixup (deterministic,
+ *                    no atomics, no zero-fill).
+ *   k_scatter     -- memo-reuse MTTKRP of a non-root level
+ *                    (p_csf_mttkrp_internal_reuse_adaptive,
+ *                    mttkrp.c:1547-1682): the reference scatters
+ *                    Y(parent) o Z(node) into output rows under 1024 striped
+ *                    locks; here level nodes are pre-sorted by output row,
+ *                    so each output row is a segment of the same reduction.
+ *   dense steps   -- Gram, Hadamard-of-Grams inverse, A = M V^-1, column
+ *                    normalisation, fit (adatm_cpd.c:276-340 and the SPLATT
+ *                    matrix routines it calls) in FP64 arithmetic.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "adatm_kern.h"
+
+#define NONE32 0xFFFFFFFFu
+
+namespace {
+
+/* ---- element loads/stores per storage type ----------------------------- */
+struct bf16raw {
+  uint16_t v;
+};
+
+__device__ __forceinline__ float ldu(float const *p) { return *p; }
+__device__ __forceinline__ double ldu(double const *p) { return *p; }
+__device__ __forceinline__ float ldu(bf16raw const *p)
+{
+  return __uint_as_float(((uint32_t)p->v) << 16);
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16); /* inf/nan */
+  u += 0x7fffu + ((u >> 16) & 1u); /* round to nearest even */
+  return (uint16_t)(u >> 16);
+}
+
+/* ---- segmented Hadamard-reduce ----------------------------------------- */
+template <typename T, typename TU, bool LEAF, int CPL>
+__device__ __forceinline__ void acc_child(T (&acc)[CPL], uint32_t c,
+                                          uint32_t const *__restrict__ ids,
+                                          T const *__restrict__ xin,
+                                          TU const *__restrict__ U,
+                                          uint32_t R, uint32_t col0)
+{
+  uint32_t const id = ids[c];
+  TU const *urow = U + (size_t)id * R;
+  if (LEAF) {
+    T const x = xin[c];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) acc[j] += x * (T)ldu(urow + col);
+    }
+  } else {
+    T const *zrow = xin + (size_t)c * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) acc[j] += zrow[col] * (T)ldu(urow + col);
+    }
+  }
+}
+
+/* children [c, e) accumulated in order; four children's loads in flight */
+template <typename T, typename TU, bool LEAF, int CPL>
+__device__ __forceinline__ void acc_range(T (&acc)[CPL], uint32_t c, uint32_t e,
+                                          uint32_t const *__restrict__ ids,
+                                          T const *__restrict__ xin,
+                                          TU const *__restrict__ U, uint32_t R,
+                                          uint32_t col0)
+{
+  for (; c + 4 <= e; c += 4) {
+    uint32_t const i0 = ids[c], i1 = ids[c + 1], i2 = ids[c + 2],
+                   i3 = ids[c + 3];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) {
+        T const u0 = (T)ldu(U + (size_t)i0 * R + col);
+        T const u1 = (T)ldu(U + (size_t)i1 * R + col);
+        T const u2 = (T)ldu(U + (size_t)i2 * R + col);
+        T const u3 = (T)ldu(U + (size_t)i3 * R + col);
+        T x0, x1, x2, x3;
+        if (LEAF) {
+          x0 = xin[c];
+          x1 = xin[c + 1];
+          x2 = xin[c + 2];
+          x3 = xin[c + 3];
+        } else {
+          x0 = xin[(size_t)c * R + col];
+          x1 = xin[(size_t)(c + 1) * R + col];
+          x2 = xin[(size_t)(c + 2) * R + col];
+          x3 = xin[(size_t)(c + 3) * R + col];
+        }
+        T a = acc[j];
+        a += x0 * u0;
+        a += x1 * u1;
+        a += x2 * u2;
+        a += x3 * u3;
+        acc[j] = a;
+      }
+    }
+  }
+  for (; c < e; ++c)
+    acc_child<T, TU, LEAF, CPL>(acc, c, ids, xin, U, R, col0);
+}
+
+template <typename T, typename TU, bool LEAF, int CPL>
+__global__ __launch_bounds__(256) void k_seg_reduce(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ ptr, uint32_t const *__restrict__ ids,
+    T const *__restrict__ xin, TU const *__restrict__ U, T *__restrict__ out,
+    T *__restrict__ carry, uint32_t R, uint32_t G, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / G;
+  if (grp >= gpw) return;
+  uint32_t const col0 = lane - grp * G;
+  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t const k = wave * gpw + grp;
+  if (k >= nitems) return;
+
+  uint32_t const a = istart[k];
+  uint32_t const slot = islot[k];
+  T acc[CPL];
+  if (slot != NONE32) {
+    uint32_t const b = istart[k + 1];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
+    acc_range<T, TU, LEAF, CPL>(acc, a, b, ids, xin, U, R, col0);
+    T *crow = carry + (size_t)slot * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) crow[col] = acc[j];
+    }
+    return;
+  }
+  uint32_t p = iseg[k];
+  uint32_t const pe = iseg[k + 1];
+  uint32_t c = a;
+  for (; p < pe; ++p) {
+    uint32_t const e = ptr[p + 1];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
+    acc_range<T, TU, LEAF, CPL>(acc, c, e, ids, xin, U, R, col0);
+    c = e;
+    T *orow = out + (size_t)p * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) orow[col] = acc[j];
+    }
+  }
+}
+
+/* out[hseg[h]] = sum of the carry rows of its pieces, in piece order */
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_fixup(
+    uint32_t nheavy, uint32_t const *__restrict__ hseg,
+    uint32_t const *__restrict__ hslot, T const *__restrict__ carry,
+    T *__restrict__ out, uint32_t R, uint32_t G, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / G;
+  if (grp >= gpw) return;
+  uint32_t const col0 = lane - grp * G;
+  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t const h = wave * gpw + grp;
+  if (h >= nheavy) return;
+  T acc[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
+  for (uint32_t s = hslot[h]; s < hslot[h + 1]; ++s) {
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) acc[j] += carry[(size_t)s * R + col];
+    }
+  }
+  T *orow = out + (size_t)hseg[h] * R;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    uint32_t const col = col0 + 64u * j;
+    if (col < R) orow[col] = acc[j];
+  }
+}
+
+/* ---- reuse scatter over id-sorted level nodes -------------------------- */
+template <typename T, typename TU, int CPL>
+__device__ __forceinline__ void scatter_node(T (&acc)[CPL], uint32_t n,
+                                             adatm_kpath const &path,
+                                             T const *__restrict__ Z,
+                                             uint32_t R, uint32_t col0)
+{
+  uint32_t const d = path.d;
+  /* ancestors: anc[l] = node at level l on the path to n, l < d */
+  uint32_t anc[ADATM_KMAXL];
+  uint32_t p = path.par[d][n];
+  anc[d - 1] = p;
+  for (uint32_t l = d - 1; l >= 1; --l) {
+    p = path.par[l][p];
+    anc[l - 1] = p;
+  }
+  T y[CPL];
+  {
+    TU const *u0 = (TU const *)path.U[0] + (size_t)anc[0] * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      y[j] = (col < R) ? (T)ldu(u0 + col) : (T)0;
+    }
+  }
+  for (uint32_t l = 1; l < d; ++l) {
+    uint32_t const id = path.ids[l][anc[l]];
+    TU const *ul = (TU const *)path.U[l] + (size_t)id * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) y[j] = y[j] * (T)ldu(ul + col);
+    }
+  }
+  T const *zrow = Z + (size_t)n * R;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    uint32_t const col = col0 + 64u * j;
+    if (col < R) acc[j] += zrow[col] * y[j];
+  }
+}
+
+template <typename T, typename TU, int CPL>
+__global__ __launch_bounds__(256) void k_scatter(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ rowptr, uint32_t const *__restrict__ perm,
+    adatm_kpath path, T const *__restrict__ Z, T *__restrict__ out,
+    T *__restrict__ carry, uint32_t R, uint32_t G, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / G;
+  if (grp >= gpw) return;
+  uint32_t const col0 = lane - grp * G;
+  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t const k = wave * gpw + grp;
+  if (k >= nitems) return;
+  uint32_t const a = istart[k];
+  uint32_t const slot = islot[k];
+  T acc[CPL];
+  if (slot != NONE32) {
+    uint32_t const b = istart[k + 1];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
+    for (uint32_t q = a; q < b; ++q)
+      scatter_node<T, TU, CPL>(acc, perm[q], path, Z, R, col0);
+    T *crow = carry + (size_t)slot * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) crow[col] = acc[j];
+    }
+    return;
+  }
+  uint32_t const pe = iseg[k + 1];
+  uint32_t q = a;
+  for (uint32_t p = iseg[k]; p < pe; ++p) {
+    uint32_t const e = rowptr[p + 1];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
+    for (; q < e; ++q) scatter_node<T, TU, CPL>(acc, perm[q], path, Z, R, col0);
+    T *orow = out + (size_t)p * R;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      uint32_t const col = col0 + 64u * j;
+      if (col < R) orow[col] = acc[j];
+    }
+  }
+}
+
+/* ---- misc --------------------------------------------------------------- */
+__global__ void k_zero_words(uint32_t *p, size_t nwords)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t const stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+__global__ void k_zero_bytes(uint8_t *p, size_t n)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <typename T>
+__global__ void k_convert_in(double const *__restrict__ src, T *__restrict__ dst,
+                             bf16raw *__restrict__ dbf, uint64_t n)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    T const v = (T)src[i];
+    dst[i] = v;
+    if (dbf) dbf[i].v = to_bf16((float)v);
+  }
+}
+
+template <typename T>
+__global__ void k_convert_out(T const *__restrict__ src, double *__restrict__ dst,
+                              uint64_t n)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = (double)src[i];
+}
+
+/* ---- dense CP-ALS steps ------------------------------------------------- */
+#define DB 256          /* threads per dense block */
+#define NBLK_MAX 256    /* partial-sum blocks */
+#define GROWS 32        /* rows staged per Gram step */
+
+/* partial Gram of rows [lo, hi); optional column scaling 1/lambda written
+ * back to A (and its bf16 copy) before accumulation */
+template <typename T>
+__global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
+                                                  bf16raw *__restrict__ Abf,
+                                                  uint64_t I, uint32_t R,
+                                                  double const *__restrict__ lambda,
+                                                  double *__restrict__ part)
+{
+  extern __shared__ double sh[]; /* GROWS * R */
+  uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
+  uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+  uint64_t const hi = lo + chunk < I ? lo + chunk : I;
+  uint32_t const RR = R * R;
+  double *myp = part + (size_t)blockIdx.x * RR;
+  for (uint32_t e0 = 0; e0 < RR; e0 += DB * 16) {
+    double acc[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+    for (uint64_t r0 = lo; r0 < hi; r0 += GROWS) {
+      uint32_t const nr = (uint32_t)((hi - r0) < GROWS ? (hi - r0) : GROWS);
+      __syncthreads();
+      for (uint32_t x = threadIdx.x; x < nr * R; x += DB) {
+        uint32_t const rr = x / R, cc = x - rr * R;
+        size_t const gi = (size_t)(r0 + rr) * R + cc;
+        double v = (double)A[gi];
+        if (lambda) {
+          T const sv = (T)(v / lambda[cc]);
+          if (e0 == 0) {
+            A[gi] = sv;
+            if (Abf) Abf[gi].v = to_bf16((float)sv);
+          }
+          v = (double)sv;
+        }
+        sh[x] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        uint32_t const e = e0 + threadIdx.x + DB * t;
+        if (e < RR) {
+          uint32_t const i = e / R, j = e - i * R;
+          double s = acc[t];
+          for (uint32_t rr = 0; rr < nr; ++rr) s += sh[rr * R + i] * sh[rr * R + j];
+          acc[t] = s;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      uint32_t const e = e0 + threadIdx.x + DB * t;
+      if (e < RR) myp[e] = acc[t];
+    }
+    /* a second entry chunk re-stages rows; scaling is applied only once */
+    if (lambda && e0 == 0 && RR > DB * 16) {
+      /* subsequent chunks must read the already-scaled values unscaled */
+      lambda = nullptr;
+    }
+  }
+}
+
+__global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ part,
+                                                  uint32_t nblk, uint32_t len,
+                                                  double *__restrict__ out)
+{
+  for (uint32_t e = threadIdx.x + blockIdx.x * DB; e < len; e += DB * gridDim.x) {
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) s += part[(size_t)b * len + e];
+    out[e] = s;
+  }
+}
+
+/* Vinv = (hadamard_{k != m} G_k)^-1 by in-place Gauss-Jordan; the matrix
+ * is symmetric positive definite, so no pivoting is needed.  One block. */
+__global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
+                                                  uint32_t N, uint32_t m,
+                                                  uint32_t R,
+                                                  double *__restrict__ Vinv,
+                                                  int *__restrict__ status)
+{
+  extern __shared__ double a[]; /* R x R, then R elimination factors */
+  double *fac = a + (size_t)R * R;
+  __shared__ double pinv;
+  __shared__ int bad;
+  for (uint32_t x = threadIdx.x; x < R * R; x += DB) {
+    double v = 1.0;
+    for (uint32_t k = 0; k < N; ++k)
+      if (k != m) v *= G[(size_t)k * R * R + x];
+    a[x] = v;
+  }
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (uint32_t k = 0; k < R; ++k) {
+    if (threadIdx.x == 0) {
+      double const p = a[k * R + k];
+      if (!(p > 0.0)) bad = 1;
+      pinv = 1.0 / p;
+    }
+    for (uint32_t i = threadIdx.x; i < R; i += DB) fac[i] = a[i * R + k];
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < R; j += DB)
+      a[k * R + j] = ((j == k) ? 1.0 : a[k * R + j]) * pinv;
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < R * R; x += DB) {
+      uint32_t const i = x / R, j = x - i * R;
+      if (i != k) a[x] = ((j == k) ? 0.0 : a[x]) - fac[i] * a[k * R + j];
+    }
+    __syncthreads();
+  }
+  for (uint32_t x = threadIdx.x; x < R * R; x += DB) Vinv[x] = a[x];
+  if (threadIdx.x == 0 && status) *status = bad;
+}
+
+/* A = M * Vinv (FP64 accumulation), per-block column sum of squares / max */
+template <typename T>
+__global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
+                                                     double const *__restrict__ Vinv,
+                                                     T *__restrict__ A, uint64_t I,
+                                                     uint32_t R,
+                                                     double *__restrict__ part)
+{
+  extern __shared__ double sv[]; /* R*R Vinv, then 2*DB reduction scratch */
+  for (uint32_t x = threadIdx.x; x < R * R; x += DB) sv[x] = Vinv[x];
+  double *red = sv + R * R;
+  __syncthreads();
+  uint32_t const rpi = DB / R; /* rows per iteration */
+  uint32_t const rr = threadIdx.x / R, cc = threadIdx.x - rr * R;
+  double ssq = 0.0, mx = -1.0e300;
+  if (rr < rpi) {
+    uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
+    uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+    uint64_t const hi = lo + chunk < I ? lo + chunk : I;
+    for (uint64_t i = lo + rr; i < hi; i += rpi) {
+      T const *mrow = M + (size_t)i * R;
+      double s = 0.0;
+      for (uint32_t j = 0; j < R; ++j) s += (double)mrow[j] * sv[j * R + cc];
+      T const o = (T)s;
+      A[(size_t)i * R + cc] = o;
+      double const od = (double)o;
+      ssq += od * od;
+      mx = od > mx ? od : mx;
+    }
+  }
+  red[threadIdx.x] = ssq;
+  red[DB + threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double s = 0.0, m = -1.0e300;
+    for (uint32_t t = 0; t < rpi; ++t) {
+      s += red[t * R + threadIdx.x];
+      double const v = red[DB + t * R + threadIdx.x];
+      m = v > m ? v : m;
+    }
+    part[(size_t)blockIdx.x * 2 * R + threadIdx.x] = s;
+    part[(size_t)blockIdx.x * 2 * R + R + threadIdx.x] = m;
+  }
+}
+
+/* lambda from the column statistics: 2-norm (first sweep) or max(1, max) */
+__global__ void k_lambda(double const *__restrict__ part, uint32_t nblk,
+                         uint32_t R, uint32_t const *__restrict__ iter,
+                         double *__restrict__ lambda)
+{
+  uint32_t const r = threadIdx.x;
+  if (r >= R) return;
+  int const first = (*iter == 0u);
+  double s = 0.0, m = -1.0e300;
+  for (uint32_t b = 0; b < nblk; ++b) {
+    s += part[(size_t)b * 2 * R + r];
+    double const v = part[(size_t)b * 2 * R + R + r];
+    m = v > m ? v : m;
+  }
+  lambda[r] = first ? sqrt(s) : (m > 1.0 ? m : 1.0);
+}
+
+/* per-block partial of sum_i A[i,r] * M[i,r] */
+template <typename T>
+__global__ __launch_bounds__(DB) void k_inner_part(T const *__restrict__ A,
+                                                   T const *__restrict__ M,
+                                                   uint64_t I, uint32_t R,
+                                                   double *__restrict__ part)
+{
+  __shared__ double red[DB];
+  uint32_t const rpi = DB / R;
+  uint32_t const rr = threadIdx.x / R, cc = threadIdx.x - rr * R;
+  double s = 0.0;
+  if (rr < rpi) {
+    uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
+    uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+    uint64_t const hi = lo + chunk < I ? lo + chunk : I;
+    for (uint64_t i = lo + rr; i < hi; i += rpi)
+      s += (double)A[(size_t)i * R + cc] * (double)M[(size_t)i * R + cc];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double t = 0.0;
+    for (uint32_t q = 0; q < rpi; ++q) t += red[q * R + threadIdx.x];
+    part[(size_t)blockIdx.x * R + threadIdx.x] = t;
+  }
+}
+
+/* per-block partial sum of squares */
+template <typename T>
+__global__ __launch_bounds__(DB) void k_sqnorm_part(T const *__restrict__ x,
+                                                    uint64_t n,
+                                                    double *__restrict__ part)
+{
+  __shared__ double red[DB];
+  uint64_t const chunk = (n + gridDim.x - 1) / gridDim.x;
+  uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+  uint64_t const hi = lo + chunk < n ? lo + chunk : n;
+  double s = 0.0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += DB) {
+    double const v = (double)x[i];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t k = DB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* fit = 1 - sqrt(|X|^2 + lambda^T (hada G) lambda - 2 <X,Z>) / |X|
+ * (adatm_cpd.c:59-90, 109-160, 179-200); one block */
+__global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ part,
+                                                  uint32_t nblk,
+                                                  double const *__restrict__ lambda,
+                                                  double const *__restrict__ G,
+                                                  uint32_t N, uint32_t R,
+                                                  double ttnormsq,
+                                                  double *__restrict__ fits,
+                                                  uint32_t *__restrict__ iter)
+{
+  __shared__ double red[DB];
+  __shared__ double red2[DB];
+  double inner = 0.0;
+  for (uint32_t r = threadIdx.x; r < R; r += DB) {
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) s += part[(size_t)b * R + r];
+    inner += s * lambda[r];
+  }
+  double nrm = 0.0;
+  for (uint32_t x = threadIdx.x; x < R * R; x += DB) {
+    uint32_t const i = x / R, j = x - i * R;
+    double h = 1.0;
+    for (uint32_t k = 0; k < N; ++k) h *= G[(size_t)k * R * R + x];
+    nrm += h * lambda[i] * lambda[j];
+  }
+  red[threadIdx.x] = inner;
+  red2[threadIdx.x] = nrm;
+  __syncthreads();
+  for (uint32_t s = DB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x] += red[threadIdx.x + s];
+      red2[threadIdx.x] += red2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double const norm_mats = fabs(red2[0]);
+    double const residual = sqrt(ttnormsq + norm_mats - 2.0 * red[0]);
+    uint32_t const it = *iter;
+    fits[it] = 1.0 - residual / sqrt(ttnormsq);
+    *iter = it + 1u;
+  }
+}
+
+/* ---- launch helpers ----------------------------------------------------- */
+void group_shape(uint32_t R, uint32_t *G, uint32_t *gpw, int *cpl)
+{
+  *G = R < 64 ? R : 64;
+  *gpw = 64 / *G;
+  uint32_t const c = (R + 63) / 64;
+  *cpl = c <= 1 ? 1 : (c <= 2 ? 2 : 4);
+}
+
+template <typename T>
+hipError_t launch_fix(hipStream_t s, adatm_kwork const *w, T const *carry,
+                      T *out, uint32_t R)
+{
+  if (w->nheavy == 0) return hipSuccess;
+  uint32_t G, gpw;
+  int cpl;
+  group_shape(R, &G, &gpw, &cpl);
+  uint32_t const per_block = 4 * gpw;
+  dim3 const grid((w->nheavy + per_block - 1) / per_block);
+  switch (cpl) {
+  case 1:
+    hipLaunchKernelGGL((k_fixup<T, 1>), grid, dim3(256), 0, s, w->nheavy,
+                       w->hseg, w->hslot, carry, out, R, G, gpw);
+    break;
+  case 2:
+    hipLaunchKernelGGL((k_fixup<T, 2>), grid, dim3(256), 0, s, w->nheavy,
+                       w->hseg, w->hslot, carry, out, R, G, gpw);
+    break;
+  default:
+    hipLaunchKernelGGL((k_fixup<T, 4>), grid, dim3(256), 0, s, w->nheavy,
+                       w->hseg, w->hslot, carry, out, R, G, gpw);
+    break;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, typename TU, bool LEAF>
+hipError_t launch_reduce(hipStream_t s, adatm_kwork const *w,
+                         uint32_t const *ptr, uint32_t const *ids,
+                         void const *xin, void const *U, void *out, void *carry,
+                         uint32_t R)
+{
+  uint32_t G, gpw;
+  int cpl;
+  group_shape(R, &G, &gpw, &cpl);
+  if (w->nitems > 0) {
+    uint32_t const per_block = 4 * gpw;
+    dim3 const grid((w->nitems + per_block - 1) / per_block);
+    T const *x = (T const *)xin;
+    TU const *u = (TU const *)U;
+    T *o = (T *)out;
+    T *cy = (T *)carry;
+    switch (cpl) {
+    case 1:
+      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 1>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
+                         o, cy, R, G, gpw);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 2>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
+                         o, cy, R, G, gpw);
+      break;
+    default:
+      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 4>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
+                         o, cy, R, G, gpw);
+      break;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_fix<T>(s, w, (T const *)carry, (T *)out, R);
+}
+
+template <typename T, typename TU>
+hipError_t launch_scatter(hipStream_t s, adatm_kwork const *w,
+                          uint32_t const *rowptr, uint32_t const *perm,
+                          adatm_kpath const *path, void const *Z, void *out,
+                          void *carry, uint32_t R)
+{
+  uint32_t G, gpw;
+  int cpl;
+  group_shape(R, &G, &gpw, &cpl);
+  if (w->nitems > 0) {
+    uint32_t const per_block = 4 * gpw;
+    dim3 const grid((w->nitems + per_block - 1) / per_block);
+    T const *z = (T const *)Z;
+    T *o = (T *)out;
+    T *cy = (T *)carry;
+    switch (cpl) {
+    case 1:
+      hipLaunchKernelGGL((k_scatter<T, TU, 1>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
+                         *path, z, o, cy, R, G, gpw);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_scatter<T, TU, 2>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
+                         *path, z, o, cy, R, G, gpw);
+      break;
+    default:
+      hipLaunchKernelGGL((k_scatter<T, TU, 4>), grid, dim3(256), 0, s,
+                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
+                         *path, z, o, cy, R, G, gpw);
+      break;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_fix<T>(s, w, (T const *)carry, (T *)out, R);
+}
+
+uint32_t dense_blocks(uint64_t I)
+{
+  uint64_t const b = (I + 255) / 256;
+  return (uint32_t)(b < 1 ? 1 : (b > NBLK_MAX ? NBLK_MAX : b));
+}
+
+template <typename T>
+hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
+                     double const *lambda, double *part, double *G_out)
+{
+  uint32_t const nb = dense_blocks(I);
+  size_t const lds = (size_t)GROWS * R * sizeof(double);
+  hipLaunchKernelGGL((k_gram_part<T>), dim3(nb), dim3(DB), lds, s, A, Abf, I,
+                     R, lambda, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  uint32_t const len = R * R;
+  hipLaunchKernelGGL(k_sum_parts, dim3((len + DB - 1) / DB), dim3(DB), 0, s,
+                     part, nb, len, G_out);
+  return hipGetLastError();
+}
+
+} // namespace
+
+/* ======================================================================== */
+extern "C" {
+
+static int herr(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
+
+int adatm_k_seg_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
+                       uint32_t const *ptr, uint32_t const *ids,
+                       void const *xin, void const *U, void *out, void *carry,
+                       uint32_t R)
+{
+  hipError_t e;
+  if (prec == KP_FP64)
+    e = leaf ? launch_reduce<double, double, true>(s, w, ptr, ids, xin, U, out, carry, R)
+             : launch_reduce<double, double, false>(s, w, ptr, ids, xin, U, out, carry, R);
+  else if (prec == KP_FP32)
+    e = leaf ? launch_reduce<float, float, true>(s, w, ptr, ids, xin, U, out, carry, R)
+             : launch_reduce<float, float, false>(s, w, ptr, ids, xin, U, out, carry, R);
+  else
+    e = leaf ? launch_reduce<float, bf16raw, true>(s, w, ptr, ids, xin, U, out, carry, R)
+             : launch_reduce<float, bf16raw, false>(s, w, ptr, ids, xin, U, out, carry, R);
+  return herr(e);
+}
+
+int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
+                     uint32_t const *rowptr, uint32_t const *perm,
+                     adatm_kpath const *path, void const *Z, void *out,
+                     void *carry, uint32_t R)
+{
+  if (path->d < 1 || path->d >= ADATM_KMAXL) return (int)hipErrorInvalidValue;
+  hipError_t e;
+  if (prec == KP_FP64)
+    e = launch_scatter<double, double>(s, w, rowptr, perm, path, Z, out, carry, R);
+  else if (prec == KP_FP32)
+    e = launch_scatter<float, float>(s, w, rowptr, perm, path, Z, out, carry, R);
+  else
+    e = launch_scatter<float, bf16raw>(s, w, rowptr, perm, path, Z, out, carry, R);
+  return herr(e);
+}
+
+int adatm_k_zero(hipStream_t s, void *p, size_t nbytes)
+{
+  if (nbytes == 0) return 0;
+  if ((((uintptr_t)p) & 3u) == 0 && (nbytes & 3u) == 0) {
+    size_t const nw = nbytes / 4;
+    size_t b = (nw + 255) / 256;
+    if (b > 65536) b = 65536;
+    hipLaunchKernelGGL(k_zero_words, dim3((uint32_t)b), dim3(256), 0, s,
+                       (uint32_t *)p, nw);
+  } else {
+    hipLaunchKernelGGL(k_zero_bytes, dim3((uint32_t)((nbytes + 255) / 256)),
+                       dim3(256), 0, s, (uint8_t *)p, nbytes);
+  }
+  return herr(hipGetLastError());
+}
+
+size_t adatm_k_part_doubles(uint32_t R)
+{
+  size_t const a = (size_t)R * R, b = 2 * (size_t)R;
+  return (size_t)NBLK_MAX * (a > b ? a : b) + 64;
+}
+
+int adatm_k_gram(hipStream_t s, int prec, void const *A, uint64_t I, uint32_t R,
+                 double *part, double *G_out)
+{
+  hipError_t e;
+  if (prec == KP_FP64)
+    e = gram_impl<double>(s, (double *)A, nullptr, I, R, nullptr, part, G_out);
+  else
+    e = gram_impl<float>(s, (float *)A, nullptr, I, R, nullptr, part, G_out);
+  return herr(e);
+}
+
+int adatm_k_sqnorm(hipStream_t s, int prec, void const *x, uint64_t n,
+                   double *part, double *out)
+{
+  uint32_t const nb = NBLK_MAX;
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((k_sqnorm_part<double>), dim3(nb), dim3(DB), 0, s,
+                       (double const *)x, n, part);
+  else
+    hipLaunchKernelGGL((k_sqnorm_part<float>), dim3(nb), dim3(DB), 0, s,
+                       (float const *)x, n, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(DB), 0, s, part, nb, 1u, out);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_solve_inv(hipStream_t s, double const *G, uint32_t N, uint32_t m,
+                      uint32_t R, double *Vinv, int *status)
+{
+  size_t const lds = ((size_t)R * R + R) * sizeof(double);
+  hipLaunchKernelGGL(k_solve_inv, dim3(1), dim3(DB), lds, s, G, N, m, R, Vinv,
+                     status);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_update(hipStream_t s, int prec, void const *M, double const *Vinv,
+                   void *A, void *Abf, uint64_t I, uint32_t R,
+                   uint32_t const *iter, double *lambda, double *part,
+                   double *G_out)
+{
+  uint32_t const nb = dense_blocks(I);
+  size_t const lds = ((size_t)R * R + 2 * DB) * sizeof(double);
+  hipError_t e;
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((k_matmul_stats<double>), dim3(nb), dim3(DB), lds, s,
+                       (double const *)M, Vinv, (double *)A, I, R, part);
+  else
+    hipLaunchKernelGGL((k_matmul_stats<float>), dim3(nb), dim3(DB), lds, s,
+                       (float const *)M, Vinv, (float *)A, I, R, part);
+  e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(R < 64 ? 64 : ((R + 63) / 64) * 64),
+                     0, s, part, nb, R, iter, lambda);
+  e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  if (prec == KP_FP64)
+    e = gram_impl<double>(s, (double *)A, nullptr, I, R, lambda, part, G_out);
+  else
+    e = gram_impl<float>(s, (float *)A, (bf16raw *)Abf, I, R, lambda, part,
+                         G_out);
+  return herr(e);
+}
+
+int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
+                uint64_t I_last, double const *lambda, double const *G,
+                uint32_t N, uint32_t R, double ttnormsq, double *part,
+                double *fits, uint32_t *iter)
+{
+  uint32_t const nb = dense_blocks(I_last);
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((k_inner_part<double>), dim3(nb), dim3(DB), 0, s,
+                       (double const *)A_last, (double const *)M, I_last, R,
+                       part);
+  else
+    hipLaunchKernelGGL((k_inner_part<float>), dim3(nb), dim3(DB), 0, s,
+                       (float const *)A_last, (float const *)M, I_last, R,
+                       part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(k_fit_final, dim3(1), dim3(DB), 0, s, part, nb, lambda, G,
+                     N, R, ttnormsq, fits, iter);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_convert_in(hipStream_t s, int prec, double const *src64, void *dst,
+                       void *dst_bf, uint64_t n)
+{
+  uint64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((k_convert_in<double>), dim3((uint32_t)b), dim3(256), 0,
+                       s, src64, (double *)dst, (bf16raw *)nullptr, n);
+  else
+    hipLaunchKernelGGL((k_convert_in<float>), dim3((uint32_t)b), dim3(256), 0,
+                       s, src64, (float *)dst, (bf16raw *)dst_bf, n);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_convert_out(hipStream_t s, int prec, void const *src, double *dst64,
+                        uint64_t n)
+{
+  uint64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((k_convert_out<double>), dim3((uint32_t)b), dim3(256), 0,
+                       s, (double const *)src, dst64, n);
+  else
+    hipLaunchKernelGGL((k_convert_out<float>), dim3((uint32_t)b), dim3(256), 0,
+                       s, (float const *)src, dst64, n);
+  return herr(hipGetLastError());
+}
+
+} /* extern "C" */

@@ -1,0 +1,191 @@
+/*
+ * adatm.h -- native entry points of libsplatt_adatm.so beyond the SPLATT API.
+ *
+ * These expose the device-resident engine directly (no host CSF copies), the
+ * synthetic workload generators and the planner's statistics.  All functions
+ * return a splatt_error_type (SPLATT_SUCCESS == 1) unless stated otherwise.
+ * Plain C ABI: pointers, sizes, no C++ or torch types.
+ */
+#ifndef ADATM_ADATM_H
+#define ADATM_ADATM_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include "splatt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADATM_MAX_NMODES 16 /* engine limit; the planner handles N = 3..16 */
+
+/* ----------------------------------------------------------------------- *
+ * Coordinate tensors in host memory.
+ * ----------------------------------------------------------------------- */
+typedef struct {
+  uint32_t nmodes;
+  uint64_t nnz;
+  uint64_t dims[ADATM_MAX_NMODES];
+  uint32_t *ind[ADATM_MAX_NMODES]; /* nnz entries each, 0-based */
+  double *vals;                    /* nnz entries */
+} adatm_coo;
+
+/* Index distributions of the synthetic generators. */
+typedef enum {
+  ADATM_DIST_UNIFORM = 0,
+  ADATM_DIST_ZIPF = 1,     /* P(k) ~ (k+1)^-s on [0, dim) by exact inverse CDF */
+  ADATM_DIST_CLUSTER = 2   /* Gaussian clusters, see adatm_synth_spec */
+} adatm_index_dist;
+
+typedef enum {
+  ADATM_VALS_UNIFORM01 = 0,  /* U(0,1) */
+  ADATM_VALS_LOGNORMAL = 1,  /* exp(N(0,1)) */
+  ADATM_VALS_UNIFORM_PM1 = 2 /* U(-1,1) */
+} adatm_value_dist;
+
+typedef struct {
+  uint32_t nmodes;
+  uint64_t dims[ADATM_MAX_NMODES];
+  uint64_t ndraws;         /* coordinates drawn (before duplicate merging) */
+  int index_dist;          /* adatm_index_dist, all modes */
+  double zipf_s[ADATM_MAX_NMODES];
+  uint64_t nclusters;      /* ADATM_DIST_CLUSTER */
+  double cluster_sigma;    /* ADATM_DIST_CLUSTER */
+  int value_dist;          /* adatm_value_dist */
+  uint64_t seed;
+} adatm_synth_spec;
+
+/* Draw a raw coordinate tensor (duplicates possible) from a spec.
+ * Philox4x32-10 counter stream: draw j only depends on (seed, j). */
+int adatm_synth(adatm_synth_spec const *spec, adatm_coo *out);
+
+/* Fill the spec of one of the named workloads:
+ *   1: 4-mode 2000x1500x1000x500, 2M uniform draws, U(0,1) values
+ *   2: 5-mode 1e5x5e4x2e4x5e3x1e3, 100M draws, Zipf(1.1) per mode, lognormal
+ *   3: 8-mode 1000^8, 50M draws, 100000 Gaussian clusters (sigma 20), U(-1,1)
+ * ndraws_override > 0 replaces the number of draws (bounded samples). */
+int adatm_synth_named(int config, uint64_t ndraws_override,
+                      adatm_synth_spec *spec);
+
+/* Merge duplicate coordinates (values summed in draw order), drop empty
+ * indices of every mode (dims shrink; indmap[m][new] = old, optional) and
+ * leave the tensor sorted lexicographically by mode 0,1,...,N-1. */
+int adatm_coo_canonicalize(adatm_coo *t, uint32_t **indmap_out);
+
+void adatm_coo_free(adatm_coo *t);
+
+/* malloc an nnz-entry tensor with the given dims (contents undefined) */
+int adatm_coo_alloc(uint32_t nmodes, uint64_t nnz, uint64_t const *dims,
+                    adatm_coo *out);
+
+/* Factor matrix initialisation used by both the CPU and the GPU paths:
+ * A[i*R + r] = U(0,1) from Philox stream (seed, mode). */
+void adatm_rand_matrix(uint64_t seed, uint32_t mode, uint64_t rows,
+                       uint64_t rank, double *out);
+
+/* Philox4x32-10 block (exposed for known-answer tests). */
+void adatm_philox4x32_10(uint32_t const ctr[4], uint32_t const key[2],
+                         uint32_t out[4]);
+
+/* ----------------------------------------------------------------------- *
+ * Memoization plans.
+ * ----------------------------------------------------------------------- */
+typedef struct {
+  uint32_t memo_mode;
+  uint32_t begin_imten;
+  uint32_t n_imten;
+  uint32_t o_opt[ADATM_MAX_NMODES];
+} adatm_group;
+
+typedef struct {
+  uint32_t n_grp;
+  adatm_group grp[ADATM_MAX_NMODES];
+  uint64_t pspace; /* analytic model: predicted bytes (reference units) */
+  uint64_t pops;   /* analytic model: predicted products */
+} adatm_plan;
+
+/* Enumerate the candidate plans of an N-mode tensor (adatm_bench.c:276-458);
+ * returns the count (<= maxc written).  N = 3 yields the one valid plan. */
+int adatm_plan_candidates(uint32_t nmodes, uint64_t const *dims,
+                          adatm_plan *cands, int maxc);
+
+/* Number of distinct length-(l+1) prefixes of the coordinates under the
+ * level order `order`, for every level l (nfibs[N-1] = nnz).  Runs on the
+ * GPU (radix sort + boundary scan) when device >= 0, else on the host. */
+int adatm_count_fibers(adatm_coo const *t, uint32_t const *order,
+                       uint64_t *nfibs, int device);
+
+/* The analytic planner (decide_parameters_auto, strategy 1 = min pops).
+ * Fills every candidate's pspace/pops, returns the chosen index in *chosen.
+ * ref_quirks = 1 keeps the reference's statistics reuse between candidates
+ * with equal group counts (adatm_bench.c:246-262). */
+int adatm_plan_analytic(adatm_coo const *t, uint64_t rank, int ref_quirks,
+                        double space_limit, int device, adatm_plan *cands,
+                        int maxc, int *ncands, int *chosen);
+
+/* Map each mode to (group, tag): tag 0 = recomputed root, 1 = memo reuse
+ * (mttkrp.c:1958-1974). */
+void adatm_plan_use(adatm_plan const *p, uint32_t nmodes, uint32_t *use_grp,
+                    uint32_t *use_tag);
+
+/* ----------------------------------------------------------------------- *
+ * Device engine: group CSFs, memo buffers, factors, output on one GPU.
+ * ----------------------------------------------------------------------- */
+typedef struct adatm_engine adatm_engine;
+
+int adatm_engine_create(adatm_coo const *t, uint64_t rank,
+                        adatm_plan const *plan, int precision, int device,
+                        adatm_engine **out);
+void adatm_engine_destroy(adatm_engine *e);
+
+/* Factor matrices are exchanged in FP64 row-major (dims[m] x rank). */
+int adatm_engine_set_factor(adatm_engine *e, uint32_t mode, double const *A);
+int adatm_engine_get_factor(adatm_engine *e, uint32_t mode, double *A);
+int adatm_engine_get_lambda(adatm_engine *e, double *lambda);
+int adatm_engine_init_factors(adatm_engine *e, uint64_t seed);
+
+/* One MTTKRP of the memoized sequence for `mode`.  A root (tag 0) mode
+ * recomputes its group's memo; a reuse mode reads the memo left by its
+ * group's root, which must have run since the factors it contracted last
+ * changed (same contract as the reference).  out (dims[mode] x rank, FP64)
+ * may be NULL to leave the result on the device. */
+int adatm_engine_mttkrp(adatm_engine *e, uint32_t mode, double *out);
+
+/* All N MTTKRPs in order with the factors held fixed (splatt bench). */
+int adatm_engine_mttkrp_seq(adatm_engine *e);
+
+/* CP-ALS: niters sweeps (stop early when |dfit| < tol and it > 0).
+ * fits (niters entries, nullable) receives the fit after every sweep.
+ * use_graph = 1 captures one sweep in a hipGraph and replays it. */
+int adatm_engine_cpd(adatm_engine *e, uint32_t niters, double tol,
+                     int use_graph, double *fits, uint32_t *iters_done);
+
+/* Median wall time (ms) over nrep of: what = 0 the MTTKRP sequence,
+ * 1 one full ALS sweep, after nwarm untimed repetitions (hipEvents on the
+ * engine's stream).  per_kernel (nullable, 24 entries) then receives, from
+ * one extra pass with an event pair around every launch, per kernel class
+ * c (0 leaf SpTTM, 1 level Hadamard-reduce, 2 reuse scatter, 3 dense ALS):
+ * [c] total device ms, [8+c] launch count, [16+c] algorithmic bytes. */
+int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
+                      int use_graph, double *ms_median, double *per_kernel);
+
+/* Structure statistics: nfibs per level of each group (n_grp x N, row-major),
+ * algorithmic (compulsory one-touch) bytes and flops of one MTTKRP
+ * sequence, and device bytes held. */
+int adatm_engine_stats(adatm_engine *e, uint64_t *nfibs, double *seq_bytes,
+                       double *seq_flops, uint64_t *device_bytes);
+
+/* Copy a group's device CSF back (u32 structure, FP64 values) for checks.
+ * Arrays sized by adatm_engine_stats' nfibs: fptr[l] has F_l+1 entries for
+ * l < N-1, fids[l] F_l entries for 1 <= l < N (fids[0] is the ordinal). */
+int adatm_engine_get_csf(adatm_engine *e, uint32_t grp, uint32_t **fptr,
+                         uint32_t **fids, double *vals);
+
+/* Last device error string (thread-local), "" when none. */
+char const *adatm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,148 @@
+"""GPU parity of the memoized MTTKRP sequence against the oracle.
+
+Every mode's MTTKRP from the HIP path (through the C ABI) is compared with
+the brute-force COO MTTKRP of the oracle (mttkrp_stream semantics) on the
+same canonical tensor and factors.  Tolerances: relative Frobenius error
+<= 1e-12 for FP64, <= 1e-5 for FP32, <= 1e-2 for BF16 factor rows.
+"""
+import numpy as np
+import pytest
+
+import adatm_amd as A
+from oracle import npref as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {A.FP64: 1e-12, A.FP32: 1e-5, A.BF16ROWS: 2e-2}
+
+
+def _tensor(dims, ndraws, dist, seed, values=A.VALS_U01):
+    spec = A.make_spec(dims, ndraws, dist=dist, zipf_s=1.1, nclusters=50, sigma=3.0,
+                       values=values, seed=seed)
+    t = A.Tensor.synth(spec)
+    t.canonicalize()
+    return t
+
+
+def _factors(t, R, seed=11):
+    return [A.rand_matrix(seed, m, d, R) for m, d in enumerate(t.dims)]
+
+
+CASES = [
+    # dims, ndraws, dist, R
+    ([40, 30, 20], 3000, A.DIST_UNIFORM, 8),
+    ([50, 40, 30, 20], 20000, A.DIST_UNIFORM, 16),
+    ([200, 150, 100, 50, 30], 50000, A.DIST_ZIPF, 32),
+    ([30, 30, 30, 30, 30, 30], 20000, A.DIST_CLUSTER, 10),
+    ([20] * 7, 15000, A.DIST_UNIFORM, 16),
+    ([16] * 8, 30000, A.DIST_CLUSTER, 64),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+def test_every_plan_every_mode(case, prec):
+    dims, nd, dist, R = CASES[case]
+    print(f"case={case} dims={dims} R={R} prec={prec}", flush=True)
+    t = _tensor(dims, nd, dist, seed=100 + case)
+    U = _factors(t, R)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
+    for plan in A.plan_candidates(t.nmodes, t.dims):
+        e = A.Engine(t, R, plan, precision=prec)
+        for m in range(t.nmodes):
+            e.set_factor(m, U[m])
+        for m in range(t.nmodes):
+            M = e.mttkrp(m)
+            err = O.rel_fro(M, ref[m])
+            assert err <= TOL[prec], (plan.groups(t.nmodes), m, err)
+        e.close()
+
+
+def test_bf16_rows():
+    dims, nd, dist, R = CASES[2]
+    t = _tensor(dims, nd, dist, seed=7)
+    U = _factors(t, R)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    plan = A.plan_candidates(t.nmodes, t.dims)[0]
+    e = A.Engine(t, R, plan, precision=A.BF16ROWS)
+    for m in range(t.nmodes):
+        e.set_factor(m, U[m])
+    for m in range(t.nmodes):
+        err = O.rel_fro(e.mttkrp(m), O.coo_mttkrp(inds, vals, U, m))
+        assert err <= TOL[A.BF16ROWS], (m, err)
+
+
+def test_heavy_segments_split():
+    """A few very long fibers (more children than one work item holds) are
+    cut into pieces and summed by the fix-up pass."""
+    rng = np.random.default_rng(3)
+    nnz = 40000
+    inds = np.stack([np.zeros(nnz, np.int64), rng.integers(0, 2, nnz),
+                     rng.integers(0, 4, nnz), rng.integers(0, 5000, nnz)])
+    inds[0, :100] = rng.integers(0, 30, 100)
+    vals = rng.random(nnz)
+    t = A.Tensor.from_arrays(inds, vals, [30, 2, 4, 5000])
+    t.canonicalize()
+    R = 16
+    U = _factors(t, R)
+    ii, vv = t.inds().astype(np.int64), t.vals.copy()
+    for plan in A.plan_candidates(4, t.dims):
+        e = A.Engine(t, R, plan, precision=A.FP64)
+        for m in range(4):
+            e.set_factor(m, U[m])
+        for m in range(4):
+            err = O.rel_fro(e.mttkrp(m), O.coo_mttkrp(ii, vv, U, m))
+            assert err <= 1e-12, (plan.groups(4), m, err)
+
+
+def test_memo_is_overwritten_between_sweeps():
+    """Repeated sequences with changing factors stay exact (the reference
+    accumulates into its memo, mttkrp.c:967/1022, and drifts)."""
+    dims, nd, dist, R = CASES[1]
+    t = _tensor(dims, nd, dist, seed=5)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    plan = A.plan_candidates(4, t.dims)[0]
+    e = A.Engine(t, R, plan, precision=A.FP64)
+    for rep in range(3):
+        U = _factors(t, R, seed=40 + rep)
+        for m in range(4):
+            e.set_factor(m, U[m])
+        for m in range(4):
+            err = O.rel_fro(e.mttkrp(m), O.coo_mttkrp(inds, vals, U, m))
+            assert err <= 1e-12, (rep, m, err)
+
+
+def test_csf_structure_bit_exact():
+    dims, nd, dist, R = CASES[2]
+    t = _tensor(dims, nd, dist, seed=9)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    plan = A.plan_candidates(t.nmodes, t.dims)[-1]
+    e = A.Engine(t, R, plan, precision=A.FP64)
+    for g, grp in enumerate(plan.groups(t.nmodes)):
+        F, fptr, fids, gv = e.get_csf(g, t.nnz)
+        Fo, fpo, fio, vo, _ = O.csf(inds, vals, grp[3])
+        assert F == Fo
+        for l in range(t.nmodes - 1):
+            assert np.array_equal(fptr[l], fpo[l])
+        for l in range(1, t.nmodes):
+            assert np.array_equal(fids[l], fio[l])
+        assert np.array_equal(gv, vo)
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+@pytest.mark.parametrize("graph", [False, True])
+def test_cpd_fit_matches_oracle(prec, graph):
+    dims, nd, dist, R = CASES[1]
+    t = _tensor(dims, nd, dist, seed=21)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    U = _factors(t, R, seed=77)
+    cands, ch = A.plan_analytic(t, R)
+    e = A.Engine(t, R, cands[ch], precision=prec)
+    for m in range(4):
+        e.set_factor(m, U[m])
+    fits = e.cpd(10, tol=0.0, graph=graph)
+    ref, _, _ = O.cpd_als(lambda m, F: O.coo_mttkrp(inds, vals, F, m), U, 10,
+                          float((vals ** 2).sum()))
+    tol = 1e-9 if prec == A.FP64 else 1e-4
+    assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= tol, (fits, ref)
