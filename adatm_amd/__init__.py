@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
     P = C.POINTER
     sig = {
         "adatm_last_error": (C.c_char_p, []),
+        "pair_sort": (None, [P(C.c_uint64), C.c_uint64]),
         "adatm_philox4x32_10": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "adatm_synth_named": (C.c_int, [C.c_int, C.c_uint64, P(SynthSpec)]),
         "adatm_synth": (C.c_int, [P(SynthSpec), P(Coo)]),
@@ -130,6 +131,9 @@ def lib() -> C.CDLL:
                                        P(C.c_double), P(C.c_uint32)]),
         "adatm_engine_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         P(C.c_double), P(C.c_double)]),
+        "adatm_engine_run_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_int,
+                                             P(C.c_double)]),
+        "adatm_engine_get_fits": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_stats": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_double),
                                          P(C.c_double), P(C.c_uint64)]),
         "adatm_engine_get_csf": (C.c_int, [C.c_void_p, C.c_uint32, P(P(C.c_uint32)),
@@ -380,6 +384,17 @@ class Engine:
         check(lib().adatm_engine_time(self.h, what, nwarm, nrep, int(graph),
                                       C.byref(ms), _dptr(pk)), "engine_time")
         return ms.value, pk
+
+    def run_timed(self, what: int, nsteps: int, graph: bool = False) -> float:
+        ms = C.c_double()
+        check(lib().adatm_engine_run_timed(self.h, what, nsteps, int(graph), C.byref(ms)),
+              "run_timed")
+        return ms.value
+
+    def fits(self, n: int) -> np.ndarray:
+        f = np.zeros(n, dtype=np.float64)
+        check(lib().adatm_engine_get_fits(self.h, n, _dptr(f)), "get_fits")
+        return f
 
     def stats(self):
         F = np.zeros(self.plan.n_grp * self.N, dtype=np.uint64)

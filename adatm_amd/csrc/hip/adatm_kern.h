@@ -18,40 +18,58 @@ extern "C" {
 enum { KP_FP64 = 0, KP_FP32 = 1, KP_BF16ROWS = 2 };
 
 #define ADATM_KMAXL 16
+#define ADATM_KMAXFUSE 8 /* levels one k_reduce launch folds */
+#define ADATM_KMAXSCAT 8 /* deepest level k_scatter handles */
 
-/* segmented-reduction work list (see adatm_segwork in the host code) */
+/* Work list of a segmented reduction (see adatm_segwork in the host code):
+ * item k walks bottom positions [start[k], start[k+1]); a light item covers
+ * whole top parents seg[k] .. seg[k+1]-1, a heavy piece (slot[k] != ~0)
+ * part of parent seg[k] and writes carry row slot[k].  anc[k*(K-1)+j-1] is
+ * the level-j ancestor (relative to the top) of the item's first position. */
 typedef struct {
   uint32_t nitems;
   uint32_t nheavy;
   uint32_t const *start; /* nitems+1 */
   uint32_t const *seg;   /* nitems+1 */
   uint32_t const *slot;  /* nitems */
+  uint32_t const *anc;   /* nitems*(K-1), NULL when K == 1 */
   uint32_t const *hseg;  /* nheavy */
   uint32_t const *hslot; /* nheavy+1 */
 } adatm_kwork;
 
-/* out[p] = sum_{c in [ptr[p],ptr[p+1])} in(c) o U[ids[c]]; in(c) is the
- * scalar xin[c] when leaf, else the row xin[c*R ...].  Heavy pieces write
- * carry slots, then the fix-up sums them. */
-int adatm_k_seg_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
-                       uint32_t const *ptr, uint32_t const *ids,
-                       void const *xin, void const *U, void *out, void *carry,
-                       uint32_t R);
+/* K fused CSF levels below a top level (relative level 0):
+ * fptr[j] (j < K) = child pointer of relative level j, ids[j] / U[j]
+ * (1 <= j <= K) = node ids and factor of relative level j. */
+typedef struct {
+  uint32_t K;
+  uint32_t const *orow; /* output row of each top node (NULL: the node) */
+  uint32_t const *fptr[ADATM_KMAXFUSE + 1];
+  uint32_t const *ids[ADATM_KMAXFUSE + 1];
+  void const *U[ADATM_KMAXFUSE + 1];
+} adatm_kchain;
 
-/* Reuse MTTKRP at CSF level d (d >= 1) over level-d nodes sorted by id:
- * out[i] = sum_{q in [rowptr[i],rowptr[i+1])} Y(par_d[n]) o Z[n], n=perm[q],
- * Y(p) = prod_{l<d} U_l[id_l(ancestor_l)], id_0 = node ordinal. */
+/* Z_top(p) = sum over bottom descendants c of p of in(c) o U_K[ids_K[c]],
+ * folded level by level with U_j[ids_j[...]] of the intermediate nodes; in(c)
+ * is the scalar xin[c] when leaf, else the row xin[c*R ...].  R <= 64. */
+int adatm_k_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
+                   adatm_kchain const *ch, void const *xin, void *out,
+                   void *carry, uint32_t R);
+
+/* Reuse MTTKRP at CSF level d over level-d nodes sorted by output row:
+ * out[i] = sum_{q in [rowptr[i], rowptr[i+1])} Y_q o Z[node[q]],
+ * Y_q = U[0][rid[0][q]] o ... o U[d-1][rid[d-1][q]].  With zscalar, Z holds
+ * one value per node (the leaf level: Z = the nonzero values). */
 typedef struct {
   uint32_t d;
-  uint32_t const *par[ADATM_KMAXL]; /* par[l]: level l -> l-1, l = 1..d */
-  uint32_t const *ids[ADATM_KMAXL]; /* ids[l], l = 1..d-1 */
-  void const *U[ADATM_KMAXL];       /* U[l] = factor of the level-l mode */
-} adatm_kpath;
+  uint32_t const *node;
+  uint32_t const *rid[ADATM_KMAXSCAT];
+  void const *U[ADATM_KMAXSCAT];
+} adatm_kscat;
 
 int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
-                    uint32_t const *rowptr, uint32_t const *perm,
-                    adatm_kpath const *path,
-                    void const *Z, void *out, void *carry, uint32_t R);
+                    uint32_t const *rowptr, adatm_kscat const *sc,
+                    void const *Z, int zscalar, void *out, void *carry,
+                    uint32_t R);
 
 /* zero n bytes with a kernel (graph-capture safe) */
 int adatm_k_zero(hipStream_t s, void *p, size_t nbytes);
@@ -85,6 +103,52 @@ int adatm_k_convert_in(hipStream_t s, int prec, double const *src64,
                        void *dst, void *dst_bf, uint64_t n);
 int adatm_k_convert_out(hipStream_t s, int prec, void const *src,
                         double *dst64, uint64_t n);
+
+/* ---- construction (construct.hip) ---- */
+uint64_t adatm_k_scan_temp_words(uint64_t n);
+int adatm_k_exscan(hipStream_t s, uint32_t const *in, uint32_t *out, uint64_t n,
+                   uint32_t *temp, uint32_t *total);
+uint64_t adatm_k_sort_temp_words(uint64_t n);
+int adatm_k_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint64_t n,
+                       uint32_t nbits, uint32_t *temp);
+int adatm_k_iota(hipStream_t s, uint32_t *p, uint64_t n);
+int adatm_k_gather(hipStream_t s, uint32_t *dst, uint32_t const *src,
+                   uint32_t const *idx, uint64_t n);
+int adatm_k_gather_vals(hipStream_t s, int prec, void *dst, double const *src,
+                        uint32_t const *idx, uint64_t n);
+int adatm_k_first_diff(hipStream_t s, uint32_t const *const *sc, uint32_t N,
+                       uint64_t n, uint8_t *fd);
+int adatm_k_level_flags(hipStream_t s, uint8_t const *fd, uint64_t n, uint32_t l,
+                        uint32_t *flags);
+int adatm_k_level_emit(hipStream_t s, uint32_t const *flags, uint32_t const *pos,
+                       uint32_t const *scl, uint64_t n, uint32_t *fids,
+                       uint32_t *start);
+int adatm_k_fptr(hipStream_t s, uint32_t const *start, uint64_t F,
+                 uint32_t const *pos_next, uint32_t F_next, uint32_t *fptr);
+int adatm_k_parent(hipStream_t s, uint32_t const *start_next, uint64_t Fn,
+                   uint32_t const *flags, uint32_t const *pos, uint32_t *par);
+int adatm_k_compose(hipStream_t s, uint32_t *cp, uint64_t n1, uint32_t const *fp);
+int adatm_k_items_count(hipStream_t s, uint32_t const *cp, uint64_t nseg,
+                        uint32_t tile, uint32_t *nitem, uint32_t *npiece,
+                        uint32_t *heavy);
+int adatm_k_items_emit(hipStream_t s, uint32_t const *cp, uint64_t nseg,
+                       uint32_t tile, uint32_t const *nitem,
+                       uint32_t const *ibase, uint32_t const *sbase,
+                       uint32_t const *heavy, uint32_t const *hbase,
+                       uint32_t *start, uint32_t *seg, uint32_t *slot,
+                       uint32_t *hseg, uint32_t *hslot);
+int adatm_k_items_tail(hipStream_t s, uint32_t const *cp, uint64_t nseg,
+                       uint32_t nitems, uint32_t nheavy, uint32_t nslots,
+                       uint32_t *start, uint32_t *seg, uint32_t *slot,
+                       uint32_t *hslot);
+int adatm_k_items_anc(hipStream_t s, uint32_t const *start, uint32_t nitems,
+                      uint32_t lb, uint32_t lt, uint32_t const *const *par,
+                      uint32_t *anc);
+int adatm_k_rowptr(hipStream_t s, uint32_t const *skey, uint64_t n,
+                   uint32_t nrows, uint32_t *rowptr);
+int adatm_k_scatter_rids(hipStream_t s, uint32_t const *node, uint64_t n,
+                         uint32_t d, uint32_t const *const *par,
+                         uint32_t const *const *fids, uint32_t *const *rid);
 
 #ifdef __cplusplus
 }

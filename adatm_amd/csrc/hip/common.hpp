@@ -1,0 +1,34 @@
+/*
+ * Device helpers shared by the HIP translation units (gfx950, wave64).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NONE32 0xFFFFFFFFu
+
+namespace {
+
+/* bfloat16 factor-row storage: raw upper half of an FP32 */
+struct bf16raw {
+  uint16_t v;
+};
+
+__device__ __forceinline__ float ldu(float const *p) { return *p; }
+__device__ __forceinline__ double ldu(double const *p) { return *p; }
+__device__ __forceinline__ float ldu(bf16raw const *p)
+{
+  return __uint_as_float(((uint32_t)p->v) << 16);
+}
+
+/* FP32 -> BF16 with round-to-nearest-even (NaN/Inf keep their class) */
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+} // namespace

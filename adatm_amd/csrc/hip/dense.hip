@@ -1,301 +1,17 @@
 /*
- * HIP kernels of the memoized MTTKRP / CP-ALS path for MI355X (gfx950).
- *
- * Wave64 throughout.  A "lane group" of G = min(R, 64) lanes owns one work
- * item; lane j of the group holds columns j, j+64, ... of the R-wide rows,
- * so every factor-row gather and every memo-row read of a group is one
- * contiguous R*sizeof(T)-byte access.
- *
- *   k_seg_reduce  -- leaf SpTTM (p_spttm, mttkrp.c:891-928) and level
- *                    Hadamard-reduce (p_spt2t_add_hada / p_spt2m_add_hada,
- *                    mttkrp.c:828-888) as one segmented reduction over the
- *                    children of a CSF level.  Work items cover whole
- *                    parents, so outputs are stored, never accumulated; a
- *                    parent with more children than one item holds is cut
- *                    into pieces whose partial rows go to carry slots and
- *                    are summed in piece order by k_fixup (deterministic,
- *                    no atomics, no zero-fill).
- *   k_scatter     -- memo-reuse MTTKRP of a non-root level
- *                    (p_csf_mttkrp_internal_reuse_adaptive,
- *                    mttkrp.c:1547-1682): the reference scatters
- *                    Y(parent) o Z(node) into output rows under 1024 striped
- *                    locks; here level nodes are pre-sorted by output row,
- *                    so each output row is a segment of the same reduction.
- *   dense steps   -- Gram, Hadamard-of-Grams inverse, A = M V^-1, column
- *                    normalisation, fit (adatm_cpd.c:276-340 and the SPLATT
- *                    matrix routines it calls) in FP64 arithmetic.
+ * Dense CP-ALS steps of the sweep on MI355X (gfx950): Gram matrices,
+ * (hadamard of Grams)^-1, A = M V^-1 with column normalisation, fit, and the
+ * FP64 <-> path-precision conversions.  FP64 arithmetic throughout
+ * (cpd_als_iterate_adaptive, src/adatm_cpd.c:276-340, and the SPLATT matrix
+ * routines it calls: mat_aTa, calc_gram_inv, mat_matmul, mat_normalize).
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "adatm_kern.h"
-
-#define NONE32 0xFFFFFFFFu
+#include "common.hpp"
 
 namespace {
-
-/* ---- element loads/stores per storage type ----------------------------- */
-struct bf16raw {
-  uint16_t v;
-};
-
-__device__ __forceinline__ float ldu(float const *p) { return *p; }
-__device__ __forceinline__ double ldu(double const *p) { return *p; }
-__device__ __forceinline__ float ldu(bf16raw const *p)
-{
-  return __uint_as_float(((uint32_t)p->v) << 16);
-}
-
-__device__ __forceinline__ uint16_t to_bf16(float f)
-{
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16); /* inf/nan */
-  u += 0x7fffu + ((u >> 16) & 1u); /* round to nearest even */
-  return (uint16_t)(u >> 16);
-}
-
-/* ---- segmented Hadamard-reduce ----------------------------------------- */
-template <typename T, typename TU, bool LEAF, int CPL>
-__device__ __forceinline__ void acc_child(T (&acc)[CPL], uint32_t c,
-                                          uint32_t const *__restrict__ ids,
-                                          T const *__restrict__ xin,
-                                          TU const *__restrict__ U,
-                                          uint32_t R, uint32_t col0)
-{
-  uint32_t const id = ids[c];
-  TU const *urow = U + (size_t)id * R;
-  if (LEAF) {
-    T const x = xin[c];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) acc[j] += x * (T)ldu(urow + col);
-    }
-  } else {
-    T const *zrow = xin + (size_t)c * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) acc[j] += zrow[col] * (T)ldu(urow + col);
-    }
-  }
-}
-
-/* children [c, e) accumulated in order; four children's loads in flight */
-template <typename T, typename TU, bool LEAF, int CPL>
-__device__ __forceinline__ void acc_range(T (&acc)[CPL], uint32_t c, uint32_t e,
-                                          uint32_t const *__restrict__ ids,
-                                          T const *__restrict__ xin,
-                                          TU const *__restrict__ U, uint32_t R,
-                                          uint32_t col0)
-{
-  for (; c + 4 <= e; c += 4) {
-    uint32_t const i0 = ids[c], i1 = ids[c + 1], i2 = ids[c + 2],
-                   i3 = ids[c + 3];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) {
-        T const u0 = (T)ldu(U + (size_t)i0 * R + col);
-        T const u1 = (T)ldu(U + (size_t)i1 * R + col);
-        T const u2 = (T)ldu(U + (size_t)i2 * R + col);
-        T const u3 = (T)ldu(U + (size_t)i3 * R + col);
-        T x0, x1, x2, x3;
-        if (LEAF) {
-          x0 = xin[c];
-          x1 = xin[c + 1];
-          x2 = xin[c + 2];
-          x3 = xin[c + 3];
-        } else {
-          x0 = xin[(size_t)c * R + col];
-          x1 = xin[(size_t)(c + 1) * R + col];
-          x2 = xin[(size_t)(c + 2) * R + col];
-          x3 = xin[(size_t)(c + 3) * R + col];
-        }
-        T a = acc[j];
-        a += x0 * u0;
-        a += x1 * u1;
-        a += x2 * u2;
-        a += x3 * u3;
-        acc[j] = a;
-      }
-    }
-  }
-  for (; c < e; ++c)
-    acc_child<T, TU, LEAF, CPL>(acc, c, ids, xin, U, R, col0);
-}
-
-template <typename T, typename TU, bool LEAF, int CPL>
-__global__ __launch_bounds__(256) void k_seg_reduce(
-    uint32_t nitems, uint32_t const *__restrict__ istart,
-    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
-    uint32_t const *__restrict__ ptr, uint32_t const *__restrict__ ids,
-    T const *__restrict__ xin, TU const *__restrict__ U, T *__restrict__ out,
-    T *__restrict__ carry, uint32_t R, uint32_t G, uint32_t gpw)
-{
-  uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / G;
-  if (grp >= gpw) return;
-  uint32_t const col0 = lane - grp * G;
-  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  uint32_t const k = wave * gpw + grp;
-  if (k >= nitems) return;
-
-  uint32_t const a = istart[k];
-  uint32_t const slot = islot[k];
-  T acc[CPL];
-  if (slot != NONE32) {
-    uint32_t const b = istart[k + 1];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-    acc_range<T, TU, LEAF, CPL>(acc, a, b, ids, xin, U, R, col0);
-    T *crow = carry + (size_t)slot * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) crow[col] = acc[j];
-    }
-    return;
-  }
-  uint32_t p = iseg[k];
-  uint32_t const pe = iseg[k + 1];
-  uint32_t c = a;
-  for (; p < pe; ++p) {
-    uint32_t const e = ptr[p + 1];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-    acc_range<T, TU, LEAF, CPL>(acc, c, e, ids, xin, U, R, col0);
-    c = e;
-    T *orow = out + (size_t)p * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) orow[col] = acc[j];
-    }
-  }
-}
-
-/* out[hseg[h]] = sum of the carry rows of its pieces, in piece order */
-template <typename T, int CPL>
-__global__ __launch_bounds__(256) void k_fixup(
-    uint32_t nheavy, uint32_t const *__restrict__ hseg,
-    uint32_t const *__restrict__ hslot, T const *__restrict__ carry,
-    T *__restrict__ out, uint32_t R, uint32_t G, uint32_t gpw)
-{
-  uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / G;
-  if (grp >= gpw) return;
-  uint32_t const col0 = lane - grp * G;
-  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  uint32_t const h = wave * gpw + grp;
-  if (h >= nheavy) return;
-  T acc[CPL];
-#pragma unroll
-  for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-  for (uint32_t s = hslot[h]; s < hslot[h + 1]; ++s) {
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) acc[j] += carry[(size_t)s * R + col];
-    }
-  }
-  T *orow = out + (size_t)hseg[h] * R;
-#pragma unroll
-  for (int j = 0; j < CPL; ++j) {
-    uint32_t const col = col0 + 64u * j;
-    if (col < R) orow[col] = acc[j];
-  }
-}
-
-/* ---- reuse scatter over id-sorted level nodes -------------------------- */
-template <typename T, typename TU, int CPL>
-__device__ __forceinline__ void scatter_node(T (&acc)[CPL], uint32_t n,
-                                             adatm_kpath const &path,
-                                             T const *__restrict__ Z,
-                                             uint32_t R, uint32_t col0)
-{
-  uint32_t const d = path.d;
-  /* ancestors: anc[l] = node at level l on the path to n, l < d */
-  uint32_t anc[ADATM_KMAXL];
-  uint32_t p = path.par[d][n];
-  anc[d - 1] = p;
-  for (uint32_t l = d - 1; l >= 1; --l) {
-    p = path.par[l][p];
-    anc[l - 1] = p;
-  }
-  T y[CPL];
-  {
-    TU const *u0 = (TU const *)path.U[0] + (size_t)anc[0] * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      y[j] = (col < R) ? (T)ldu(u0 + col) : (T)0;
-    }
-  }
-  for (uint32_t l = 1; l < d; ++l) {
-    uint32_t const id = path.ids[l][anc[l]];
-    TU const *ul = (TU const *)path.U[l] + (size_t)id * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) y[j] = y[j] * (T)ldu(ul + col);
-    }
-  }
-  T const *zrow = Z + (size_t)n * R;
-#pragma unroll
-  for (int j = 0; j < CPL; ++j) {
-    uint32_t const col = col0 + 64u * j;
-    if (col < R) acc[j] += zrow[col] * y[j];
-  }
-}
-
-template <typename T, typename TU, int CPL>
-__global__ __launch_bounds__(256) void k_scatter(
-    uint32_t nitems, uint32_t const *__restrict__ istart,
-    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
-    uint32_t const *__restrict__ rowptr, uint32_t const *__restrict__ perm,
-    adatm_kpath path, T const *__restrict__ Z, T *__restrict__ out,
-    T *__restrict__ carry, uint32_t R, uint32_t G, uint32_t gpw)
-{
-  uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / G;
-  if (grp >= gpw) return;
-  uint32_t const col0 = lane - grp * G;
-  uint32_t const wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  uint32_t const k = wave * gpw + grp;
-  if (k >= nitems) return;
-  uint32_t const a = istart[k];
-  uint32_t const slot = islot[k];
-  T acc[CPL];
-  if (slot != NONE32) {
-    uint32_t const b = istart[k + 1];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-    for (uint32_t q = a; q < b; ++q)
-      scatter_node<T, TU, CPL>(acc, perm[q], path, Z, R, col0);
-    T *crow = carry + (size_t)slot * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) crow[col] = acc[j];
-    }
-    return;
-  }
-  uint32_t const pe = iseg[k + 1];
-  uint32_t q = a;
-  for (uint32_t p = iseg[k]; p < pe; ++p) {
-    uint32_t const e = rowptr[p + 1];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = (T)0;
-    for (; q < e; ++q) scatter_node<T, TU, CPL>(acc, perm[q], path, Z, R, col0);
-    T *orow = out + (size_t)p * R;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      uint32_t const col = col0 + 64u * j;
-      if (col < R) orow[col] = acc[j];
-    }
-  }
-}
 
 /* ---- misc --------------------------------------------------------------- */
 __global__ void k_zero_words(uint32_t *p, size_t nwords)
@@ -609,119 +325,6 @@ __global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ par
   }
 }
 
-/* ---- launch helpers ----------------------------------------------------- */
-void group_shape(uint32_t R, uint32_t *G, uint32_t *gpw, int *cpl)
-{
-  *G = R < 64 ? R : 64;
-  *gpw = 64 / *G;
-  uint32_t const c = (R + 63) / 64;
-  *cpl = c <= 1 ? 1 : (c <= 2 ? 2 : 4);
-}
-
-template <typename T>
-hipError_t launch_fix(hipStream_t s, adatm_kwork const *w, T const *carry,
-                      T *out, uint32_t R)
-{
-  if (w->nheavy == 0) return hipSuccess;
-  uint32_t G, gpw;
-  int cpl;
-  group_shape(R, &G, &gpw, &cpl);
-  uint32_t const per_block = 4 * gpw;
-  dim3 const grid((w->nheavy + per_block - 1) / per_block);
-  switch (cpl) {
-  case 1:
-    hipLaunchKernelGGL((k_fixup<T, 1>), grid, dim3(256), 0, s, w->nheavy,
-                       w->hseg, w->hslot, carry, out, R, G, gpw);
-    break;
-  case 2:
-    hipLaunchKernelGGL((k_fixup<T, 2>), grid, dim3(256), 0, s, w->nheavy,
-                       w->hseg, w->hslot, carry, out, R, G, gpw);
-    break;
-  default:
-    hipLaunchKernelGGL((k_fixup<T, 4>), grid, dim3(256), 0, s, w->nheavy,
-                       w->hseg, w->hslot, carry, out, R, G, gpw);
-    break;
-  }
-  return hipGetLastError();
-}
-
-template <typename T, typename TU, bool LEAF>
-hipError_t launch_reduce(hipStream_t s, adatm_kwork const *w,
-                         uint32_t const *ptr, uint32_t const *ids,
-                         void const *xin, void const *U, void *out, void *carry,
-                         uint32_t R)
-{
-  uint32_t G, gpw;
-  int cpl;
-  group_shape(R, &G, &gpw, &cpl);
-  if (w->nitems > 0) {
-    uint32_t const per_block = 4 * gpw;
-    dim3 const grid((w->nitems + per_block - 1) / per_block);
-    T const *x = (T const *)xin;
-    TU const *u = (TU const *)U;
-    T *o = (T *)out;
-    T *cy = (T *)carry;
-    switch (cpl) {
-    case 1:
-      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 1>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
-                         o, cy, R, G, gpw);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 2>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
-                         o, cy, R, G, gpw);
-      break;
-    default:
-      hipLaunchKernelGGL((k_seg_reduce<T, TU, LEAF, 4>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, ptr, ids, x, u,
-                         o, cy, R, G, gpw);
-      break;
-    }
-    hipError_t const e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return launch_fix<T>(s, w, (T const *)carry, (T *)out, R);
-}
-
-template <typename T, typename TU>
-hipError_t launch_scatter(hipStream_t s, adatm_kwork const *w,
-                          uint32_t const *rowptr, uint32_t const *perm,
-                          adatm_kpath const *path, void const *Z, void *out,
-                          void *carry, uint32_t R)
-{
-  uint32_t G, gpw;
-  int cpl;
-  group_shape(R, &G, &gpw, &cpl);
-  if (w->nitems > 0) {
-    uint32_t const per_block = 4 * gpw;
-    dim3 const grid((w->nitems + per_block - 1) / per_block);
-    T const *z = (T const *)Z;
-    T *o = (T *)out;
-    T *cy = (T *)carry;
-    switch (cpl) {
-    case 1:
-      hipLaunchKernelGGL((k_scatter<T, TU, 1>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
-                         *path, z, o, cy, R, G, gpw);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_scatter<T, TU, 2>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
-                         *path, z, o, cy, R, G, gpw);
-      break;
-    default:
-      hipLaunchKernelGGL((k_scatter<T, TU, 4>), grid, dim3(256), 0, s,
-                         w->nitems, w->start, w->seg, w->slot, rowptr, perm,
-                         *path, z, o, cy, R, G, gpw);
-      break;
-    }
-    hipError_t const e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return launch_fix<T>(s, w, (T const *)carry, (T *)out, R);
-}
-
 uint32_t dense_blocks(uint64_t I)
 {
   uint64_t const b = (I + 255) / 256;
@@ -750,40 +353,6 @@ hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
 extern "C" {
 
 static int herr(hipError_t e) { return e == hipSuccess ? 0 : (int)e; }
-
-int adatm_k_seg_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
-                       uint32_t const *ptr, uint32_t const *ids,
-                       void const *xin, void const *U, void *out, void *carry,
-                       uint32_t R)
-{
-  hipError_t e;
-  if (prec == KP_FP64)
-    e = leaf ? launch_reduce<double, double, true>(s, w, ptr, ids, xin, U, out, carry, R)
-             : launch_reduce<double, double, false>(s, w, ptr, ids, xin, U, out, carry, R);
-  else if (prec == KP_FP32)
-    e = leaf ? launch_reduce<float, float, true>(s, w, ptr, ids, xin, U, out, carry, R)
-             : launch_reduce<float, float, false>(s, w, ptr, ids, xin, U, out, carry, R);
-  else
-    e = leaf ? launch_reduce<float, bf16raw, true>(s, w, ptr, ids, xin, U, out, carry, R)
-             : launch_reduce<float, bf16raw, false>(s, w, ptr, ids, xin, U, out, carry, R);
-  return herr(e);
-}
-
-int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
-                     uint32_t const *rowptr, uint32_t const *perm,
-                     adatm_kpath const *path, void const *Z, void *out,
-                     void *carry, uint32_t R)
-{
-  if (path->d < 1 || path->d >= ADATM_KMAXL) return (int)hipErrorInvalidValue;
-  hipError_t e;
-  if (prec == KP_FP64)
-    e = launch_scatter<double, double>(s, w, rowptr, perm, path, Z, out, carry, R);
-  else if (prec == KP_FP32)
-    e = launch_scatter<float, float>(s, w, rowptr, perm, path, Z, out, carry, R);
-  else
-    e = launch_scatter<float, bf16raw>(s, w, rowptr, perm, path, Z, out, carry, R);
-  return herr(e);
-}
 
 int adatm_k_zero(hipStream_t s, void *p, size_t nbytes)
 {

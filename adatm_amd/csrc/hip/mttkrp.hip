@@ -1,0 +1,349 @@
+/*
+ * Memoized-MTTKRP kernels for MI355X (gfx950, wave64).
+ *
+ * A lane group of G = R lanes (R <= 64; 64/R groups per wavefront) owns one
+ * work item and lane j holds column j of every R-wide row, so each factor-row
+ * gather and each memo-row access of a group is one contiguous R*sizeof(T)
+ * transaction.  Work items are balanced by the number of children they walk;
+ * a parent longer than one item is cut into pieces whose partial rows go to
+ * carry slots, summed in a fixed order by k_fixup, so results are
+ * deterministic and no output needs zero-filling or atomics.
+ *
+ *   k_reduce<K>  Z_top(p) = sum over the level-`top+K` descendants c of p of
+ *                in(c) o U_K[id(c)], folded up level by level with the
+ *                factor rows of the intermediate nodes.  K = 1 is the leaf
+ *                SpTTM (p_spttm, src/mttkrp.c:891-928) or one Hadamard-reduce
+ *                (p_spt2t_add_hada / p_spt2m_add_hada, src/mttkrp.c:828-888);
+ *                K > 1 fuses K such passes so that the levels in between --
+ *                the reference's temporaries tmp_vals / tmp_vals_2,
+ *                src/mttkrp.c:991-1066 -- never reach HBM.  Per-level running
+ *                sums live in registers, as p_propagate_up keeps them in
+ *                per-level buffers (src/mttkrp.c:241-304).
+ *   k_scatter<D> reuse MTTKRP of CSF level D (p_csf_mttkrp_internal_reuse_
+ *                adaptive, src/mttkrp.c:1547-1682): out[i] = sum over level-D
+ *                nodes n with id i of Y(n) o Z_D(n), Y(n) = product of the
+ *                factor rows of n's ancestors.  The reference scatters under
+ *                1024 striped locks; here nodes are pre-sorted by output row
+ *                with their ancestors' row ids, so every output row is one
+ *                segment of a segmented reduction.
+ *   k_fixup      one workgroup per split parent: tree sum of its carry rows.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "adatm_kern.h"
+#include "common.hpp"
+
+#define PF 4 /* children whose loads are issued together */
+
+namespace {
+
+template <typename T, typename TU, bool LEAF, int K>
+__global__ __launch_bounds__(256) void k_reduce(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ ianc, adatm_kchain ch,
+    T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
+    uint32_t R, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / R;
+  if (grp >= gpw) return;
+  uint32_t const col = lane - grp * R;
+  uint32_t const k =
+      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+  if (k >= nitems) return;
+
+  uint32_t const a = istart[k], b = istart[k + 1];
+  uint32_t const slot = islot[k];
+  uint32_t node[K], endc[K];
+  node[0] = iseg[k];
+#pragma unroll
+  for (int j = 1; j < K; ++j) node[j] = ianc[(size_t)k * (K - 1) + (j - 1)];
+#pragma unroll
+  for (int j = 0; j < K; ++j) endc[j] = ch.fptr[j][node[j] + 1];
+  T acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = (T)0;
+  uint32_t const *__restrict__ idb = ch.ids[K];
+  TU const *__restrict__ Ub = (TU const *)ch.U[K];
+
+  for (uint32_t c0 = a; c0 < b; c0 += PF) {
+    T xv[PF], uv[PF];
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      uint32_t const c = (c0 + i < b) ? c0 + i : b - 1;
+      uint32_t const id = idb[c];
+      uv[i] = (T)ldu(Ub + (size_t)id * R + col);
+      xv[i] = LEAF ? xin[c] : xin[(size_t)c * R + col];
+    }
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      uint32_t const c = c0 + i;
+      if (c < b) {
+        acc[K - 1] += xv[i] * uv[i];
+        bool const last = (c + 1 == b);
+        uint32_t nxt = c + 1;
+        bool go = true;
+#pragma unroll
+        for (int j = K - 1; j >= 0; --j) {
+          if (go && (last || nxt == endc[j])) {
+            if (j > 0) {
+              /* node[j] complete (or the item ends inside it): fold it into
+                 its parent with its own factor row */
+              uint32_t const nid = ch.ids[j][node[j]];
+              T const u = (T)ldu((TU const *)ch.U[j] + (size_t)nid * R + col);
+              acc[j - 1] += acc[j] * u;
+              acc[j] = (T)0;
+              nxt = ++node[j];
+              if (!last) endc[j] = ch.fptr[j][nxt + 1];
+            } else {
+              T *dst = (slot != NONE32)
+                           ? carry + (size_t)slot * R
+                           : out + (size_t)(ch.orow ? ch.orow[node[0]] : node[0]) * R;
+              dst[col] = acc[0];
+              acc[0] = (T)0;
+              ++node[0];
+              if (!last) endc[0] = ch.fptr[0][node[0] + 1];
+            }
+          } else {
+            go = false;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T, typename TU, int D, bool ZS>
+__device__ __forceinline__ T scatter_term(uint32_t q, adatm_kscat const &sc,
+                                          T const *__restrict__ Z, uint32_t R,
+                                          uint32_t col)
+{
+  T y = (T)ldu((TU const *)sc.U[0] + (size_t)sc.rid[0][q] * R + col);
+#pragma unroll
+  for (int l = 1; l < D; ++l)
+    y = y * (T)ldu((TU const *)sc.U[l] + (size_t)sc.rid[l][q] * R + col);
+  uint32_t const n = sc.node[q];
+  return (ZS ? Z[n] : Z[(size_t)n * R + col]) * y;
+}
+
+template <typename T, typename TU, int D, bool ZS>
+__global__ __launch_bounds__(256) void k_scatter(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ rowptr, adatm_kscat sc,
+    T const *__restrict__ Z, T *__restrict__ out, T *__restrict__ carry,
+    uint32_t R, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / R;
+  if (grp >= gpw) return;
+  uint32_t const col = lane - grp * R;
+  uint32_t const k =
+      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+  if (k >= nitems) return;
+  uint32_t const a = istart[k], b = istart[k + 1];
+  uint32_t const slot = islot[k];
+  if (slot != NONE32) {
+    T acc = (T)0;
+    uint32_t q = a;
+    for (; q + PF <= b; q += PF) {
+      T v[PF];
+#pragma unroll
+      for (int i = 0; i < PF; ++i) v[i] = scatter_term<T, TU, D, ZS>(q + i, sc, Z, R, col);
+#pragma unroll
+      for (int i = 0; i < PF; ++i) acc += v[i];
+    }
+    for (; q < b; ++q) acc += scatter_term<T, TU, D, ZS>(q, sc, Z, R, col);
+    carry[(size_t)slot * R + col] = acc;
+    return;
+  }
+  /* light item: whole rows iseg[k] .. iseg[k+1]-1, positions [a, b) */
+  uint32_t p = iseg[k];
+  uint32_t const pe = iseg[k + 1];
+  uint32_t e = rowptr[p + 1];
+  while (e == a) { /* leading empty rows */
+    out[(size_t)p * R + col] = (T)0;
+    if (++p == pe) return;
+    e = rowptr[p + 1];
+  }
+  T acc = (T)0;
+  for (uint32_t c0 = a; c0 < b; c0 += PF) {
+    T v[PF];
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      uint32_t const c = (c0 + i < b) ? c0 + i : b - 1;
+      v[i] = scatter_term<T, TU, D, ZS>(c, sc, Z, R, col);
+    }
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      uint32_t const c = c0 + i;
+      if (c < b) {
+        acc += v[i];
+        if (c + 1 == e) {
+          out[(size_t)p * R + col] = acc;
+          acc = (T)0;
+          /* next non-empty row; empty rows in between get zeros */
+          while (++p < pe) {
+            e = rowptr[p + 1];
+            if (e != c + 1) break;
+            out[(size_t)p * R + col] = (T)0;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* out[hseg[h]] = sum of carry rows [hslot[h], hslot[h+1]) in a fixed tree
+ * order; one workgroup per split parent */
+template <typename T>
+__global__ __launch_bounds__(256) void k_fixup(uint32_t const *__restrict__ hseg,
+                                               uint32_t const *__restrict__ hslot,
+                                               T const *__restrict__ carry,
+                                               T *__restrict__ out, uint32_t R)
+{
+  __shared__ T red[256];
+  uint32_t const h = blockIdx.x;
+  uint32_t const rpi = 256u / R;
+  uint32_t const sl = threadIdx.x / R;
+  uint32_t const col = threadIdx.x - sl * R;
+  T acc = (T)0;
+  if (sl < rpi)
+    for (uint32_t s = hslot[h] + sl; s < hslot[h + 1]; s += rpi)
+      acc += carry[(size_t)s * R + col];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    T t = (T)0;
+    for (uint32_t q = 0; q < rpi; ++q) t += red[q * R + threadIdx.x];
+    out[(size_t)hseg[h] * R + threadIdx.x] = t;
+  }
+}
+
+/* ---- launchers ----------------------------------------------------------- */
+template <typename T>
+hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
+                        void *out, uint32_t R)
+{
+  if (w->nheavy == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_fixup<T>), dim3(w->nheavy), dim3(256), 0, s, w->hseg,
+                     w->hslot, (T const *)carry, (T *)out, R);
+  return hipGetLastError();
+}
+
+template <typename T, typename TU, bool LEAF, int K>
+void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
+               void const *xin, void *out, void *carry, uint32_t R)
+{
+  uint32_t const gpw = 64u / R;
+  uint32_t const per_block = 4u * gpw;
+  dim3 const grid((w->nitems + per_block - 1) / per_block);
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K>), grid, dim3(256), 0, s,
+                     w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+                     (T const *)xin, (T *)out, (T *)carry, R, gpw);
+}
+
+template <typename T, typename TU, bool LEAF>
+hipError_t reduce_k(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
+                    void const *xin, void *out, void *carry, uint32_t R)
+{
+  if (w->nitems > 0) {
+    switch (ch->K) {
+    case 1: go_reduce<T, TU, LEAF, 1>(s, w, ch, xin, out, carry, R); break;
+    case 2: go_reduce<T, TU, LEAF, 2>(s, w, ch, xin, out, carry, R); break;
+    case 3: go_reduce<T, TU, LEAF, 3>(s, w, ch, xin, out, carry, R); break;
+    case 4: go_reduce<T, TU, LEAF, 4>(s, w, ch, xin, out, carry, R); break;
+    case 5: go_reduce<T, TU, LEAF, 5>(s, w, ch, xin, out, carry, R); break;
+    case 6: go_reduce<T, TU, LEAF, 6>(s, w, ch, xin, out, carry, R); break;
+    case 7: go_reduce<T, TU, LEAF, 7>(s, w, ch, xin, out, carry, R); break;
+    default: go_reduce<T, TU, LEAF, 8>(s, w, ch, xin, out, carry, R); break;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_fixup<T>(s, w, carry, out, R);
+}
+
+template <typename T, typename TU, int D>
+void go_scatter(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
+                adatm_kscat const *sc, void const *Z, int zs, void *out,
+                void *carry, uint32_t R)
+{
+  uint32_t const gpw = 64u / R;
+  uint32_t const per_block = 4u * gpw;
+  dim3 const grid((w->nitems + per_block - 1) / per_block);
+  if (zs)
+    hipLaunchKernelGGL((k_scatter<T, TU, D, true>), grid, dim3(256), 0, s,
+                       w->nitems, w->start, w->seg, w->slot, rowptr, *sc,
+                       (T const *)Z, (T *)out, (T *)carry, R, gpw);
+  else
+    hipLaunchKernelGGL((k_scatter<T, TU, D, false>), grid, dim3(256), 0, s, w->nitems,
+                     w->start, w->seg, w->slot, rowptr, *sc, (T const *)Z,
+                     (T *)out, (T *)carry, R, gpw);
+}
+
+template <typename T, typename TU>
+hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
+                     adatm_kscat const *sc, void const *Z, int zs, void *out,
+                     void *carry, uint32_t R)
+{
+  if (w->nitems > 0) {
+    switch (sc->d) {
+    case 1: go_scatter<T, TU, 1>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 2: go_scatter<T, TU, 2>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 3: go_scatter<T, TU, 3>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 4: go_scatter<T, TU, 4>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 5: go_scatter<T, TU, 5>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 6: go_scatter<T, TU, 6>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 7: go_scatter<T, TU, 7>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    default: go_scatter<T, TU, 8>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return launch_fixup<T>(s, w, carry, out, R);
+}
+
+} // namespace
+
+extern "C" {
+
+int adatm_k_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
+                   adatm_kchain const *ch, void const *xin, void *out,
+                   void *carry, uint32_t R)
+{
+  if (ch->K < 1 || ch->K > ADATM_KMAXFUSE || R < 1 || R > 64)
+    return (int)hipErrorInvalidValue;
+  hipError_t e;
+  if (prec == KP_FP64)
+    e = leaf ? reduce_k<double, double, true>(s, w, ch, xin, out, carry, R)
+             : reduce_k<double, double, false>(s, w, ch, xin, out, carry, R);
+  else if (prec == KP_FP32)
+    e = leaf ? reduce_k<float, float, true>(s, w, ch, xin, out, carry, R)
+             : reduce_k<float, float, false>(s, w, ch, xin, out, carry, R);
+  else
+    e = leaf ? reduce_k<float, bf16raw, true>(s, w, ch, xin, out, carry, R)
+             : reduce_k<float, bf16raw, false>(s, w, ch, xin, out, carry, R);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
+                    uint32_t const *rowptr, adatm_kscat const *sc,
+                    void const *Z, int zscalar, void *out, void *carry,
+                    uint32_t R)
+{
+  if (sc->d < 1 || sc->d > ADATM_KMAXSCAT || R < 1 || R > 64)
+    return (int)hipErrorInvalidValue;
+  hipError_t e;
+  if (prec == KP_FP64)
+    e = scatter_d<double, double>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+  else if (prec == KP_FP32)
+    e = scatter_d<float, float>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+  else
+    e = scatter_d<float, bf16raw>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+} /* extern "C" */

@@ -70,6 +70,21 @@ int adatm_sort_by_id(uint32_t const *ids, uint64_t n, uint64_t nrows,
 int adatm_plan_enumerate(uint32_t nmodes, uint64_t const *dims,
                          adatm_plan *cands, int maxc);
 
+/* ---- engine internals shared with the SPLATT layer (engine.c) ---------- */
+int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
+                             uint64_t const *dims, uint64_t nnz,
+                             double ttnormsq, uint64_t rank,
+                             adatm_plan const *plan, int precision, int device,
+                             uint32_t const *use_grp, uint32_t const *use_tag,
+                             adatm_engine **out);
+uint32_t adatm_engine_nmodes(adatm_engine const *e);
+uint64_t adatm_engine_dim(adatm_engine const *e, uint32_t m);
+uint32_t adatm_engine_rank(adatm_engine const *e);
+int adatm_engine_precision(adatm_engine const *e);
+int adatm_count_fibers_host(adatm_coo const *t, uint32_t const *order,
+                            uint64_t *nfibs);
+int adatm_coo_from_sptensor(sptensor_t const *tt, adatm_coo *t);
+
 /* ---- seconds since an arbitrary origin --------------------------------- */
 double adatm_wtime(void);
 

@@ -1,0 +1,935 @@
+/*
+ * SPLATT-compatible C API of libsplatt_adatm (include/splatt.h).
+ *
+ * The reference patches SPLATT v1.1.1, whose public API is kept here with
+ * identical signatures (reference include/splatt.h:384-525) and the AdaTM
+ * entry points that the reference defines (src/adatm_rcsf.c, src/mttkrp.c,
+ * src/adatm_cpd.c).  Each CSF array handed out is registered with the device
+ * state that serves it: the plan, the (group, tag) use map and -- created on
+ * the first call that knows the rank -- an adatm_engine on the GPU.
+ *
+ * Allocation schemes (opts[SPLATT_OPTION_CSF_ALLOC]):
+ *   ADAPTIVE, ADAPTIVE_TIME_EFFICIENT  one CSF per memoization group of the
+ *        analytic (or learned) plan; root modes recompute their group's memo,
+ *        the others reuse it (src/adatm_rcsf.c:503-526, src/mttkrp.c:1958);
+ *   ONEMODE / TWOMODE / ALLMODE  SPLATT's plain CSF sets: the root of a CSF
+ *        by a root pass, any other level from the leaves (tag 2), as
+ *        mttkrp_csf does (src/mttkrp.c:1885-1936);
+ *   ADAPTIVE_SPACE_EFFICIENT is rejected (SPLATT_ERROR_BADINPUT).
+ *
+ * Differences from the reference, on purpose:
+ *   - splatt_mttkrp serves adaptive CSFs (the reference's mttkrp_csf has no
+ *     adaptive case and leaves the output zero, src/mttkrp.c:1904-1935);
+ *   - a reuse MTTKRP checks that the factors its memo contracted are the ones
+ *     passed now (content hash) and recomputes the memo when they are not;
+ *   - the root level keeps its ids (fids[0] is not NULL), so tensors with
+ *     empty slices factor correctly (the untiled reference drops them,
+ *     src/adatm_rcsf.c:51-56, and needs every slice non-empty).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "adatm_host.h"
+
+#define MAXN ADATM_MAX_NMODES
+
+typedef struct reg_ent {
+  splatt_csf *csf; /* key: the array returned to the caller */
+  uint32_t n_csf;
+  uint32_t N;
+  uint64_t nnz;
+  uint64_t dims[MAXN];
+  adatm_plan plan;
+  uint32_t use_grp[MAXN], use_tag[MAXN];
+  double ttnormsq;
+  int device;
+  adatm_engine *eng;
+  uint32_t eng_rank;
+  int eng_prec;
+  uint64_t fac_hash[MAXN];
+  int fac_on_dev[MAXN];
+  uint64_t memo_fac[MAXN][MAXN]; /* per group: factor hashes at its last root */
+  int memo_ok[MAXN];
+  struct reg_ent *next;
+} reg_ent;
+
+static reg_ent *g_reg = NULL;
+static pthread_mutex_t g_reg_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static reg_ent *reg_find(splatt_csf const *csf)
+{
+  pthread_mutex_lock(&g_reg_lock);
+  reg_ent *r = g_reg;
+  while (r && r->csf != csf) r = r->next;
+  pthread_mutex_unlock(&g_reg_lock);
+  return r;
+}
+
+static void reg_add(reg_ent *r)
+{
+  pthread_mutex_lock(&g_reg_lock);
+  r->next = g_reg;
+  g_reg = r;
+  pthread_mutex_unlock(&g_reg_lock);
+}
+
+static reg_ent *reg_remove(splatt_csf const *csf)
+{
+  pthread_mutex_lock(&g_reg_lock);
+  reg_ent **pp = &g_reg;
+  while (*pp && (*pp)->csf != csf) pp = &(*pp)->next;
+  reg_ent *r = *pp;
+  if (r) *pp = r->next;
+  pthread_mutex_unlock(&g_reg_lock);
+  return r;
+}
+
+/* ---- options (reference include/splatt.h:384-391) ----------------------- */
+double *splatt_default_opts(void)
+{
+  double *o = (double *)malloc(SPLATT_OPTION_NOPTIONS * sizeof(double));
+  if (!o) return NULL;
+#ifdef _OPENMP
+  o[SPLATT_OPTION_NTHREADS] = omp_get_max_threads();
+#else
+  o[SPLATT_OPTION_NTHREADS] = 1;
+#endif
+  o[SPLATT_OPTION_TOLERANCE] = 1e-5;
+  o[SPLATT_OPTION_NITER] = 50;
+  o[SPLATT_OPTION_VERBOSITY] = SPLATT_VERBOSITY_LOW;
+  o[SPLATT_OPTION_RANDSEED] = (double)time(NULL);
+  o[SPLATT_OPTION_CSF_ALLOC] = SPLATT_ADAPTIVE_TIME_EFFICIENT;
+  o[SPLATT_OPTION_TILE] = SPLATT_NOTILE;
+  o[SPLATT_OPTION_TILEDEPTH] = 1;
+  o[SPLATT_OPTION_DECOMP] = SPLATT_DECOMP_MEDIUM;
+  o[SPLATT_OPTION_COMM] = SPLATT_COMM_ALL2ALL;
+  o[SPLATT_OPTION_ADATM_PRECISION] = SPLATT_ADATM_FP64;
+  o[SPLATT_OPTION_ADATM_PLANNER] = SPLATT_ADATM_PLAN_ANALYTIC;
+  o[SPLATT_OPTION_ADATM_GRAPH] = 1;
+  o[SPLATT_OPTION_ADATM_REFQUIRKS] = 1;
+  o[SPLATT_OPTION_ADATM_SPACE_LIMIT] = (double)SPLATT_ADATM_SLIMIT;
+  o[SPLATT_OPTION_ADATM_DEVICE] = 0;
+  return o;
+}
+
+void splatt_free_opts(double *opts) { free(opts); }
+
+/* ---- plain SPLATT CSF sets as plans ------------------------------------- */
+/* modes sorted by dimension, shortest first, ties by mode index */
+static void small_first(uint32_t N, uint64_t const *dims, uint32_t *order)
+{
+  for (uint32_t m = 0; m < N; ++m) order[m] = m;
+  for (uint32_t i = 1; i < N; ++i)
+    for (uint32_t j = i; j > 0 && dims[order[j]] < dims[order[j - 1]]; --j) {
+      uint32_t const t = order[j];
+      order[j] = order[j - 1];
+      order[j - 1] = t;
+    }
+}
+
+static void root_first(uint32_t N, uint64_t const *dims, uint32_t root,
+                       uint32_t *order)
+{
+  uint32_t sf[MAXN], k = 1;
+  small_first(N, dims, sf);
+  order[0] = root;
+  for (uint32_t i = 0; i < N; ++i)
+    if (sf[i] != root) order[k++] = sf[i];
+}
+
+static int plain_plan(int alloc, uint32_t N, uint64_t const *dims,
+                      adatm_plan *p, uint32_t *use_grp, uint32_t *use_tag)
+{
+  memset(p, 0, sizeof(*p));
+  uint32_t o[MAXN] = {0};
+  small_first(N, dims, o);
+  switch (alloc) {
+  case SPLATT_CSF_ONEMODE:
+    p->n_grp = 1;
+    memcpy(p->grp[0].o_opt, o, N * sizeof(uint32_t));
+    p->grp[0].memo_mode = o[0];
+    p->grp[0].begin_imten = 1;
+    for (uint32_t m = 0; m < N; ++m) {
+      use_grp[m] = 0;
+      use_tag[m] = (m == o[0]) ? 0 : 2;
+    }
+    return SPLATT_SUCCESS;
+  case SPLATT_CSF_TWOMODE:
+    p->n_grp = 2;
+    memcpy(p->grp[0].o_opt, o, N * sizeof(uint32_t));
+    p->grp[0].memo_mode = o[0];
+    p->grp[0].begin_imten = 1;
+    root_first(N, dims, o[N - 1], p->grp[1].o_opt);
+    p->grp[1].memo_mode = o[N - 1];
+    p->grp[1].begin_imten = 1;
+    for (uint32_t m = 0; m < N; ++m) {
+      if (m == o[N - 1]) {
+        use_grp[m] = 1;
+        use_tag[m] = 0;
+      } else {
+        use_grp[m] = 0;
+        use_tag[m] = (m == o[0]) ? 0 : 2;
+      }
+    }
+    return SPLATT_SUCCESS;
+  case SPLATT_CSF_ALLMODE:
+    p->n_grp = N;
+    for (uint32_t m = 0; m < N; ++m) {
+      root_first(N, dims, m, p->grp[m].o_opt);
+      p->grp[m].memo_mode = m;
+      p->grp[m].begin_imten = 1;
+      use_grp[m] = m;
+      use_tag[m] = 0;
+    }
+    return SPLATT_SUCCESS;
+  default:
+    return SPLATT_ERROR_BADINPUT;
+  }
+}
+
+/* ---- host CSF (u32) <-> SPLATT layout ------------------------------------ */
+static splatt_csf *csf_array_from_hcsf(adatm_hcsf const *h, uint32_t n,
+                                       uint32_t N, uint64_t const *dims)
+{
+  splatt_csf *out = (splatt_csf *)calloc(n, sizeof(splatt_csf));
+  if (!out) return NULL;
+  for (uint32_t g = 0; g < n; ++g) {
+    splatt_csf *c = out + g;
+    c->nnz = h[g].nnz;
+    c->nmodes = N;
+    for (uint32_t m = 0; m < N; ++m) {
+      c->dims[m] = dims[m];
+      c->dim_perm[m] = h[g].order[m];
+      c->tile_dims[m] = 1;
+    }
+    c->which_tile = SPLATT_NOTILE;
+    c->ntiles = 1;
+    c->pt = (csf_sparsity *)calloc(1, sizeof(csf_sparsity));
+    if (!c->pt) return out;
+    csf_sparsity *pt = c->pt;
+    for (uint32_t l = 0; l < N; ++l) {
+      uint64_t const F = h[g].F[l];
+      pt->nfibs[l] = F;
+      pt->fids[l] = (splatt_idx_t *)malloc((F ? F : 1) * sizeof(splatt_idx_t));
+      for (uint64_t i = 0; i < F; ++i) pt->fids[l][i] = h[g].fids[l][i];
+      if (l + 1 < N) {
+        pt->fptr[l] = (splatt_idx_t *)malloc((F + 1) * sizeof(splatt_idx_t));
+        for (uint64_t i = 0; i <= F; ++i) pt->fptr[l][i] = h[g].fptr[l][i];
+      }
+    }
+    pt->vals = (splatt_val_t *)malloc(h[g].nnz * sizeof(splatt_val_t));
+    memcpy(pt->vals, h[g].vals, h[g].nnz * sizeof(double));
+  }
+  return out;
+}
+
+/* u32 host CSF (with parent arrays) of a registered SPLATT CSF */
+static int hcsf_from_csf(splatt_csf const *c, adatm_hcsf *h)
+{
+  uint32_t const N = (uint32_t)c->nmodes;
+  memset(h, 0, sizeof(*h));
+  h->N = N;
+  h->nnz = c->nnz;
+  csf_sparsity const *pt = c->pt;
+  for (uint32_t l = 0; l < N; ++l) {
+    h->order[l] = (uint32_t)c->dim_perm[l];
+    h->F[l] = pt->nfibs[l];
+  }
+  for (uint32_t l = 0; l < N; ++l) {
+    uint64_t const F = h->F[l];
+    h->fids[l] = (uint32_t *)malloc((F ? F : 1) * sizeof(uint32_t));
+    if (!h->fids[l]) goto oom;
+    for (uint64_t i = 0; i < F; ++i)
+      h->fids[l][i] = pt->fids[l] ? (uint32_t)pt->fids[l][i] : (uint32_t)i;
+    if (l + 1 < N) {
+      h->fptr[l] = (uint32_t *)malloc((F + 1) * sizeof(uint32_t));
+      if (!h->fptr[l]) goto oom;
+      for (uint64_t i = 0; i <= F; ++i) h->fptr[l][i] = (uint32_t)pt->fptr[l][i];
+    }
+    if (l >= 1) {
+      h->par[l] = (uint32_t *)malloc((F ? F : 1) * sizeof(uint32_t));
+      if (!h->par[l]) goto oom;
+    }
+  }
+  for (uint32_t l = 0; l + 1 < N; ++l)
+    for (uint64_t p = 0; p < h->F[l]; ++p)
+      for (uint32_t c2 = h->fptr[l][p]; c2 < h->fptr[l][p + 1]; ++c2)
+        h->par[l + 1][c2] = (uint32_t)p;
+  h->vals = (double *)malloc(c->nnz * sizeof(double));
+  if (!h->vals) goto oom;
+  memcpy(h->vals, pt->vals, c->nnz * sizeof(double));
+  return SPLATT_SUCCESS;
+oom:
+  adatm_hcsf_free(h);
+  return SPLATT_ERROR_NOMEMORY;
+}
+
+static void free_csf_array(splatt_csf *t, uint32_t n)
+{
+  if (!t) return;
+  for (uint32_t g = 0; g < n; ++g) {
+    csf_sparsity *pt = t[g].pt;
+    if (!pt) continue;
+    for (uint32_t l = 0; l < SPLATT_MAX_NMODES; ++l) {
+      free(pt->fptr[l]);
+      free(pt->fids[l]);
+    }
+    free(pt->vals);
+    free(pt);
+  }
+  free(t);
+}
+
+/* ---- construction ---------------------------------------------------------- */
+static int build_from_coo(adatm_coo const *t, double const *opts,
+                          adatm_plan const *given, uint32_t const *given_grp,
+                          uint32_t const *given_tag, splatt_csf **tensors,
+                          uint32_t *n_csf_out)
+{
+  uint32_t const N = t->nmodes;
+  int const alloc = (int)opts[SPLATT_OPTION_CSF_ALLOC];
+  reg_ent *r = (reg_ent *)calloc(1, sizeof(reg_ent));
+  if (!r) return SPLATT_ERROR_NOMEMORY;
+  r->N = N;
+  r->nnz = t->nnz;
+  memcpy(r->dims, t->dims, N * sizeof(uint64_t));
+  r->device = (int)opts[SPLATT_OPTION_ADATM_DEVICE];
+  int rc = SPLATT_SUCCESS;
+  if (given) {
+    r->plan = *given;
+    memcpy(r->use_grp, given_grp, N * sizeof(uint32_t));
+    memcpy(r->use_tag, given_tag, N * sizeof(uint32_t));
+  } else if (alloc == SPLATT_ADAPTIVE || alloc == SPLATT_ADAPTIVE_TIME_EFFICIENT) {
+    adatm_plan cands[256];
+    int nc = 0, ch = 0;
+    /* the analytic choice (min pops) does not depend on the rank: every
+       product count is proportional to it; rank 1 only loosens the space
+       filter, which the engine re-checks against the device memory */
+    rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
+                             opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands,
+                             256, &nc, &ch);
+    if (rc == SPLATT_SUCCESS) {
+      r->plan = cands[ch];
+      adatm_plan_use(&r->plan, N, r->use_grp, r->use_tag);
+    }
+  } else if (alloc == SPLATT_ADAPTIVE_SPACE_EFFICIENT) {
+    adatm_set_error("SPLATT_ADAPTIVE_SPACE_EFFICIENT is not supported");
+    rc = SPLATT_ERROR_BADINPUT;
+  } else {
+    rc = plain_plan(alloc, N, t->dims, &r->plan, r->use_grp, r->use_tag);
+  }
+  if (rc != SPLATT_SUCCESS) {
+    free(r);
+    return rc;
+  }
+  uint32_t const ng = r->plan.n_grp;
+  adatm_hcsf *h = (adatm_hcsf *)calloc(ng, sizeof(adatm_hcsf));
+  for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < ng; ++g)
+    rc = adatm_hcsf_build(t, r->plan.grp[g].o_opt, h + g);
+  splatt_csf *arr = NULL;
+  if (rc == SPLATT_SUCCESS) {
+    arr = csf_array_from_hcsf(h, ng, N, t->dims);
+    if (!arr) rc = SPLATT_ERROR_NOMEMORY;
+  }
+  for (uint32_t g = 0; g < ng; ++g) adatm_hcsf_free(h + g);
+  free(h);
+  if (rc != SPLATT_SUCCESS) {
+    free_csf_array(arr, ng);
+    free(r);
+    return rc;
+  }
+  double ss = 0.0;
+  for (uint64_t j = 0; j < t->nnz; ++j) ss += t->vals[j] * t->vals[j];
+  r->ttnormsq = ss;
+  r->csf = arr;
+  r->n_csf = ng;
+  reg_add(r);
+  *tensors = arr;
+  if (n_csf_out) *n_csf_out = ng;
+  return SPLATT_SUCCESS;
+}
+
+int splatt_csf_convert(splatt_idx_t const nmodes, splatt_idx_t const nnz,
+                       splatt_idx_t **const inds, splatt_val_t *const vals,
+                       splatt_csf **tensors, double const *const options)
+{
+  if (!tensors || !inds || !vals || nmodes < 3 || nmodes > MAXN || nnz == 0 ||
+      nnz >= (1ull << 32)) {
+    adatm_set_error("splatt_csf_convert: unsupported input (nmodes=%llu, nnz=%llu)",
+                    (unsigned long long)nmodes, (unsigned long long)nnz);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  double *dflt = NULL;
+  double const *opts = options;
+  if (!opts) opts = dflt = splatt_default_opts();
+  adatm_coo t;
+  uint64_t dims[MAXN];
+  for (uint32_t m = 0; m < nmodes; ++m) {
+    uint64_t mx = 0;
+    for (uint64_t j = 0; j < nnz; ++j)
+      if (inds[m][j] > mx) mx = inds[m][j];
+    dims[m] = mx + 1;
+    if (dims[m] > (1ull << 31)) {
+      free(dflt);
+      return SPLATT_ERROR_BADINPUT;
+    }
+  }
+  int rc = adatm_coo_alloc((uint32_t)nmodes, nnz, dims, &t);
+  if (rc != SPLATT_SUCCESS) {
+    free(dflt);
+    return rc;
+  }
+  for (uint32_t m = 0; m < nmodes; ++m)
+    for (uint64_t j = 0; j < nnz; ++j) t.ind[m][j] = (uint32_t)inds[m][j];
+  memcpy(t.vals, vals, nnz * sizeof(double));
+  rc = build_from_coo(&t, opts, NULL, NULL, NULL, tensors, NULL);
+  adatm_coo_free(&t);
+  free(dflt);
+  return rc;
+}
+
+splatt_csf *splatt_csf_alloc(sptensor_t *const tt, double const *const opts)
+{
+  splatt_csf *out = NULL;
+  if (!tt) return NULL;
+  if (splatt_csf_convert(tt->nmodes, tt->nnz, tt->ind, tt->vals, &out, opts) !=
+      SPLATT_SUCCESS)
+    return NULL;
+  return out;
+}
+
+/* Coordinate text format: one nonzero per line, N 1-based indices then the
+ * value; '#' starts a comment line. */
+int adatm_read_tns(char const *fname, uint32_t *nmodes_out, uint64_t *nnz_out,
+                   splatt_idx_t ***inds_out, splatt_val_t **vals_out)
+{
+  FILE *f = fopen(fname, "r");
+  if (!f) {
+    adatm_set_error("cannot open %s", fname);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  char *line = NULL;
+  size_t cap = 0;
+  ssize_t len;
+  uint32_t N = 0;
+  uint64_t n = 0, ncap = 1 << 16;
+  splatt_idx_t **ind = NULL;
+  splatt_val_t *val = NULL;
+  int rc = SPLATT_SUCCESS;
+  while ((len = getline(&line, &cap, f)) >= 0) {
+    char *p = line;
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '#' || *p == '\n' || *p == '\0') continue;
+    double tok[MAXN + 2];
+    uint32_t nt = 0;
+    char *end;
+    while (nt < MAXN + 2) {
+      double const v = strtod(p, &end);
+      if (end == p) break;
+      tok[nt++] = v;
+      p = end;
+    }
+    if (N == 0) {
+      if (nt < 4 || nt > MAXN + 1) {
+        adatm_set_error("%s: %u columns per line", fname, nt);
+        rc = SPLATT_ERROR_BADINPUT;
+        break;
+      }
+      N = nt - 1;
+      ind = (splatt_idx_t **)calloc(N, sizeof(splatt_idx_t *));
+      for (uint32_t m = 0; m < N; ++m)
+        ind[m] = (splatt_idx_t *)malloc(ncap * sizeof(splatt_idx_t));
+      val = (splatt_val_t *)malloc(ncap * sizeof(splatt_val_t));
+    }
+    if (nt != N + 1) {
+      adatm_set_error("%s: line %llu has %u columns, expected %u", fname,
+                      (unsigned long long)n + 1, nt, N + 1);
+      rc = SPLATT_ERROR_BADINPUT;
+      break;
+    }
+    if (n == ncap) {
+      ncap *= 2;
+      for (uint32_t m = 0; m < N; ++m)
+        ind[m] = (splatt_idx_t *)realloc(ind[m], ncap * sizeof(splatt_idx_t));
+      val = (splatt_val_t *)realloc(val, ncap * sizeof(splatt_val_t));
+    }
+    for (uint32_t m = 0; m < N; ++m) {
+      if (tok[m] < 1.0) {
+        adatm_set_error("%s: index below 1 on line %llu", fname,
+                        (unsigned long long)n + 1);
+        rc = SPLATT_ERROR_BADINPUT;
+      }
+      ind[m][n] = (splatt_idx_t)tok[m] - 1;
+    }
+    val[n] = tok[N];
+    ++n;
+    if (rc != SPLATT_SUCCESS) break;
+  }
+  free(line);
+  fclose(f);
+  if (rc == SPLATT_SUCCESS && n == 0) {
+    adatm_set_error("%s: no nonzeros", fname);
+    rc = SPLATT_ERROR_BADINPUT;
+  }
+  if (rc != SPLATT_SUCCESS) {
+    if (ind)
+      for (uint32_t m = 0; m < N; ++m) free(ind[m]);
+    free(ind);
+    free(val);
+    return rc;
+  }
+  *nmodes_out = N;
+  *nnz_out = n;
+  *inds_out = ind;
+  *vals_out = val;
+  return SPLATT_SUCCESS;
+}
+
+int splatt_csf_load(char const *const fname, splatt_idx_t *nmodes,
+                    splatt_csf **tensors, double const *const options)
+{
+  uint32_t N;
+  uint64_t nnz;
+  splatt_idx_t **ind;
+  splatt_val_t *val;
+  int rc = adatm_read_tns(fname, &N, &nnz, &ind, &val);
+  if (rc != SPLATT_SUCCESS) return rc;
+  rc = splatt_csf_convert(N, nnz, ind, val, tensors, options);
+  for (uint32_t m = 0; m < N; ++m) free(ind[m]);
+  free(ind);
+  free(val);
+  if (rc == SPLATT_SUCCESS && nmodes) *nmodes = N;
+  return rc;
+}
+
+void splatt_free_csf(splatt_csf *tensors, double const *const options)
+{
+  if (!tensors) return;
+  reg_ent *r = reg_remove(tensors);
+  uint32_t n;
+  if (r) {
+    n = r->n_csf;
+    adatm_engine_destroy(r->eng);
+    free(r);
+  } else {
+    int const alloc = options ? (int)options[SPLATT_OPTION_CSF_ALLOC]
+                              : SPLATT_CSF_ONEMODE;
+    n = (alloc == SPLATT_CSF_ALLMODE) ? (uint32_t)tensors->nmodes
+        : (alloc == SPLATT_CSF_TWOMODE) ? 2 : 1;
+  }
+  free_csf_array(tensors, n);
+}
+
+void splatt_free_kruskal(splatt_kruskal *factored)
+{
+  if (!factored) return;
+  for (splatt_idx_t m = 0; m < factored->nmodes && m < SPLATT_MAX_NMODES; ++m) {
+    free(factored->factors[m]);
+    factored->factors[m] = NULL;
+  }
+  free(factored->lambda);
+  factored->lambda = NULL;
+}
+
+/* ---- engine management ------------------------------------------------------- */
+static uint64_t hash_matrix(double const *A, uint64_t n)
+{
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ n;
+  uint64_t const *w = (uint64_t const *)A;
+  #pragma omp parallel for reduction(^ : h) schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    uint64_t x = w[i] + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 29;
+    h ^= x;
+  }
+  return h;
+}
+
+static int ensure_engine(reg_ent *r, uint64_t rank, int prec)
+{
+  if (r->eng && r->eng_rank == rank && r->eng_prec == prec) return SPLATT_SUCCESS;
+  if (r->eng) {
+    adatm_engine_destroy(r->eng);
+    r->eng = NULL;
+  }
+  memset(r->fac_on_dev, 0, sizeof(r->fac_on_dev));
+  memset(r->memo_ok, 0, sizeof(r->memo_ok));
+  adatm_hcsf *h = (adatm_hcsf *)calloc(r->n_csf, sizeof(adatm_hcsf));
+  if (!h) return SPLATT_ERROR_NOMEMORY;
+  int rc = SPLATT_SUCCESS;
+  for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < r->n_csf; ++g)
+    rc = hcsf_from_csf(r->csf + g, h + g);
+  if (rc == SPLATT_SUCCESS)
+    rc = adatm_engine_create_hcsf(h, r->N, r->dims, r->nnz, r->ttnormsq, rank,
+                                  &r->plan, prec, r->device, r->use_grp,
+                                  r->use_tag, &r->eng);
+  for (uint32_t g = 0; g < r->n_csf; ++g) adatm_hcsf_free(h + g);
+  free(h);
+  if (rc == SPLATT_SUCCESS) {
+    r->eng_rank = (uint32_t)rank;
+    r->eng_prec = prec;
+  }
+  return rc;
+}
+
+static int opt_prec(double const *opts)
+{
+  int const p = opts ? (int)opts[SPLATT_OPTION_ADATM_PRECISION] : SPLATT_ADATM_FP64;
+  return (p >= SPLATT_ADATM_FP64 && p <= SPLATT_ADATM_BF16ROWS) ? p
+                                                               : SPLATT_ADATM_FP64;
+}
+
+/* upload the factors whose contents changed; returns their hashes */
+static int sync_factors(reg_ent *r, splatt_val_t **mats, uint32_t skip,
+                        uint64_t *hashes)
+{
+  for (uint32_t m = 0; m < r->N; ++m) {
+    if (m == skip || !mats[m]) continue;
+    uint64_t const h = hash_matrix(mats[m], r->dims[m] * r->eng_rank);
+    hashes[m] = h;
+    if (!r->fac_on_dev[m] || r->fac_hash[m] != h) {
+      int const rc = adatm_engine_set_factor(r->eng, m, mats[m]);
+      if (rc != SPLATT_SUCCESS) return rc;
+      r->fac_hash[m] = h;
+      r->fac_on_dev[m] = 1;
+    }
+  }
+  return SPLATT_SUCCESS;
+}
+
+/* level-d reuse reads the memo at level L = b - min(n_imten, b - d)
+ * (src/mttkrp.c:1578-1591); it is current when the factors of levels
+ * L+1 .. N-1 are the ones its group's last root pass contracted */
+static int memo_current(reg_ent const *r, uint32_t g, uint32_t d,
+                        uint64_t const *hashes)
+{
+  adatm_group const *G = r->plan.grp + g;
+  if (!r->memo_ok[g]) return 0;
+  uint32_t const bd = G->begin_imten - d;
+  uint32_t const L = G->begin_imten - (G->n_imten < bd ? G->n_imten : bd);
+  for (uint32_t l = L + 1; l < r->N; ++l)
+    if (r->memo_fac[g][G->o_opt[l]] != hashes[G->o_opt[l]]) return 0;
+  return 1;
+}
+
+static int mttkrp_registered(reg_ent *r, uint32_t mode, splatt_val_t **mats,
+                             splatt_val_t *out)
+{
+  uint64_t hashes[MAXN] = {0};
+  uint32_t const g = r->use_grp[mode];
+  uint32_t const root = r->plan.grp[g].memo_mode;
+  int rc = sync_factors(r, mats, mode, hashes);
+  if (rc != SPLATT_SUCCESS) return rc;
+  if (r->use_tag[mode] == 1) {
+    uint32_t d = 0;
+    while (r->plan.grp[g].o_opt[d] != mode) ++d;
+    if (!memo_current(r, g, d, hashes)) {
+      /* rebuild the memo: a root pass reads every factor but the root's */
+      if (!mats[mode]) {
+        adatm_set_error("splatt_mttkrp: memo of group %u is stale and the "
+                        "factor of mode %u is needed to rebuild it", g, mode);
+        return SPLATT_ERROR_BADINPUT;
+      }
+      rc = sync_factors(r, mats, root, hashes);
+      if (rc != SPLATT_SUCCESS) return rc;
+      rc = adatm_engine_mttkrp(r->eng, root, NULL);
+      if (rc != SPLATT_SUCCESS) return rc;
+      memcpy(r->memo_fac[g], hashes, sizeof(hashes));
+      r->memo_ok[g] = 1;
+    }
+  }
+  rc = adatm_engine_mttkrp(r->eng, mode, out);
+  if (rc == SPLATT_SUCCESS && r->use_tag[mode] == 0 && r->plan.grp[g].n_imten) {
+    memcpy(r->memo_fac[g], hashes, sizeof(hashes));
+    r->memo_ok[g] = 1;
+  }
+  return rc;
+}
+
+int splatt_mttkrp(splatt_idx_t const mode, splatt_idx_t const ncolumns,
+                  splatt_csf const *const tensors, splatt_val_t **matrices,
+                  splatt_val_t *const matout, double const *const options)
+{
+  reg_ent *r = reg_find(tensors);
+  if (!r) {
+    adatm_set_error("splatt_mttkrp: CSF not created by this library");
+    return SPLATT_ERROR_BADINPUT;
+  }
+  if (mode >= r->N || ncolumns < 1 || !matrices || !matout)
+    return SPLATT_ERROR_BADINPUT;
+  for (uint32_t m = 0; m < r->N; ++m)
+    if (m != mode && !matrices[m]) return SPLATT_ERROR_BADINPUT;
+  int const prec = opt_prec(options);
+  int rc = ensure_engine(r, ncolumns, prec);
+  if (rc != SPLATT_SUCCESS) return rc;
+  return mttkrp_registered(r, (uint32_t)mode, matrices, matout);
+}
+
+/* ---- AdaTM compatibility entry points ------------------------------------ */
+splatt_csf *csf_alloc_adaptive(sptensor_t *const tt,
+                               group_properties *const grp_prop,
+                               splatt_idx_t const n_grp,
+                               double const *const opts, splatt_idx_t *n_csf)
+{
+  if (!tt || !grp_prop || n_grp < 1 || tt->nmodes > MAXN) return NULL;
+  if (opts && (int)opts[SPLATT_OPTION_CSF_ALLOC] == SPLATT_ADAPTIVE_SPACE_EFFICIENT &&
+      n_grp > 1) {
+    fprintf(stderr, "csf_alloc_adaptive: SPACE_EFFICIENT is not supported\n");
+    return NULL;
+  }
+  uint32_t const N = (uint32_t)tt->nmodes;
+  adatm_plan p;
+  memset(&p, 0, sizeof(p));
+  p.n_grp = (uint32_t)n_grp;
+  for (uint32_t g = 0; g < n_grp; ++g) {
+    p.grp[g].memo_mode = (uint32_t)grp_prop[g].memo_mode;
+    p.grp[g].begin_imten = (uint32_t)grp_prop[g].begin_imten;
+    p.grp[g].n_imten = (uint32_t)grp_prop[g].n_imten;
+    for (uint32_t m = 0; m < N; ++m) p.grp[g].o_opt[m] = (uint32_t)grp_prop[g].o_opt[m];
+  }
+  uint32_t ug[MAXN], ut[MAXN];
+  adatm_plan_use(&p, N, ug, ut);
+  adatm_coo t;
+  if (adatm_coo_from_sptensor(tt, &t) != SPLATT_SUCCESS) return NULL;
+  double *dflt = opts ? NULL : splatt_default_opts();
+  splatt_csf *out = NULL;
+  uint32_t nc = 0;
+  int const rc = build_from_coo(&t, opts ? opts : dflt, &p, ug, ut, &out, &nc);
+  adatm_coo_free(&t);
+  free(dflt);
+  if (rc != SPLATT_SUCCESS) return NULL;
+  if (n_csf) *n_csf = nc;
+  return out;
+}
+
+rcsf_seq_adaptive *rcsf_alloc_adaptive(splatt_csf *const ct,
+                                       splatt_idx_t const nfactors,
+                                       group_properties *const grp_prop,
+                                       splatt_idx_t const n_grp,
+                                       splatt_idx_t const n_csf,
+                                       double const *const opts,
+                                       splatt_idx_t *n_rcsf)
+{
+  reg_ent *r = reg_find(ct);
+  if (!r) return NULL;
+  if (ensure_engine(r, nfactors, opt_prec(opts)) != SPLATT_SUCCESS) return NULL;
+  rcsf_seq_adaptive *ret =
+      (rcsf_seq_adaptive *)calloc(n_csf ? n_csf : 1, sizeof(rcsf_seq_adaptive));
+  splatt_idx_t j = 0;
+  for (splatt_idx_t g = 0; g < n_grp; ++g) {
+    if (grp_prop[g].n_imten == 0) continue;
+    uint32_t const b = (uint32_t)grp_prop[g].begin_imten;
+    uint32_t const nim = (uint32_t)grp_prop[g].n_imten;
+    ret[j].begin_imten = b;
+    ret[j].n_imten = nim;
+    ret[j].rcsfs = (splatt_csf *)calloc(nim, sizeof(splatt_csf));
+    splatt_csf const *c = ct + g;
+    for (uint32_t k = 0; k < nim; ++k) {
+      /* vCSF k describes the memo of level b-1-k (adatm_rcsf.c:227-276);
+         structure aliases the CSF, values live on the device */
+      splatt_csf *v = ret[j].rcsfs + k;
+      uint32_t const lvl = b - 1 - k;
+      uint32_t const nm = lvl + 2;
+      v->nmodes = nm;
+      for (uint32_t m = 0; m < nm; ++m) {
+        v->dims[m] = c->dims[c->dim_perm[m]];
+        v->dim_perm[m] = c->dim_perm[m];
+        v->tile_dims[m] = 1;
+      }
+      v->dims[nm - 1] = nfactors;
+      v->which_tile = SPLATT_NOTILE;
+      v->ntiles = 1;
+      v->nnz = c->pt->nfibs[lvl] * nfactors;
+      v->pt = (csf_sparsity *)calloc(1, sizeof(csf_sparsity));
+      for (uint32_t m = 0; m + 1 < nm; ++m) v->pt->nfibs[m] = c->pt->nfibs[m];
+      v->pt->nfibs[nm - 1] = v->nnz;
+      for (uint32_t m = 0; m + 2 < nm; ++m) {
+        v->pt->fptr[m] = c->pt->fptr[m];
+        v->pt->fids[m] = c->pt->fids[m];
+      }
+      v->pt->fids[nm - 2] = c->pt->fids[nm - 2];
+      v->pt->vals = NULL;
+    }
+    ++j;
+  }
+  if (n_rcsf) *n_rcsf = j;
+  return ret;
+}
+
+size_t csf_storage_adaptive(splatt_csf const *const tensors,
+                            splatt_idx_t const n_csf,
+                            group_properties *const grp_prop, int const n_grp)
+{
+  (void)grp_prop;
+  (void)n_grp;
+  size_t bytes = 0;
+  for (splatt_idx_t g = 0; g < n_csf; ++g) {
+    splatt_csf const *c = tensors + g;
+    bytes += c->nnz * sizeof(splatt_val_t) + c->nnz * sizeof(splatt_idx_t) +
+             c->ntiles * sizeof(csf_sparsity);
+    for (splatt_idx_t m = 0; m + 1 < c->nmodes; ++m) {
+      bytes += (c->pt->nfibs[m] + 1) * sizeof(splatt_idx_t);
+      if (c->pt->fids[m]) bytes += c->pt->nfibs[m] * sizeof(splatt_idx_t);
+    }
+  }
+  return bytes;
+}
+
+size_t rcsf_storage_adaptive(rcsf_seq_adaptive const *const seq_rcsf,
+                             splatt_idx_t const n_rcsf)
+{
+  size_t bytes = 0;
+  for (splatt_idx_t s = 0; s < n_rcsf; ++s)
+    for (splatt_idx_t k = 0; k < seq_rcsf[s].n_imten; ++k)
+      bytes += seq_rcsf[s].rcsfs[k].nnz * sizeof(splatt_val_t) +
+               seq_rcsf[s].rcsfs[k].ntiles * sizeof(csf_sparsity);
+  return bytes;
+}
+
+void mttkrp_csf_adaptive(splatt_csf const *const tensors,
+                         rcsf_seq_adaptive const *const rs_seq,
+                         splatt_idx_t const n_csf, matrix_t **mats,
+                         splatt_idx_t const mode, void *const thds,
+                         group_properties *const grp_prop, int const n_grp,
+                         splatt_idx_t const use_csf, splatt_idx_t const use_tag,
+                         double const *const opts)
+{
+  (void)rs_seq; (void)n_csf; (void)thds; (void)grp_prop; (void)n_grp;
+  (void)use_csf; (void)use_tag;
+  reg_ent *r = reg_find(tensors);
+  if (!r || !mats || mode >= r->N) {
+    fprintf(stderr, "mttkrp_csf_adaptive: unknown CSF or bad mode\n");
+    return;
+  }
+  splatt_val_t *m[MAXN];
+  for (uint32_t k = 0; k < r->N; ++k) m[k] = mats[k] ? mats[k]->vals : NULL;
+  matrix_t *M = mats[SPLATT_MAX_NMODES];
+  M->I = tensors[0].dims[mode];
+  int const rc = splatt_mttkrp(mode, mats[0]->J, tensors, m, M->vals, opts);
+  if (rc != SPLATT_SUCCESS)
+    fprintf(stderr, "mttkrp_csf_adaptive: %s\n", adatm_last_error());
+}
+
+/* ---- CP-ALS ------------------------------------------------------------------ */
+/* columns to unit 2-norm with the norms folded into lambda (SPLATT's
+ * cpd_post_process) */
+static void post_process(uint32_t N, uint64_t const *dims, uint64_t R,
+                         double **A, double *lambda)
+{
+  for (uint32_t m = 0; m < N; ++m)
+    for (uint64_t r = 0; r < R; ++r) {
+      double s = 0.0;
+      for (uint64_t i = 0; i < dims[m]; ++i) s += A[m][i * R + r] * A[m][i * R + r];
+      s = sqrt(s);
+      if (s > 0.0)
+        for (uint64_t i = 0; i < dims[m]; ++i) A[m][i * R + r] /= s;
+      lambda[r] *= s;
+    }
+}
+
+static int cpd_registered(reg_ent *r, splatt_idx_t nfactors,
+                          double const *opts, splatt_kruskal *out)
+{
+  int const prec = opt_prec(opts);
+  int rc = ensure_engine(r, nfactors, prec);
+  if (rc != SPLATT_SUCCESS) return rc;
+  uint64_t const seed = (uint64_t)opts[SPLATT_OPTION_RANDSEED];
+  rc = adatm_engine_init_factors(r->eng, seed);
+  if (rc != SPLATT_SUCCESS) return rc;
+  memset(r->fac_on_dev, 0, sizeof(r->fac_on_dev));
+  memset(r->memo_ok, 0, sizeof(r->memo_ok));
+  uint32_t const niters = (uint32_t)opts[SPLATT_OPTION_NITER];
+  double const tol = opts[SPLATT_OPTION_TOLERANCE];
+  int const graph = (int)opts[SPLATT_OPTION_ADATM_GRAPH];
+  int const verb = (int)opts[SPLATT_OPTION_VERBOSITY];
+  double fit = 0.0, oldfit = 0.0;
+  rc = adatm_engine_cpd_begin(r->eng);
+  if (rc != SPLATT_SUCCESS) return rc;
+  double const t_all = adatm_wtime();
+  for (uint32_t it = 0; it < niters; ++it) {
+    double const t0 = adatm_wtime();
+    rc = adatm_engine_cpd_sweep(r->eng, it, graph, &fit);
+    if (rc != SPLATT_SUCCESS) return rc;
+    if (verb > SPLATT_VERBOSITY_NONE)
+      printf("  its = %3u (%0.3fs)  fit = %0.5f  delta = %+0.4e\n", it + 1,
+             adatm_wtime() - t0, fit, fit - oldfit);
+    if (it > 0 && fabs(fit - oldfit) < tol) break;
+    oldfit = fit;
+  }
+  if (verb > SPLATT_VERBOSITY_NONE) printf("CPD = %0.3fs\n", adatm_wtime() - t_all);
+  memset(out, 0, sizeof(*out));
+  out->rank = nfactors;
+  out->nmodes = r->N;
+  out->fit = fit;
+  out->lambda = (splatt_val_t *)malloc(nfactors * sizeof(splatt_val_t));
+  if (!out->lambda) return SPLATT_ERROR_NOMEMORY;
+  rc = adatm_engine_get_lambda(r->eng, out->lambda);
+  for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < r->N; ++m) {
+    out->dims[m] = r->dims[m];
+    out->factors[m] = (splatt_val_t *)malloc(r->dims[m] * nfactors * sizeof(splatt_val_t));
+    if (!out->factors[m]) return SPLATT_ERROR_NOMEMORY;
+    rc = adatm_engine_get_factor(r->eng, m, out->factors[m]);
+  }
+  if (rc == SPLATT_SUCCESS)
+    post_process(r->N, r->dims, nfactors, out->factors, out->lambda);
+  return rc;
+}
+
+int splatt_cpd_als(splatt_csf const *const tensors, splatt_idx_t const nfactors,
+                   double const *const options, splatt_kruskal *factored)
+{
+  reg_ent *r = reg_find(tensors);
+  if (!r || !factored || nfactors < 1) {
+    adatm_set_error("splatt_cpd_als: CSF not created by this library");
+    return SPLATT_ERROR_BADINPUT;
+  }
+  double *dflt = options ? NULL : splatt_default_opts();
+  int const rc = cpd_registered(r, nfactors, options ? options : dflt, factored);
+  free(dflt);
+  return rc;
+}
+
+int splatt_cpd_als_adaptive(splatt_csf const *const tensors,
+                            rcsf_seq_adaptive *const rs_seq,
+                            splatt_idx_t const n_csf,
+                            group_properties const *const grp_prop,
+                            splatt_idx_t const n_grp,
+                            splatt_idx_t const *const use_csfs,
+                            splatt_idx_t const *const use_tags,
+                            splatt_idx_t const nfactors,
+                            double const *const options,
+                            splatt_kruskal *factored)
+{
+  (void)rs_seq; (void)n_csf;
+  reg_ent *r = reg_find(tensors);
+  if (!r) {
+    adatm_set_error("splatt_cpd_als_adaptive: CSF not created by this library");
+    return SPLATT_ERROR_BADINPUT;
+  }
+  /* the registered plan must be the one described by the arguments */
+  if (n_grp != r->plan.n_grp) return SPLATT_ERROR_BADINPUT;
+  for (splatt_idx_t g = 0; g < n_grp; ++g)
+    if (grp_prop[g].memo_mode != r->plan.grp[g].memo_mode ||
+        grp_prop[g].n_imten != r->plan.grp[g].n_imten)
+      return SPLATT_ERROR_BADINPUT;
+  for (uint32_t m = 0; m < r->N; ++m)
+    if (use_csfs[m] != r->use_grp[m] || use_tags[m] != r->use_tag[m])
+      return SPLATT_ERROR_BADINPUT;
+  return splatt_cpd_als(tensors, nfactors, options, factored);
+}
+
+/* The engine behind a registered CSF (NULL before the rank is known). */
+adatm_engine *adatm_csf_engine(splatt_csf const *tensors)
+{
+  reg_ent *r = reg_find(tensors);
+  return r ? r->eng : NULL;
+}

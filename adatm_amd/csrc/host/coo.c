@@ -539,13 +539,14 @@ int adatm_sort_by_id(uint32_t const *ids, uint64_t n, uint64_t nrows,
   return SPLATT_SUCCESS;
 }
 
-/* Fiber counts for the planner on the device path.  The statistics come from
- * the same host sort as the CSF build until the device sort is in place. */
+int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
+                                   uint32_t norders, uint64_t *F, int device);
+
+/* Fiber counts on the device (radix sort + boundary scan). */
 int adatm_count_fibers_device(adatm_coo const *t, uint32_t const *order,
                               uint64_t *nfibs, int device)
 {
-  (void)device;
-  return adatm_count_fibers_host(t, order, nfibs);
+  return adatm_count_fibers_device_many(t, order, 1, nfibs, device);
 }
 
 /* Allocate an empty coordinate tensor (malloc'd, freed by adatm_coo_free). */

@@ -38,6 +38,23 @@ typedef struct {
   uint32_t nslots;
 } dwork;
 
+/* one fused reduction pass of a group: level lb (children) -> lt (output) */
+typedef struct {
+  uint32_t lb, lt;
+  dwork w;
+} dpass;
+
+/* reuse scatter of level d: id-sorted nodes with their ancestors' rows */
+typedef struct {
+  uint32_t d;
+  uint32_t *node;
+  uint32_t *rid[ADATM_KMAXSCAT];
+  uint32_t *rowptr;
+  dwork w;
+} dscat;
+
+#define MAXPASS 32
+
 typedef struct {
   uint32_t memo_mode, b, nim;
   uint32_t order[MAXN];
@@ -45,13 +62,12 @@ typedef struct {
   void *vals;
   uint32_t *ids[MAXN];
   uint32_t *ptr[MAXN];
-  uint32_t *par[MAXN];
-  dwork red[MAXN];
   void *memo[MAXN];
-  uint32_t *sperm[MAXN];
-  uint32_t *rowptr[MAXN];
-  dwork scat[MAXN];
-  uint32_t pmax; /* deepest level with a parent array */
+  dpass pass[MAXPASS];
+  uint32_t npass;
+  dscat scat[MAXN]; /* by level */
+  uint32_t *orow;   /* root node -> output row; NULL when they coincide */
+  int zero_root;    /* output rows without a root node: zero M first */
 } dgroup;
 
 enum { KC_LEAF = 0, KC_LEVEL = 1, KC_SCATTER = 2, KC_DENSE = 3, KC_NCLASS = 4 };
@@ -136,10 +152,15 @@ static int dupload(adatm_engine *e, void **p, void const *h, size_t bytes)
   return SPLATT_SUCCESS;
 }
 
-static int upload_work(adatm_engine *e, adatm_segwork const *w, dwork *d)
+static int upload_work(adatm_engine *e, adatm_segwork const *w,
+                       uint32_t const *anc, size_t nanc, dwork *d)
 {
   int rc;
   memset(d, 0, sizeof(*d));
+  if (anc && nanc &&
+      (rc = dupload(e, (void **)&d->kw.anc, anc, nanc * sizeof(uint32_t))) !=
+          SPLATT_SUCCESS)
+    return rc;
   d->kw.nitems = w->nitems;
   d->kw.nheavy = w->nheavy;
   d->nslots = w->nslots;
@@ -165,6 +186,7 @@ static int upload_work(adatm_engine *e, adatm_segwork const *w, dwork *d)
 
 static void free_work(dwork *d)
 {
+  hipFree((void *)d->kw.anc);
   hipFree((void *)d->kw.start);
   hipFree((void *)d->kw.seg);
   hipFree((void *)d->kw.slot);
@@ -199,87 +221,146 @@ static void ev_end(adatm_engine *e)
   e->nev++;
 }
 
-/* algorithmic (compulsory one-touch) bytes of one reduce pass into level
- * l-1 from level l; idx = 4 bytes on the device */
-static double bytes_reduce(adatm_engine const *e, dgroup const *g, uint32_t l)
+/* Compulsory one-touch bytes of a fused pass lb -> lt: bottom input, the
+ * structure of every level it walks, each factor once, the output once
+ * (4-byte indices on the device). */
+static double bytes_pass(adatm_engine const *e, dgroup const *g, uint32_t lb,
+                         uint32_t lt)
 {
   double const z = (double)e->vsz, w = (double)e->wsz, R = (double)e->R;
-  double const Fp = (double)g->F[l - 1], Fc = (double)g->F[l];
-  double const Iu = (double)e->dims[g->order[l]];
-  if (l == e->N - 1) /* leaf: values + leaf ids + parent ptr + factor + out */
-    return Fc * (z + 4.0) + (Fp + 1.0) * 4.0 + Iu * R * w + Fp * R * z;
-  return (Fp + 1.0) * 4.0 + Fc * 4.0 + Fc * R * z + Iu * R * w + Fp * R * z;
+  double b;
+  if (lb == e->N - 1) /* leaf values and leaf ids */
+    b = (double)g->F[lb] * (z + 4.0);
+  else /* memo rows and their node ids */
+    b = (double)g->F[lb] * (R * z + 4.0);
+  for (uint32_t l = lt + 1; l < lb; ++l) /* intermediate ids + pointers */
+    b += (double)g->F[l] * 4.0 + ((double)g->F[l] + 1.0) * 4.0;
+  b += ((double)g->F[lt] + 1.0) * 4.0;
+  for (uint32_t l = lt + 1; l <= lb; ++l)
+    b += (double)e->dims[g->order[l]] * R * w;
+  b += (double)g->F[lt] * R * z;
+  return b;
 }
 
 static double bytes_scatter(adatm_engine const *e, dgroup const *g, uint32_t d)
 {
   double const z = (double)e->vsz, w = (double)e->wsz, R = (double)e->R;
-  double b = (double)g->F[d] * R * z + (double)e->dims[g->order[d]] * R * z;
-  for (uint32_t l = 0; l < d; ++l)
-    b += ((double)g->F[l] + 1.0) * 4.0 + (double)g->F[l + 1] * 4.0 +
-         (double)e->dims[g->order[l]] * R * w;
+  double const Id = (double)e->dims[g->order[d]];
+  double const zrow = (d == e->N - 1) ? z : R * z;
+  double b = (double)g->F[d] * (zrow + 4.0 * (d + 1)) + (Id + 1.0) * 4.0 +
+             Id * R * z;
+  for (uint32_t l = 0; l < d; ++l) b += (double)e->dims[g->order[l]] * R * w;
   return b;
 }
 
 /* ---- one MTTKRP ----------------------------------------------------------- */
-static int run_reduce(adatm_engine *e, dgroup *g, uint32_t l, void const *in,
-                      int leaf, void *dst)
+static dpass *find_pass(dgroup *g, uint32_t lb, uint32_t lt)
 {
-  ev_begin(e, leaf ? KC_LEAF : KC_LEVEL, bytes_reduce(e, g, l));
-  KCHK(adatm_k_seg_reduce(e->stream, e->prec, leaf, &g->red[l].kw,
-                          g->ptr[l - 1], g->ids[l], in,
-                          gather_factor(e, g->order[l]), dst, e->carry, e->R),
-       "seg_reduce");
+  for (uint32_t i = 0; i < g->npass; ++i)
+    if (g->pass[i].lb == lb && g->pass[i].lt == lt) return g->pass + i;
+  return NULL;
+}
+
+static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
+                    void const *in, void *dst)
+{
+  dpass *P = find_pass(g, lb, lt);
+  if (!P) {
+    adatm_set_error("engine: no work list for pass %u -> %u", lb, lt);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  adatm_kchain ch;
+  memset(&ch, 0, sizeof(ch));
+  ch.K = lb - lt;
+  ch.orow = (lt == 0) ? g->orow : NULL;
+  for (uint32_t j = 0; j < ch.K; ++j) ch.fptr[j] = g->ptr[lt + j];
+  for (uint32_t j = 1; j <= ch.K; ++j) {
+    ch.ids[j] = g->ids[lt + j];
+    ch.U[j] = gather_factor(e, g->order[lt + j]);
+  }
+  int const leaf = (lb == e->N - 1);
+  ev_begin(e, leaf ? KC_LEAF : KC_LEVEL, bytes_pass(e, g, lb, lt));
+  KCHK(adatm_k_reduce(e->stream, e->prec, leaf, &P->w.kw, &ch, in, dst,
+                      e->carry, e->R),
+       "reduce");
   ev_end(e);
   return SPLATT_SUCCESS;
 }
 
-static int mttkrp_root(adatm_engine *e, dgroup *g)
+/* levels a fused pass may stop at walking up from lb: the next materialised
+ * level, at most ADATM_KMAXFUSE levels up */
+static uint32_t pass_top(dgroup const *g, uint32_t lb, uint32_t floor_lvl)
 {
-  uint32_t const N = e->N;
+  uint32_t lt = lb - 1;
+  while (lt > floor_lvl && !is_memo_level(g, lt) && lb - lt < ADATM_KMAXFUSE)
+    --lt;
+  return lt;
+}
+
+static int mttkrp_root(adatm_engine *e, dgroup *g, uint32_t mode)
+{
+  if (g->zero_root)
+    KCHK(adatm_k_zero(e->stream, e->M, e->dims[mode] * e->R * e->vsz),
+         "zero M");
   void const *in = g->vals;
-  int leaf = 1, pp = 0;
-  for (uint32_t l = N - 1; l >= 1; --l) {
+  int pp = 0;
+  uint32_t lb = e->N - 1;
+  while (lb > 0) {
+    uint32_t const lt = pass_top(g, lb, 0);
     void *dst;
-    if (l == 1)
+    if (lt == 0)
       dst = e->M;
-    else if (is_memo_level(g, l - 1))
-      dst = g->memo[l - 1];
+    else if (is_memo_level(g, lt))
+      dst = g->memo[lt];
     else {
       dst = e->tmp[pp];
       pp ^= 1;
     }
-    int const rc = run_reduce(e, g, l, in, leaf, dst);
+    int const rc = run_pass(e, g, lb, lt, in, dst);
     if (rc != SPLATT_SUCCESS) return rc;
     in = dst;
-    leaf = 0;
+    lb = lt;
   }
   return SPLATT_SUCCESS;
 }
 
-static int mttkrp_reuse(adatm_engine *e, dgroup *g, uint32_t d)
+static uint32_t reuse_source(dgroup const *g, uint32_t d)
 {
   uint32_t const bd = g->b - d;
   uint32_t const indeg = g->nim < bd ? g->nim : bd;
-  uint32_t const L = g->b - indeg;
-  void const *in = g->memo[L];
+  return g->b - indeg;
+}
+
+/* Non-root MTTKRP of level d: Z_d from the memo at level reuse_source
+ * (tag 1) or from the leaves (tag 2), then the level-d scatter. */
+static int mttkrp_level(adatm_engine *e, dgroup *g, uint32_t d, int tag)
+{
+  uint32_t const N = e->N;
+  uint32_t lb = (tag == 1) ? reuse_source(g, d) : N - 1;
+  void const *in = (tag == 1) ? g->memo[lb] : g->vals;
   int pp = 0;
-  for (uint32_t l = L; l > d; --l) {
+  while (lb > d) {
+    uint32_t lt = lb > d + ADATM_KMAXFUSE ? lb - ADATM_KMAXFUSE : d;
     void *dst = e->tmp[pp];
     pp ^= 1;
-    int const rc = run_reduce(e, g, l, in, 0, dst);
+    int const rc = run_pass(e, g, lb, lt, in, dst);
     if (rc != SPLATT_SUCCESS) return rc;
     in = dst;
+    lb = lt;
   }
-  adatm_kpath path;
-  memset(&path, 0, sizeof(path));
-  path.d = d;
-  for (uint32_t l = 1; l <= d; ++l) path.par[l] = g->par[l];
-  for (uint32_t l = 1; l < d; ++l) path.ids[l] = g->ids[l];
-  for (uint32_t l = 0; l < d; ++l) path.U[l] = gather_factor(e, g->order[l]);
+  dscat const *S = g->scat + d;
+  adatm_kscat sc;
+  memset(&sc, 0, sizeof(sc));
+  sc.d = d;
+  sc.node = S->node;
+  for (uint32_t l = 0; l < d; ++l) {
+    sc.rid[l] = S->rid[l];
+    sc.U[l] = gather_factor(e, g->order[l]);
+  }
+  int const zscalar = (d == N - 1); /* leaf level: Z = the values */
   ev_begin(e, KC_SCATTER, bytes_scatter(e, g, d));
-  KCHK(adatm_k_scatter(e->stream, e->prec, &g->scat[d].kw, g->rowptr[d],
-                       g->sperm[d], &path, in, e->M, e->carry, e->R),
+  KCHK(adatm_k_scatter(e->stream, e->prec, &S->w.kw, S->rowptr, &sc, in,
+                       zscalar, e->M, e->carry, e->R),
        "scatter");
   ev_end(e);
   return SPLATT_SUCCESS;
@@ -288,8 +369,9 @@ static int mttkrp_reuse(adatm_engine *e, dgroup *g, uint32_t d)
 static int mttkrp_dev(adatm_engine *e, uint32_t mode)
 {
   dgroup *g = e->g + e->use_grp[mode];
-  if (e->use_tag[mode] == 0) return mttkrp_root(e, g);
-  return mttkrp_reuse(e, g, e->level_of[e->use_grp[mode]][mode]);
+  if (e->use_tag[mode] == 0) return mttkrp_root(e, g, mode);
+  return mttkrp_level(e, g, e->level_of[e->use_grp[mode]][mode],
+                      (int)e->use_tag[mode]);
 }
 
 /* ---- construction --------------------------------------------------------- */
@@ -331,16 +413,106 @@ static int engine_alloc_common(adatm_engine *e)
   return SPLATT_SUCCESS;
 }
 
-/* deepest level that a reuse mode of this group scatters from */
-static uint32_t group_pmax(adatm_engine const *e, uint32_t gi)
+/* work list of the fused pass lb -> lt of a host CSF */
+static int build_pass(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
+                      uint32_t lb, uint32_t lt)
 {
-  uint32_t pm = 0;
-  for (uint32_t m = 0; m < e->N; ++m)
-    if (e->use_grp[m] == gi && e->use_tag[m] == 1) {
-      uint32_t const d = e->level_of[gi][m];
-      if (d > pm) pm = d;
+  if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
+  if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
+  uint64_t const np = h->F[lt];
+  uint32_t *cp = (uint32_t *)malloc((np + 1) * sizeof(uint32_t));
+  if (!cp) return SPLATT_ERROR_NOMEMORY;
+  /* first bottom descendant of each top node (fptr composition) */
+  #pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p <= (int64_t)np; ++p) {
+    uint64_t v = (uint64_t)p;
+    for (uint32_t l = lt; l < lb; ++l) v = h->fptr[l][v];
+    cp[p] = (uint32_t)v;
+  }
+  adatm_segwork w;
+  int rc = adatm_segwork_build(cp, np, RED_TILE, &w);
+  free(cp);
+  if (rc != SPLATT_SUCCESS) return rc;
+  uint32_t const K = lb - lt;
+  if (lt == 0 && g->orow)
+    for (uint32_t hh = 0; hh < w.nheavy; ++hh) w.hseg[hh] = h->fids[0][w.hseg[hh]];
+  uint32_t *anc = NULL;
+  if (K > 1) {
+    anc = (uint32_t *)malloc((size_t)w.nitems * (K - 1) * sizeof(uint32_t));
+    if (!anc) {
+      adatm_segwork_free(&w);
+      return SPLATT_ERROR_NOMEMORY;
     }
-  return pm;
+    #pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)w.nitems; ++k) {
+      uint32_t v = w.start[k];
+      for (uint32_t l = lb; l > lt + 1; --l) {
+        v = h->par[l][v];
+        anc[(size_t)k * (K - 1) + (l - 1 - lt) - 1] = v;
+      }
+    }
+  }
+  dpass *P = g->pass + g->npass;
+  P->lb = lb;
+  P->lt = lt;
+  rc = upload_work(e, &w, anc, anc ? (size_t)w.nitems * (K - 1) : 0, &P->w);
+  size_t const cb = (size_t)w.nslots * e->R * e->vsz;
+  if (cb > e->carry_bytes) e->carry_bytes = cb;
+  free(anc);
+  adatm_segwork_free(&w);
+  if (rc == SPLATT_SUCCESS) g->npass++;
+  return rc;
+}
+
+/* id-sorted level-d nodes with the factor-row ids of their ancestors */
+static int build_scatter(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
+                         uint32_t d)
+{
+  dscat *S = g->scat + d;
+  if (S->node) return SPLATT_SUCCESS;
+  if (d < 1 || d > ADATM_KMAXSCAT) {
+    adatm_set_error("engine: reuse level %u beyond %d", d, ADATM_KMAXSCAT);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  uint64_t const nrows = e->dims[g->order[d]];
+  uint64_t const Fd = h->F[d];
+  uint32_t *perm = (uint32_t *)malloc(Fd * sizeof(uint32_t));
+  uint32_t *rowptr = (uint32_t *)malloc((nrows + 1) * sizeof(uint32_t));
+  uint32_t *rid = (uint32_t *)malloc((size_t)d * Fd * sizeof(uint32_t));
+  int rc = SPLATT_ERROR_NOMEMORY;
+  adatm_segwork w;
+  memset(&w, 0, sizeof(w));
+  if (!perm || !rowptr || !rid) goto out;
+  rc = adatm_sort_by_id(h->fids[d], Fd, nrows, perm, rowptr);
+  if (rc != SPLATT_SUCCESS) goto out;
+  #pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < (int64_t)Fd; ++q) {
+    uint32_t v = perm[q];
+    for (uint32_t l = d; l >= 1; --l) {
+      v = h->par[l][v]; /* ancestor at level l-1 */
+      rid[(size_t)(l - 1) * Fd + q] = h->fids[l - 1][v];
+    }
+  }
+  rc = adatm_segwork_build(rowptr, nrows, SCAT_TILE, &w);
+  if (rc != SPLATT_SUCCESS) goto out;
+  S->d = d;
+  rc = dupload(e, (void **)&S->node, perm, Fd * sizeof(uint32_t));
+  for (uint32_t l = 0; rc == SPLATT_SUCCESS && l < d; ++l)
+    rc = dupload(e, (void **)&S->rid[l], rid + (size_t)l * Fd,
+                 Fd * sizeof(uint32_t));
+  if (rc == SPLATT_SUCCESS)
+    rc = dupload(e, (void **)&S->rowptr, rowptr, (nrows + 1) * sizeof(uint32_t));
+  if (rc == SPLATT_SUCCESS) rc = upload_work(e, &w, NULL, 0, &S->w);
+  {
+    size_t const cb = (size_t)w.nslots * e->R * e->vsz;
+    if (cb > e->carry_bytes) e->carry_bytes = cb;
+  }
+out:
+  adatm_segwork_free(&w);
+  free(perm);
+  free(rowptr);
+  free(rid);
+  return rc;
 }
 
 static int engine_add_group(adatm_engine *e, uint32_t gi, adatm_hcsf const *h)
@@ -375,61 +547,45 @@ static int engine_add_group(adatm_engine *e, uint32_t gi, adatm_hcsf const *h)
     if ((rc = dupload(e, (void **)&g->ptr[l], h->fptr[l],
                       (h->F[l] + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
       return rc;
-  g->pmax = group_pmax(e, gi);
-  for (uint32_t l = 1; l <= g->pmax; ++l)
-    if ((rc = dupload(e, (void **)&g->par[l], h->par[l],
-                      h->F[l] * sizeof(uint32_t))) != SPLATT_SUCCESS)
+  /* root ids: identity after empty-index compaction, else a row map */
+  {
+    int ident = (h->F[0] == e->dims[g->order[0]]);
+    for (uint64_t n = 0; ident && n < h->F[0]; ++n) ident = (h->fids[0][n] == n);
+    g->zero_root = (h->F[0] < e->dims[g->order[0]]);
+    if (!ident &&
+        (rc = dupload(e, (void **)&g->orow, h->fids[0],
+                      h->F[0] * sizeof(uint32_t))) != SPLATT_SUCCESS)
       return rc;
-  /* reduction work per level */
-  for (uint32_t l = 1; l < N; ++l) {
-    adatm_segwork w;
-    if ((rc = adatm_segwork_build(h->fptr[l - 1], h->F[l - 1], RED_TILE, &w)) !=
-        SPLATT_SUCCESS)
-      return rc;
-    rc = upload_work(e, &w, &g->red[l]);
-    size_t const cb = (size_t)w.nslots * R * e->vsz;
-    if (cb > e->carry_bytes) e->carry_bytes = cb;
-    adatm_segwork_free(&w);
-    if (rc != SPLATT_SUCCESS) return rc;
   }
   /* memo levels */
   for (uint32_t l = 1; l + 1 < N; ++l)
     if (is_memo_level(g, l) &&
         (rc = dalloc(e, &g->memo[l], g->F[l] * R * e->vsz)) != SPLATT_SUCCESS)
       return rc;
-  /* temporaries: every non-root level may be a temporary */
-  for (uint32_t l = 1; l + 1 < N; ++l) {
-    size_t const tb = (size_t)g->F[l] * R * e->vsz;
-    if (tb > e->tmp_bytes) e->tmp_bytes = tb;
-  }
-  /* reuse scatters */
-  for (uint32_t m = 0; m < N; ++m) {
-    if (e->use_grp[m] != gi || e->use_tag[m] != 1) continue;
-    uint32_t const d = e->level_of[gi][m];
-    uint64_t const nrows = e->dims[g->order[d]];
-    uint32_t *perm = (uint32_t *)malloc(g->F[d] * sizeof(uint32_t));
-    uint32_t *rowptr = (uint32_t *)malloc((nrows + 1) * sizeof(uint32_t));
-    if (!perm || !rowptr) {
-      free(perm);
-      free(rowptr);
-      return SPLATT_ERROR_NOMEMORY;
+  /* the root chain's passes */
+  for (uint32_t lb = N - 1; lb > 0;) {
+    uint32_t const lt = pass_top(g, lb, 0);
+    if ((rc = build_pass(e, g, h, lb, lt)) != SPLATT_SUCCESS) return rc;
+    if (lt > 0 && !is_memo_level(g, lt)) {
+      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
+      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
     }
-    rc = adatm_sort_by_id(h->fids[d], g->F[d], nrows, perm, rowptr);
-    adatm_segwork w;
-    memset(&w, 0, sizeof(w));
-    if (rc == SPLATT_SUCCESS) rc = adatm_segwork_build(rowptr, nrows, SCAT_TILE, &w);
-    if (rc == SPLATT_SUCCESS)
-      rc = dupload(e, (void **)&g->sperm[d], perm, g->F[d] * sizeof(uint32_t));
-    if (rc == SPLATT_SUCCESS)
-      rc = dupload(e, (void **)&g->rowptr[d], rowptr,
-                   (nrows + 1) * sizeof(uint32_t));
-    if (rc == SPLATT_SUCCESS) rc = upload_work(e, &w, &g->scat[d]);
-    size_t const cb = (size_t)w.nslots * R * e->vsz;
-    if (cb > e->carry_bytes) e->carry_bytes = cb;
-    adatm_segwork_free(&w);
-    free(perm);
-    free(rowptr);
-    if (rc != SPLATT_SUCCESS) return rc;
+    lb = lt;
+  }
+  /* non-root modes: propagation passes (memo or leaves) and scatters */
+  for (uint32_t m = 0; m < N; ++m) {
+    if (e->use_grp[m] != gi || e->use_tag[m] == 0) continue;
+    uint32_t const d = e->level_of[gi][m];
+    uint32_t lb = (e->use_tag[m] == 1) ? reuse_source(g, d) : N - 1;
+    while (lb > d) {
+      uint32_t const lt = lb > d + ADATM_KMAXFUSE
+                              ? lb - ADATM_KMAXFUSE : d;
+      if ((rc = build_pass(e, g, h, lb, lt)) != SPLATT_SUCCESS) return rc;
+      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
+      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
+      lb = lt;
+    }
+    if ((rc = build_scatter(e, g, h, d)) != SPLATT_SUCCESS) return rc;
   }
   return SPLATT_SUCCESS;
 }
@@ -447,7 +603,8 @@ static int engine_finish(adatm_engine *e)
 
 static int engine_begin(adatm_engine *e, uint32_t N, uint64_t const *dims,
                         uint64_t nnz, uint64_t rank, adatm_plan const *plan,
-                        int precision, int device)
+                        int precision, int device, uint32_t const *use_grp,
+                        uint32_t const *use_tag)
 {
   if (N < 3 || N > MAXN || rank < 1 || rank > 64 || plan->n_grp < 1 ||
       plan->n_grp > N) {
@@ -467,17 +624,31 @@ static int engine_begin(adatm_engine *e, uint32_t N, uint64_t const *dims,
   e->nnz = nnz;
   memcpy(e->dims, dims, N * sizeof(uint64_t));
   e->plan = *plan;
-  adatm_plan_use(plan, N, e->use_grp, e->use_tag);
+  if (use_grp && use_tag) {
+    memcpy(e->use_grp, use_grp, N * sizeof(uint32_t));
+    memcpy(e->use_tag, use_tag, N * sizeof(uint32_t));
+  } else {
+    adatm_plan_use(plan, N, e->use_grp, e->use_tag);
+  }
   for (uint32_t gi = 0; gi < plan->n_grp; ++gi)
     for (uint32_t l = 0; l < N; ++l) e->level_of[gi][plan->grp[gi].o_opt[l]] = l;
   /* validate the plan: each reuse level must sit above the group's memo */
   for (uint32_t m = 0; m < N; ++m) {
     uint32_t const gi = e->use_grp[m];
     adatm_group const *pg = plan->grp + gi;
+    if (gi >= plan->n_grp || e->use_tag[m] > 2) {
+      adatm_set_error("engine: bad use map for mode %u", m);
+      return SPLATT_ERROR_BADINPUT;
+    }
     if (e->use_tag[m] == 1) {
       uint32_t const d = e->level_of[gi][m];
       if (pg->n_imten == 0 || d < 1 || d >= pg->begin_imten) {
         adatm_set_error("engine: mode %u cannot reuse group %u's memo", m, gi);
+        return SPLATT_ERROR_BADINPUT;
+      }
+    } else if (e->use_tag[m] == 2) {
+      if (e->level_of[gi][m] < 1) {
+        adatm_set_error("engine: mode %u is group %u's root", m, gi);
         return SPLATT_ERROR_BADINPUT;
       }
     } else if (e->level_of[gi][m] != 0) {
@@ -507,11 +678,13 @@ int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
                              uint64_t const *dims, uint64_t nnz,
                              double ttnormsq, uint64_t rank,
                              adatm_plan const *plan, int precision, int device,
+                             uint32_t const *use_grp, uint32_t const *use_tag,
                              adatm_engine **out)
 {
   adatm_engine *e = (adatm_engine *)calloc(1, sizeof(adatm_engine));
   if (!e) return SPLATT_ERROR_NOMEMORY;
-  int rc = engine_begin(e, N, dims, nnz, rank, plan, precision, device);
+  int rc = engine_begin(e, N, dims, nnz, rank, plan, precision, device,
+                        use_grp, use_tag);
   for (uint32_t gi = 0; rc == SPLATT_SUCCESS && gi < plan->n_grp; ++gi)
     rc = engine_add_group(e, gi, groups + gi);
   if (rc == SPLATT_SUCCESS) rc = engine_finish(e);
@@ -527,27 +700,474 @@ int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
   return SPLATT_SUCCESS;
 }
 
+/* ======================================================================== *
+ * Construction on the device: sort, CSF levels, work lists, scatter tuples.
+ * ======================================================================== */
+
+/* coordinates resident on the device (indices u32, values FP64) */
+typedef struct {
+  uint32_t N;
+  uint64_t nnz;
+  uint64_t dims[MAXN];
+  uint32_t *ind[MAXN];
+  double *vals;
+} dcoo;
+
+static void dcoo_free(dcoo *c)
+{
+  for (uint32_t m = 0; m < MAXN; ++m) hipFree(c->ind[m]);
+  hipFree(c->vals);
+  memset(c, 0, sizeof(*c));
+}
+
+static int dcoo_upload(dcoo *c, adatm_coo const *t, int with_vals)
+{
+  memset(c, 0, sizeof(*c));
+  c->N = t->nmodes;
+  c->nnz = t->nnz;
+  memcpy(c->dims, t->dims, t->nmodes * sizeof(uint64_t));
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    HIPCHK(hipMalloc((void **)&c->ind[m], t->nnz * 4), "upload coo");
+    HIPCHK(hipMemcpy(c->ind[m], t->ind[m], t->nnz * 4, hipMemcpyHostToDevice),
+           "upload coo");
+  }
+  if (with_vals) {
+    HIPCHK(hipMalloc((void **)&c->vals, t->nnz * 8), "upload coo");
+    HIPCHK(hipMemcpy(c->vals, t->vals, t->nnz * 8, hipMemcpyHostToDevice),
+           "upload coo");
+  }
+  return SPLATT_SUCCESS;
+}
+
+static uint32_t nbits_for(uint64_t dim)
+{
+  uint32_t b = 0;
+  while (b < 32 && (1ull << b) < dim) ++b;
+  return b;
+}
+
+/* perm := stable lexicographic order of the coordinates by `order` */
+static int dcoo_lexsort(hipStream_t s, dcoo const *c, uint32_t const *order,
+                        uint32_t *perm, uint32_t *keys, uint32_t *stemp)
+{
+  uint64_t const n = c->nnz;
+  KCHK(adatm_k_iota(s, perm, n), "iota");
+  for (int64_t k = (int64_t)c->N - 1; k >= 0; --k) {
+    uint32_t const nb = nbits_for(c->dims[order[k]]);
+    if (nb == 0) continue;
+    KCHK(adatm_k_gather(s, keys, c->ind[order[k]], perm, n), "gather keys");
+    KCHK(adatm_k_sort_pairs(s, keys, perm, n, nb, stemp), "radix sort");
+  }
+  return SPLATT_SUCCESS;
+}
+
+static int d2h_u32(hipStream_t s, uint32_t const *dsrc, uint32_t *h)
+{
+  HIPCHK(hipMemcpyAsync(h, dsrc, 4, hipMemcpyDeviceToHost, s), "download");
+  HIPCHK(hipStreamSynchronize(s), "sync");
+  return SPLATT_SUCCESS;
+}
+
+/* exclusive scan with the total brought back to the host */
+static int dscan(hipStream_t s, uint32_t const *in, uint32_t *out, uint64_t n,
+                 uint32_t *stemp, uint32_t *tot_dev, uint32_t *total)
+{
+  KCHK(adatm_k_exscan(s, in, out, n, stemp, tot_dev), "scan");
+  return d2h_u32(s, tot_dev, total);
+}
+
+/* Fiber counts of the device coordinates under `order` */
+static int dcoo_count(hipStream_t s, dcoo const *c, uint32_t const *order,
+                      uint64_t *F)
+{
+  uint64_t const n = c->nnz;
+  uint32_t const N = c->N;
+  uint32_t *perm = NULL, *keys = NULL, *stemp = NULL, *flags = NULL,
+           *pos = NULL, *tot = NULL;
+  uint32_t *sc[MAXN] = {0};
+  uint8_t *fd = NULL;
+  int rc = SPLATT_ERROR_NOMEMORY;
+  uint64_t const tw = adatm_k_sort_temp_words(n) + adatm_k_scan_temp_words(n);
+  if (hipMalloc((void **)&perm, n * 4) || hipMalloc((void **)&keys, n * 4) ||
+      hipMalloc((void **)&stemp, tw * 4) || hipMalloc((void **)&fd, n) ||
+      hipMalloc((void **)&flags, n * 4) || hipMalloc((void **)&pos, n * 4) ||
+      hipMalloc((void **)&tot, 16))
+    goto out;
+  for (uint32_t l = 0; l < N; ++l)
+    if (hipMalloc((void **)&sc[l], n * 4)) goto out;
+  if ((rc = dcoo_lexsort(s, c, order, perm, keys, stemp)) != SPLATT_SUCCESS)
+    goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  for (uint32_t l = 0; l < N; ++l)
+    if (adatm_k_gather(s, sc[l], c->ind[order[l]], perm, n)) goto out;
+  if (adatm_k_first_diff(s, (uint32_t const *const *)sc, N, n, fd)) goto out;
+  for (uint32_t l = 0; l + 1 < N; ++l) {
+    uint32_t total = 0;
+    if (adatm_k_level_flags(s, fd, n, l, flags)) goto out;
+    if (dscan(s, flags, pos, n, stemp, tot, &total) != SPLATT_SUCCESS) goto out;
+    F[l] = total;
+  }
+  F[N - 1] = n;
+  rc = SPLATT_SUCCESS;
+out:
+  hipFree(perm); hipFree(keys); hipFree(stemp); hipFree(fd); hipFree(flags);
+  hipFree(pos); hipFree(tot);
+  for (uint32_t l = 0; l < N; ++l) hipFree(sc[l]);
+  if (rc != SPLATT_SUCCESS && !adatm_last_error()[0])
+    adatm_set_error("device fiber count failed");
+  return rc;
+}
+
+int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
+                                   uint32_t norders, uint64_t *F, int device)
+{
+  HIPCHK(hipSetDevice(device), "hipSetDevice");
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+  dcoo c;
+  int rc = dcoo_upload(&c, t, 0);
+  for (uint32_t k = 0; rc == SPLATT_SUCCESS && k < norders; ++k)
+    rc = dcoo_count(s, &c, orders + (size_t)k * t->nmodes, F + (size_t)k * t->nmodes);
+  dcoo_free(&c);
+  hipStreamDestroy(s);
+  return rc;
+}
+
+/* Work list of a segmented reduction over segments cp[0..nseg] on the device
+ * (light items start where the bucket cp/tile changes, long segments are cut
+ * into tile-sized pieces with carry slots). */
+static int dwork_build(adatm_engine *e, uint32_t const *cp, uint64_t nseg,
+                       uint32_t tile, dwork *d, uint32_t *stemp, uint32_t *tot)
+{
+  hipStream_t const s = e->stream;
+  memset(d, 0, sizeof(*d));
+  uint32_t *nitem = NULL, *npiece = NULL, *heavy = NULL, *ibase = NULL,
+           *sbase = NULL, *hbase = NULL;
+  int rc = SPLATT_ERROR_NOMEMORY;
+  uint32_t nitems = 0, nslots = 0, nheavy = 0;
+  uint64_t const ns = nseg ? nseg : 1;
+  if (hipMalloc((void **)&nitem, ns * 4) || hipMalloc((void **)&npiece, ns * 4) ||
+      hipMalloc((void **)&heavy, ns * 4) || hipMalloc((void **)&ibase, ns * 4) ||
+      hipMalloc((void **)&sbase, ns * 4) || hipMalloc((void **)&hbase, ns * 4))
+    goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  if (nseg) {
+    if (adatm_k_items_count(s, cp, nseg, tile, nitem, npiece, heavy)) goto out;
+    if (dscan(s, nitem, ibase, nseg, stemp, tot, &nitems) != SPLATT_SUCCESS) goto out;
+    if (dscan(s, npiece, sbase, nseg, stemp, tot, &nslots) != SPLATT_SUCCESS) goto out;
+    if (dscan(s, heavy, hbase, nseg, stemp, tot, &nheavy) != SPLATT_SUCCESS) goto out;
+  }
+  if ((rc = dalloc(e, (void **)&d->kw.start, (nitems + 1) * 4)) ||
+      (rc = dalloc(e, (void **)&d->kw.seg, (nitems + 1) * 4)) ||
+      (rc = dalloc(e, (void **)&d->kw.slot, (nitems + 1) * 4)) ||
+      (rc = dalloc(e, (void **)&d->kw.hslot, (nheavy + 1) * 4)))
+    goto out;
+  if (nheavy && (rc = dalloc(e, (void **)&d->kw.hseg, nheavy * 4))) goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  if (nseg &&
+      adatm_k_items_emit(s, cp, nseg, tile, nitem, ibase, sbase, heavy, hbase,
+                         (uint32_t *)d->kw.start, (uint32_t *)d->kw.seg,
+                         (uint32_t *)d->kw.slot, (uint32_t *)d->kw.hseg,
+                         (uint32_t *)d->kw.hslot))
+    goto out;
+  if (adatm_k_items_tail(s, cp, nseg, nitems, nheavy, nslots,
+                         (uint32_t *)d->kw.start, (uint32_t *)d->kw.seg,
+                         (uint32_t *)d->kw.slot, (uint32_t *)d->kw.hslot))
+    goto out;
+  d->kw.nitems = nitems;
+  d->kw.nheavy = nheavy;
+  d->nslots = nslots;
+  {
+    size_t const cb = (size_t)nslots * e->R * e->vsz;
+    if (cb > e->carry_bytes) e->carry_bytes = cb;
+  }
+  rc = SPLATT_SUCCESS;
+out:
+  hipFree(nitem); hipFree(npiece); hipFree(heavy); hipFree(ibase);
+  hipFree(sbase); hipFree(hbase);
+  return rc;
+}
+
+/* construction state of one group on the device */
+typedef struct {
+  uint32_t *par[MAXN];  /* par[l]: level l -> l-1 */
+  uint32_t *root_ids;   /* fids of level 0 */
+  uint32_t *stemp, *tot;
+} dbuild;
+
+static int build_pass_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t lb,
+                          uint32_t lt)
+{
+  if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
+  if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
+  hipStream_t const s = e->stream;
+  uint64_t const np = g->F[lt];
+  uint32_t *cp = NULL;
+  HIPCHK(hipMalloc((void **)&cp, (np + 1) * 4), "alloc cp");
+  HIPCHK(hipMemcpyAsync(cp, g->ptr[lt], (np + 1) * 4, hipMemcpyDeviceToDevice, s),
+         "copy cp");
+  for (uint32_t l = lt + 1; l < lb; ++l)
+    KCHK(adatm_k_compose(s, cp, np + 1, g->ptr[l]), "compose");
+  dpass *P = g->pass + g->npass;
+  P->lb = lb;
+  P->lt = lt;
+  int rc = dwork_build(e, cp, np, RED_TILE, &P->w, B->stemp, B->tot);
+  hipFree(cp);
+  if (rc != SPLATT_SUCCESS) return rc;
+  uint32_t const K = lb - lt;
+  if (lt == 0 && g->orow && P->w.kw.nheavy) {
+    /* split root parents are written through the root ids */
+    uint32_t *mapped = NULL;
+    HIPCHK(hipMalloc((void **)&mapped, (size_t)P->w.kw.nheavy * 4), "alloc map");
+    KCHK(adatm_k_gather(s, mapped, B->root_ids, P->w.kw.hseg, P->w.kw.nheavy),
+         "map heavy rows");
+    HIPCHK(hipMemcpyAsync((void *)P->w.kw.hseg, mapped, (size_t)P->w.kw.nheavy * 4,
+                          hipMemcpyDeviceToDevice, s), "copy map");
+    HIPCHK(hipStreamSynchronize(s), "map sync");
+    hipFree(mapped);
+  }
+  if (K > 1 && P->w.kw.nitems) {
+    if ((rc = dalloc(e, (void **)&P->w.kw.anc,
+                     (size_t)P->w.kw.nitems * (K - 1) * 4)) != SPLATT_SUCCESS)
+      return rc;
+    KCHK(adatm_k_items_anc(s, P->w.kw.start, P->w.kw.nitems, lb, lt,
+                           (uint32_t const *const *)B->par, (uint32_t *)P->w.kw.anc),
+         "item ancestors");
+  }
+  g->npass++;
+  return SPLATT_SUCCESS;
+}
+
+static int build_scatter_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t d)
+{
+  dscat *S = g->scat + d;
+  if (S->node) return SPLATT_SUCCESS;
+  if (d < 1 || d > ADATM_KMAXSCAT) {
+    adatm_set_error("engine: reuse level %u beyond %d", d, ADATM_KMAXSCAT);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  hipStream_t const s = e->stream;
+  uint64_t const nrows = e->dims[g->order[d]];
+  uint64_t const Fd = g->F[d];
+  int rc;
+  uint32_t *keys = NULL, *sortmp = NULL;
+  HIPCHK(hipMalloc((void **)&keys, Fd * 4), "alloc keys");
+  HIPCHK(hipMalloc((void **)&sortmp, adatm_k_sort_temp_words(Fd) * 4), "alloc sort");
+  HIPCHK(hipMemcpyAsync(keys, g->ids[d], Fd * 4, hipMemcpyDeviceToDevice, s),
+         "copy ids");
+  S->d = d;
+  if ((rc = dalloc(e, (void **)&S->node, Fd * 4)) != SPLATT_SUCCESS) return rc;
+  KCHK(adatm_k_iota(s, S->node, Fd), "iota");
+  KCHK(adatm_k_sort_pairs(s, keys, S->node, Fd, nbits_for(nrows), sortmp),
+       "sort by row");
+  if ((rc = dalloc(e, (void **)&S->rowptr, (nrows + 1) * 4)) != SPLATT_SUCCESS)
+    return rc;
+  KCHK(adatm_k_rowptr(s, keys, Fd, (uint32_t)nrows, S->rowptr), "rowptr");
+  uint32_t const *fids[MAXN] = {0};
+  fids[0] = B->root_ids;
+  for (uint32_t l = 1; l < d; ++l) fids[l] = g->ids[l];
+  for (uint32_t l = 0; l < d; ++l)
+    if ((rc = dalloc(e, (void **)&S->rid[l], Fd * 4)) != SPLATT_SUCCESS) return rc;
+  KCHK(adatm_k_scatter_rids(s, S->node, Fd, d, (uint32_t const *const *)B->par,
+                            fids, (uint32_t *const *)S->rid),
+       "scatter rids");
+  rc = dwork_build(e, S->rowptr, nrows, SCAT_TILE, &S->w, B->stemp, B->tot);
+  HIPCHK(hipStreamSynchronize(s), "scatter build sync");
+  hipFree(keys);
+  hipFree(sortmp);
+  return rc;
+}
+
+static int engine_add_group_dev(adatm_engine *e, uint32_t gi, dcoo const *c)
+{
+  uint32_t const N = e->N, R = e->R;
+  uint64_t const n = c->nnz;
+  hipStream_t const s = e->stream;
+  dgroup *g = e->g + gi;
+  adatm_group const *pg = e->plan.grp + gi;
+  g->memo_mode = pg->memo_mode;
+  g->b = pg->begin_imten;
+  g->nim = pg->n_imten;
+  memcpy(g->order, pg->o_opt, N * sizeof(uint32_t));
+  int rc = SPLATT_ERROR_NOMEMORY;
+  dbuild B;
+  memset(&B, 0, sizeof(B));
+  uint32_t *perm = NULL, *keys = NULL, *stemp = NULL;
+  uint32_t *sc[MAXN] = {0};
+  uint8_t *fd = NULL;
+  uint32_t *flags[2] = {0}, *pos[2] = {0}, *start[2] = {0};
+  uint64_t const tw = adatm_k_sort_temp_words(n) + adatm_k_scan_temp_words(n);
+  if (hipMalloc((void **)&perm, n * 4) || hipMalloc((void **)&keys, n * 4) ||
+      hipMalloc((void **)&stemp, tw * 4) || hipMalloc((void **)&fd, n) ||
+      hipMalloc((void **)&B.tot, 16))
+    goto out;
+  B.stemp = stemp;
+  for (uint32_t l = 0; l < N; ++l)
+    if (hipMalloc((void **)&sc[l], n * 4)) goto out;
+  for (int k = 0; k < 2; ++k)
+    if (hipMalloc((void **)&flags[k], n * 4) || hipMalloc((void **)&pos[k], n * 4))
+      goto out;
+  if ((rc = dcoo_lexsort(s, c, g->order, perm, keys, stemp)) != SPLATT_SUCCESS)
+    goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  for (uint32_t l = 0; l < N; ++l)
+    if (adatm_k_gather(s, sc[l], c->ind[g->order[l]], perm, n)) goto out;
+  if ((rc = dalloc(e, &g->vals, n * e->vsz)) != SPLATT_SUCCESS) goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  if (adatm_k_gather_vals(s, e->vsz == 8 ? KP_FP64 : KP_FP32, g->vals, c->vals,
+                          perm, n))
+    goto out;
+  if (adatm_k_first_diff(s, (uint32_t const *const *)sc, N, n, fd)) goto out;
+  /* levels 0 .. N-2, two at a time for pointers and parents */
+  for (uint32_t l = 0; l + 1 < N; ++l) {
+    int const cur = l & 1, prv = cur ^ 1;
+    uint32_t Fl = 0;
+    if (adatm_k_level_flags(s, fd, n, l, flags[cur])) goto out;
+    if (dscan(s, flags[cur], pos[cur], n, stemp, B.tot, &Fl) != SPLATT_SUCCESS)
+      goto out;
+    g->F[l] = Fl;
+    uint32_t *fids_l = NULL;
+    if (l == 0) {
+      if (hipMalloc((void **)&B.root_ids, (size_t)Fl * 4)) goto out;
+      fids_l = B.root_ids;
+    } else {
+      if ((rc = dalloc(e, (void **)&g->ids[l], (size_t)Fl * 4)) != SPLATT_SUCCESS)
+        goto out;
+      rc = SPLATT_ERROR_DEVICE;
+      fids_l = g->ids[l];
+    }
+    hipFree(start[cur]);
+    start[cur] = NULL;
+    if (hipMalloc((void **)&start[cur], (size_t)Fl * 4)) goto out;
+    if (adatm_k_level_emit(s, flags[cur], pos[cur], sc[l], n, fids_l, start[cur]))
+      goto out;
+    if (l >= 1) {
+      /* fptr of level l-1 (into level l) and parents of level l */
+      if ((rc = dalloc(e, (void **)&g->ptr[l - 1], (g->F[l - 1] + 1) * 4)) !=
+          SPLATT_SUCCESS)
+        goto out;
+      rc = SPLATT_ERROR_DEVICE;
+      if (adatm_k_fptr(s, start[prv], g->F[l - 1], pos[cur], Fl, g->ptr[l - 1]))
+        goto out;
+      if (hipMalloc((void **)&B.par[l], (size_t)Fl * 4)) goto out;
+      if (adatm_k_parent(s, start[cur], Fl, flags[prv], pos[prv], B.par[l]))
+        goto out;
+    }
+  }
+  {
+    /* leaf level: nodes are the sorted positions */
+    uint32_t const lp = N - 2;
+    int const cur = lp & 1;
+    g->F[N - 1] = n;
+    g->ids[N - 1] = sc[N - 1];
+    sc[N - 1] = NULL;
+    e->dev_bytes += n * 4;
+    if ((rc = dalloc(e, (void **)&g->ptr[lp], (g->F[lp] + 1) * 4)) != SPLATT_SUCCESS)
+      goto out;
+    rc = SPLATT_ERROR_DEVICE;
+    if (adatm_k_fptr(s, start[cur], g->F[lp], NULL, (uint32_t)n, g->ptr[lp])) goto out;
+    if (hipMalloc((void **)&B.par[N - 1], n * 4)) goto out;
+    if (adatm_k_parent(s, NULL, n, flags[cur], pos[cur], B.par[N - 1])) goto out;
+  }
+  /* root ids: identity exactly when every root index is occupied */
+  g->zero_root = (g->F[0] < e->dims[g->order[0]]);
+  if (g->zero_root) {
+    g->orow = B.root_ids;
+    B.root_ids = NULL;
+    e->dev_bytes += g->F[0] * 4;
+  }
+  /* free the sort's big temporaries before the passes */
+  hipFree(keys); keys = NULL;
+  hipFree(perm); perm = NULL;
+  hipFree(fd); fd = NULL;
+  for (uint32_t l = 0; l < N; ++l) {
+    hipFree(sc[l]);
+    sc[l] = NULL;
+  }
+  if (!B.root_ids) B.root_ids = g->orow;
+  /* memo levels */
+  for (uint32_t l = 1; l + 1 < N; ++l)
+    if (is_memo_level(g, l) &&
+        (rc = dalloc(e, &g->memo[l], g->F[l] * R * e->vsz)) != SPLATT_SUCCESS)
+      goto out;
+  /* root chain */
+  for (uint32_t lb = N - 1; lb > 0;) {
+    uint32_t const lt = pass_top(g, lb, 0);
+    if ((rc = build_pass_dev(e, g, &B, lb, lt)) != SPLATT_SUCCESS) goto out;
+    if (lt > 0 && !is_memo_level(g, lt)) {
+      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
+      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
+    }
+    lb = lt;
+  }
+  /* non-root modes */
+  for (uint32_t m = 0; m < N; ++m) {
+    if (e->use_grp[m] != gi || e->use_tag[m] == 0) continue;
+    uint32_t const d = e->level_of[gi][m];
+    uint32_t lb = (e->use_tag[m] == 1) ? reuse_source(g, d) : N - 1;
+    while (lb > d) {
+      uint32_t const lt = lb > d + ADATM_KMAXFUSE ? lb - ADATM_KMAXFUSE : d;
+      if ((rc = build_pass_dev(e, g, &B, lb, lt)) != SPLATT_SUCCESS) goto out;
+      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
+      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
+      lb = lt;
+    }
+    if ((rc = build_scatter_dev(e, g, &B, d)) != SPLATT_SUCCESS) goto out;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    rc = hip_fail(hipGetLastError(), "group build");
+    goto out;
+  }
+  rc = SPLATT_SUCCESS;
+out:
+  hipStreamSynchronize(s);
+  hipFree(perm); hipFree(keys); hipFree(stemp); hipFree(fd); hipFree(B.tot);
+  for (uint32_t l = 0; l < N; ++l) hipFree(sc[l]);
+  for (int k = 0; k < 2; ++k) {
+    hipFree(flags[k]);
+    hipFree(pos[k]);
+    hipFree(start[k]);
+  }
+  for (uint32_t l = 0; l < MAXN; ++l) hipFree(B.par[l]);
+  if (B.root_ids != g->orow) hipFree(B.root_ids);
+  if (rc != SPLATT_SUCCESS && !adatm_last_error()[0])
+    adatm_set_error("device group build failed (%d)", rc);
+  return rc;
+}
+
 int adatm_engine_create(adatm_coo const *t, uint64_t rank,
                         adatm_plan const *plan, int precision, int device,
                         adatm_engine **out)
 {
   *out = NULL;
   uint32_t const N = t->nmodes;
-  if (N < 3 || N > MAXN || t->nnz == 0) return SPLATT_ERROR_BADINPUT;
-  adatm_hcsf *h = (adatm_hcsf *)calloc(plan->n_grp, sizeof(adatm_hcsf));
-  if (!h) return SPLATT_ERROR_NOMEMORY;
-  int rc = SPLATT_SUCCESS;
+  if (N < 3 || N > MAXN || t->nnz == 0 || t->nnz >= (1ull << 32))
+    return SPLATT_ERROR_BADINPUT;
+  adatm_set_error("%s", "");
+  adatm_engine *e = (adatm_engine *)calloc(1, sizeof(adatm_engine));
+  if (!e) return SPLATT_ERROR_NOMEMORY;
+  int rc = engine_begin(e, N, t->dims, t->nnz, rank, plan, precision, device,
+                        NULL, NULL);
+  dcoo c;
+  memset(&c, 0, sizeof(c));
+  if (rc == SPLATT_SUCCESS) rc = dcoo_upload(&c, t, 1);
   for (uint32_t gi = 0; rc == SPLATT_SUCCESS && gi < plan->n_grp; ++gi)
-    rc = adatm_hcsf_build(t, plan->grp[gi].o_opt, h + gi);
+    rc = engine_add_group_dev(e, gi, &c);
+  dcoo_free(&c);
+  if (rc == SPLATT_SUCCESS) rc = engine_finish(e);
   double ss = 0.0;
   #pragma omp parallel for reduction(+ : ss)
   for (int64_t j = 0; j < (int64_t)t->nnz; ++j) ss += t->vals[j] * t->vals[j];
-  if (rc == SPLATT_SUCCESS)
-    rc = adatm_engine_create_hcsf(h, N, t->dims, t->nnz, ss, rank, plan,
-                                  precision, device, out);
-  for (uint32_t gi = 0; gi < plan->n_grp; ++gi) adatm_hcsf_free(h + gi);
-  free(h);
-  return rc;
+  e->ttnormsq = ss;
+  if (rc != SPLATT_SUCCESS) {
+    char msg[512];
+    snprintf(msg, sizeof(msg), "%s", adatm_last_error());
+    adatm_engine_destroy(e);
+    adatm_set_error("%s", msg);
+    return rc;
+  }
+  *out = e;
+  return SPLATT_SUCCESS;
 }
 
 void adatm_engine_destroy(adatm_engine *e)
@@ -564,13 +1184,15 @@ void adatm_engine_destroy(adatm_engine *e)
     for (uint32_t l = 0; l < MAXN; ++l) {
       hipFree(g->ids[l]);
       hipFree(g->ptr[l]);
-      hipFree(g->par[l]);
       hipFree(g->memo[l]);
-      hipFree(g->sperm[l]);
-      hipFree(g->rowptr[l]);
-      free_work(&g->red[l]);
-      free_work(&g->scat[l]);
+      dscat *S = g->scat + l;
+      hipFree(S->node);
+      hipFree(S->rowptr);
+      for (uint32_t k = 0; k < ADATM_KMAXSCAT; ++k) hipFree(S->rid[k]);
+      free_work(&S->w);
     }
+    for (uint32_t i = 0; i < g->npass; ++i) free_work(&g->pass[i].w);
+    hipFree(g->orow);
   }
   for (uint32_t m = 0; m < MAXN; ++m) {
     hipFree(e->A[m]);
@@ -910,22 +1532,27 @@ int adatm_engine_stats(adatm_engine *e, uint64_t *nfibs, double *seq_bytes,
   double B = 0.0, Fl = 0.0;
   double const R = (double)e->R;
   for (uint32_t m = 0; m < N; ++m) {
-    dgroup const *g = e->g + e->use_grp[m];
+    dgroup *g = e->g + e->use_grp[m];
+    uint32_t lb, stop;
     if (e->use_tag[m] == 0) {
-      for (uint32_t l = N - 1; l >= 1; --l) {
-        B += bytes_reduce(e, g, l);
-        Fl += 2.0 * R * (double)g->F[l];
-      }
+      lb = N - 1;
+      stop = 0;
     } else {
-      uint32_t const d = e->level_of[e->use_grp[m]][m];
-      uint32_t const bd = g->b - d;
-      uint32_t const L = g->b - (g->nim < bd ? g->nim : bd);
-      for (uint32_t l = L; l > d; --l) {
-        B += bytes_reduce(e, g, l);
-        Fl += 2.0 * R * (double)g->F[l];
-      }
-      B += bytes_scatter(e, g, d);
-      Fl += R * (double)(d + 1) * (double)g->F[d];
+      stop = e->level_of[e->use_grp[m]][m];
+      lb = (e->use_tag[m] == 1) ? reuse_source(g, stop) : N - 1;
+    }
+    while (lb > stop) {
+      uint32_t const lt = (e->use_tag[m] == 0)
+                              ? pass_top(g, lb, 0)
+                              : (lb > stop + ADATM_KMAXFUSE
+                                     ? lb - ADATM_KMAXFUSE : stop);
+      B += bytes_pass(e, g, lb, lt);
+      for (uint32_t l = lt + 1; l <= lb; ++l) Fl += 2.0 * R * (double)g->F[l];
+      lb = lt;
+    }
+    if (e->use_tag[m] != 0) {
+      B += bytes_scatter(e, g, stop);
+      Fl += R * (double)(stop + 1) * (double)g->F[stop];
     }
   }
   if (seq_bytes) *seq_bytes = B;
@@ -974,3 +1601,87 @@ uint32_t adatm_engine_nmodes(adatm_engine const *e) { return e->N; }
 uint64_t adatm_engine_dim(adatm_engine const *e, uint32_t m) { return e->dims[m]; }
 uint32_t adatm_engine_rank(adatm_engine const *e) { return e->R; }
 int adatm_engine_precision(adatm_engine const *e) { return e->prec; }
+
+/* Time nsteps back-to-back repetitions (what = 0 MTTKRP sequence, 1 ALS
+ * sweep) between two events on the engine's stream, synchronised before and
+ * after; *ms_total is the elapsed device time. */
+int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
+                           int use_graph, double *ms_total)
+{
+  if (!e || nsteps == 0) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(e->device), "hipSetDevice");
+  int rc;
+  if (what == 1) {
+    rc = cpd_prepare(e);
+    if (rc != SPLATT_SUCCESS) return rc;
+    if (use_graph) {
+      rc = graph_build(e);
+      if (rc != SPLATT_SUCCESS) return rc;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(e->stream), "pre sync");
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a), "event");
+  HIPCHK(hipEventCreate(&b), "event");
+  hipEventRecord(a, e->stream);
+  rc = SPLATT_SUCCESS;
+  for (uint32_t i = 0; i < nsteps && rc == SPLATT_SUCCESS; ++i) {
+    if (what == 1 && i + 1 >= e->fits_cap) break;
+    rc = (what == 0) ? seq_enqueue(e) : sweep_launch(e, use_graph);
+  }
+  hipEventRecord(b, e->stream);
+  hipError_t const s = hipEventSynchronize(b);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc != SPLATT_SUCCESS) return rc;
+  HIPCHK(s, "timed sync");
+  *ms_total = ms;
+  return SPLATT_SUCCESS;
+}
+
+/* fits of the sweeps run since the last CP-ALS preparation */
+int adatm_engine_get_fits(adatm_engine *e, uint32_t n, double *fits)
+{
+  if (!e || n > e->fits_cap) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(e->device), "hipSetDevice");
+  HIPCHK(hipStreamSynchronize(e->stream), "sync");
+  HIPCHK(hipMemcpy(fits, e->fits, n * sizeof(double), hipMemcpyDeviceToHost),
+         "fits download");
+  return SPLATT_SUCCESS;
+}
+
+/* Stepwise CP-ALS for callers that report every sweep: begin computes the
+ * Grams of the current factors and resets the sweep counter; each sweep runs
+ * synchronously and returns its fit. */
+int adatm_engine_cpd_begin(adatm_engine *e)
+{
+  if (!e) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(e->device), "hipSetDevice");
+  int const rc = cpd_prepare(e);
+  if (rc != SPLATT_SUCCESS) return rc;
+  HIPCHK(hipStreamSynchronize(e->stream), "cpd begin sync");
+  return SPLATT_SUCCESS;
+}
+
+int adatm_engine_cpd_sweep(adatm_engine *e, uint32_t it, int use_graph,
+                           double *fit)
+{
+  if (!e || it >= e->fits_cap) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(e->device), "hipSetDevice");
+  int const rc = sweep_launch(e, use_graph);
+  if (rc != SPLATT_SUCCESS) return rc;
+  HIPCHK(hipMemcpyAsync(fit, e->fits + it, sizeof(double),
+                        hipMemcpyDeviceToHost, e->stream),
+         "fit download");
+  HIPCHK(hipStreamSynchronize(e->stream), "sweep sync");
+  return SPLATT_SUCCESS;
+}
+
+uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
+                                 uint32_t level)
+{
+  if (!e || grp >= e->plan.n_grp || level >= e->N) return 0;
+  return e->g[grp].memo[level] ? e->g[grp].F[level] * e->R * e->vsz : 0;
+}

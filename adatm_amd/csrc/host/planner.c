@@ -30,6 +30,8 @@ int adatm_count_fibers_host(adatm_coo const *t, uint32_t const *order,
                             uint64_t *nfibs);
 int adatm_count_fibers_device(adatm_coo const *t, uint32_t const *order,
                               uint64_t *nfibs, int device);
+int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
+                                   uint32_t norders, uint64_t *F, int device);
 
 /* ---- quicksort of (mode, dim) pairs; tie order as adatm_sort.c:10-67 ---- */
 static void pair_qsort(pair *a, size_t lo, size_t hi)
@@ -320,6 +322,37 @@ static int predict(adatm_coo const *t, uint64_t R, adatm_plan *cands, int nc,
   fcache_ent *cache = (fcache_ent *)malloc(64 * sizeof(fcache_ent));
   int ncache = 0;
   if (!cache) return SPLATT_ERROR_NOMEMORY;
+  if (device >= 0) {
+    /* every order the model will read, counted with one device upload */
+    uint32_t *orders = (uint32_t *)malloc(64 * N * sizeof(uint32_t));
+    uint32_t no = 0, old = 0;
+    for (int c = 0; c < nc && orders; ++c) {
+      int const refresh = (c == 0) || !quirks || (cands[c].n_grp != old);
+      old = cands[c].n_grp;
+      if (!refresh) continue;
+      for (uint32_t g = 0; g < cands[c].n_grp && no < 64; ++g) {
+        int dup = 0;
+        for (uint32_t k = 0; k < no && !dup; ++k)
+          dup = !memcmp(orders + k * N, cands[c].grp[g].o_opt, N * sizeof(uint32_t));
+        if (!dup) memcpy(orders + (no++) * N, cands[c].grp[g].o_opt, N * sizeof(uint32_t));
+      }
+    }
+    uint64_t *F = (uint64_t *)malloc((size_t)(no ? no : 1) * N * sizeof(uint64_t));
+    int rc = (orders && F) ? adatm_count_fibers_device_many(t, orders, no, F, device)
+                           : SPLATT_ERROR_NOMEMORY;
+    if (rc == SPLATT_SUCCESS)
+      for (uint32_t k = 0; k < no; ++k) {
+        memcpy(cache[k].order, orders + k * N, N * sizeof(uint32_t));
+        memcpy(cache[k].F, F + (size_t)k * N, N * sizeof(uint64_t));
+      }
+    if (rc == SPLATT_SUCCESS) ncache = (int)no;
+    free(orders);
+    free(F);
+    if (rc != SPLATT_SUCCESS) {
+      free(cache);
+      return rc;
+    }
+  }
   uint64_t last_F[ADATM_MAX_NMODES] = {0};
   uint64_t csf_bytes = 0, csf_ops = 0;
   uint32_t old_ngrp = 0;

@@ -169,6 +169,25 @@ int adatm_engine_cpd(adatm_engine *e, uint32_t niters, double tol,
 int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
                       int use_graph, double *ms_median, double *per_kernel);
 
+/* nsteps back-to-back MTTKRP sequences (what = 0) or ALS sweeps (what = 1,
+ * after fresh Grams), bracketed by stream synchronisation; *ms_total is the
+ * device time between two events around them. */
+int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
+                           int use_graph, double *ms_total);
+
+/* Stepwise CP-ALS: begin (Grams of the current factors, counter reset),
+ * then one synchronous sweep at a time returning its fit. */
+int adatm_engine_cpd_begin(adatm_engine *e);
+int adatm_engine_cpd_sweep(adatm_engine *e, uint32_t it, int use_graph,
+                           double *fit);
+
+/* Device bytes of a group's memo at a CSF level (0 when not memoised). */
+uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
+                                 uint32_t level);
+
+/* fits of the first n sweeps since the last CP-ALS start */
+int adatm_engine_get_fits(adatm_engine *e, uint32_t n, double *fits);
+
 /* Structure statistics: nfibs per level of each group (n_grp x N, row-major),
  * algorithmic (compulsory one-touch) bytes and flops of one MTTKRP
  * sequence, and device bytes held. */

@@ -1,0 +1,93 @@
+"""Host-side construction (CPU only): canonical form, generators, exports."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import adatm_amd as A
+from oracle import npref as O
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.mark.parametrize("dist,vals", [(A.DIST_UNIFORM, A.VALS_U01), (A.DIST_ZIPF, A.VALS_LOGNORMAL),
+                                       (A.DIST_CLUSTER, A.VALS_PM1)])
+def test_canonicalize_bit_exact(dist, vals):
+    spec = A.make_spec([40, 30, 20, 9, 7], 20000, dist=dist, zipf_s=1.3, nclusters=30,
+                       sigma=2.5, values=vals, seed=17)
+    t = A.Tensor.synth(spec)
+    inds0, vals0 = t.inds().astype(np.int64), t.vals.copy()
+    maps = t.canonicalize()
+    ui, uv, nd, om = O.canonicalize(inds0, vals0, [40, 30, 20, 9, 7])
+    assert t.dims == nd
+    assert np.array_equal(t.inds(), ui)
+    assert np.array_equal(t.vals, uv)  # duplicates summed in draw order, bit-exact
+    for a, b in zip(maps, om):
+        assert np.array_equal(a, b)
+
+
+def test_canonicalize_edge_cases():
+    # all nonzeros identical -> one merged entry; a mode of length one
+    inds = np.zeros((3, 50), dtype=np.int64)
+    inds[1] = 4
+    vals = np.arange(50, dtype=np.float64) * 0.1
+    t = A.Tensor.from_arrays(inds, vals, [1, 9, 3])
+    t.canonicalize()
+    assert t.nnz == 1 and t.dims == [1, 1, 1]
+    acc = 0.0
+    for v in vals:
+        acc += v
+    assert t.vals[0] == acc
+    # out-of-range index is rejected
+    bad = A.Tensor.from_arrays(np.array([[0, 5], [0, 0], [0, 0]]), np.ones(2), [3, 2, 2])
+    with pytest.raises(A.AdatmError):
+        bad.canonicalize()
+
+
+def test_generator_is_counter_based():
+    """A prefix of the draw stream does not depend on the number of draws."""
+    a = A.Tensor.synth(A.named_spec(2, 5000))
+    b = A.Tensor.synth(A.named_spec(2, 2000))
+    assert np.array_equal(a.inds()[:, :2000], b.inds())
+    assert np.array_equal(a.vals[:2000], b.vals)
+
+
+def test_zipf_frequencies():
+    spec = A.make_spec([50, 50, 50], 200000, dist=A.DIST_ZIPF, zipf_s=1.1, seed=3)
+    t = A.Tensor.synth(spec)
+    k = np.arange(1, 51)
+    p = k ** -1.1
+    p /= p.sum()
+    f = np.bincount(t.ind(0), minlength=50) / t.nnz
+    assert np.max(np.abs(f - p)) < 0.004
+
+
+def test_rand_matrix_range_and_determinism():
+    a = A.rand_matrix(5, 2, 100, 16)
+    b = A.rand_matrix(5, 2, 100, 16)
+    assert np.array_equal(a, b)
+    assert a.min() >= 0.0 and a.max() < 1.0
+    assert not np.array_equal(a, A.rand_matrix(5, 3, 100, 16))
+
+
+def _declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        txt = re.sub(r"//.*", "", txt)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
+            name = m.group(1)
+            if name in ("if", "while", "for", "return", "sizeof", "defined"):
+                continue
+            names.add(name)
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(A.LIB_PATH))
+    declared = _declared_functions()
+    assert len(declared) > 40
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing

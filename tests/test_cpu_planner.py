@@ -1,0 +1,157 @@
+"""Planner known answers (CPU only).
+
+The expected plans below were derived by hand from src/adatm_bench.c:276-458
+and src/adatm_sort.c:10-67 (candidate groups per order N, level orders at the
+three named workloads, quicksort tie permutations) -- independent of both the
+library's C planner and the oracle's Python transliteration, which are
+checked against them and against each other.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import adatm_amd as A
+from oracle import npref as O
+
+# (memo_mode, begin_imten, n_imten) per group, candidates in order
+APPENDIX_A = {
+    3: [[(0, 2, 1), (2, 1, 0)]],
+    4: [[(0, 3, 2), (3, 1, 0)], [(0, 3, 1), (3, 1, 0)], [(0, 2, 1), (2, 2, 1)]],
+    5: [[(0, 4, 3), (4, 1, 0)], [(0, 4, 2), (4, 1, 0)], [(0, 4, 1), (4, 1, 0)],
+        [(0, 3, 2), (3, 2, 1)]],
+    6: [[(0, 5, k), (5, 1, 0)] for k in (4, 3, 2, 1)] +
+       [[(0, 3, 2), (3, 3, 2)], [(0, 3, 1), (3, 3, 1)],
+        [(0, 2, 1), (2, 2, 1), (4, 2, 1)]],
+    7: [[(0, 6, k), (6, 1, 0)] for k in (5, 4, 3, 2, 1)] +
+       [[(0, 4, 3), (4, 3, 2)], [(0, 4, 2), (4, 3, 1)],
+        [(0, 3, 2), (3, 2, 1), (5, 2, 1)]],
+    8: [[(0, 7, k), (7, 1, 0)] for k in (6, 5, 4, 3, 2, 1)] +
+       [[(0, 4, k), (4, 4, k)] for k in (3, 2, 1)] +
+       [[(0, 3, 2), (3, 3, 2), (6, 2, 1)],
+        [(0, 2, 1), (2, 2, 1), (4, 2, 1), (6, 2, 1)]],
+}
+
+# level orders at the named workloads' dims
+ORDERS_C1 = {0: [[0, 1, 2, 3], [3, 2, 1, 0]], 1: [[0, 1, 2, 3], [3, 2, 1, 0]],
+             2: [[0, 1, 3, 2], [2, 3, 1, 0]]}
+ORDERS_C2 = {c: [[0, 1, 2, 3, 4], [4, 3, 2, 1, 0]] for c in range(3)}
+ORDERS_C2[3] = [[0, 1, 2, 4, 3], [3, 4, 2, 1, 0]]
+ORDERS_C3 = {c: [[0, 1, 2, 3, 4, 5, 6, 7], [7, 4, 6, 1, 5, 2, 0, 3]] for c in range(6)}
+ORDERS_C3.update({c: [[0, 1, 2, 3, 4, 7, 5, 6], [4, 5, 6, 7, 0, 3, 1, 2]] for c in (6, 7, 8)})
+ORDERS_C3[9] = [[0, 1, 2, 7, 6, 4, 3, 5], [3, 4, 5, 2, 1, 7, 6, 0], [6, 7, 4, 0, 1, 5, 2, 3]]
+ORDERS_C3[10] = [[0, 1, 6, 2, 3, 7, 4, 5], [2, 3, 0, 4, 5, 1, 6, 7],
+                 [4, 5, 2, 6, 7, 3, 0, 1], [6, 7, 4, 0, 1, 5, 2, 3]]
+
+# pair_sort output when every key ties: position permutation per length
+APPENDIX_B = {1: [0], 2: [0, 1], 3: [2, 0, 1], 4: [0, 3, 1, 2], 5: [4, 3, 1, 0, 2],
+              6: [4, 0, 1, 5, 2, 3], 7: [4, 6, 1, 5, 2, 0, 3]}
+
+
+@pytest.mark.parametrize("N", sorted(APPENDIX_A))
+def test_candidate_groups(N):
+    dims = [1000 + 10 * m for m in range(N)]
+    got = [[(g[0], g[1], g[2]) for g in p.groups(N)] for p in A.plan_candidates(N, dims)]
+    assert got == APPENDIX_A[N]
+    ora = [[(g[0], g[1], g[2]) for g in p] for p in O.enumerate_plans(N, dims)]
+    assert ora == APPENDIX_A[N]
+
+
+@pytest.mark.parametrize("dims,orders", [
+    ([2000, 1500, 1000, 500], ORDERS_C1),
+    ([100000, 50000, 20000, 5000, 1000], ORDERS_C2),
+    ([1000] * 8, ORDERS_C3),
+])
+def test_level_orders_at_named_workloads(dims, orders):
+    N = len(dims)
+    cands = A.plan_candidates(N, dims)
+    for c, exp in orders.items():
+        assert [g[3] for g in cands[c].groups(N)] == exp, c
+    ora = O.enumerate_plans(N, dims)
+    for c, exp in orders.items():
+        assert [list(g[3]) for g in ora[c]] == exp, c
+
+
+@pytest.mark.parametrize("n", sorted(APPENDIX_B))
+def test_pair_sort_all_ties(n):
+    pairs = [[p, 7] for p in range(n)]
+    assert [x for x, _ in O.pair_sort(pairs)] == APPENDIX_B[n]
+    arr = (A.C.c_uint64 * (2 * n))()
+    for p in range(n):
+        arr[2 * p], arr[2 * p + 1] = p, 7
+    A.lib().pair_sort(arr, n)
+    assert [arr[2 * i] for i in range(n)] == APPENDIX_B[n]
+
+
+def test_pair_sort_random_ties_library_vs_transliteration():
+    rng = np.random.default_rng(0)
+    L = A.lib()
+    for _ in range(500):
+        n = int(rng.integers(1, 12))
+        ys = rng.integers(0, 4, n)
+        pairs = [[i, int(y)] for i, y in enumerate(ys)]
+        arr = (A.C.c_uint64 * (2 * n))()
+        for i, (x, y) in enumerate(pairs):
+            arr[2 * i], arr[2 * i + 1] = x, y
+        L.pair_sort(arr, n)
+        got = [[arr[2 * i], arr[2 * i + 1]] for i in range(n)]
+        assert got == O.pair_sort(pairs)
+        assert [y for _, y in got] == sorted(ys.tolist())
+
+
+def test_candidate_counts_formula():
+    """n_grp_cal = ceil(N / sqrt(2(N-1))) and the counts 1,3,4,7,8,11."""
+    for N, cnt in zip(range(3, 9), [1, 3, 4, 7, 8, 11]):
+        assert len(A.plan_candidates(N, [100] * N)) == cnt
+        assert math.ceil(N / math.sqrt(2 * (N - 1))) == (2 if N <= 6 else 3)
+
+
+def _small(seed, dims, nnz, dist):
+    spec = A.make_spec(dims, nnz, dist=dist, zipf_s=1.1, nclusters=40, sigma=2.0, seed=seed)
+    t = A.Tensor.synth(spec)
+    t.canonicalize()
+    return t
+
+
+@pytest.mark.parametrize("quirks", [True, False])
+@pytest.mark.parametrize("case", [([30, 25, 20, 15], 5000, 0), ([40, 30, 20, 10, 8], 8000, 1),
+                                  ([12] * 6, 6000, 2), ([8] * 8, 9000, 2)])
+def test_analytic_model_matches_oracle(case, quirks):
+    dims, nnz, dist = case
+    t = _small(3, dims, nnz, dist)
+    inds = t.inds().astype(np.int64)
+    R = 16
+    cands, ch = A.plan_analytic(t, R, ref_quirks=quirks)
+    ocfg = O.enumerate_plans(t.nmodes, t.dims)
+    pred = O.predict(ocfg, lambda o: O.count_fibers(inds, o), t.nnz, t.nmodes, R,
+                     quirks=quirks)
+    assert [(p.pspace, p.pops) for p in cands] == pred
+    assert ch == O.choose(pred)
+
+
+def test_fiber_counts_brute_force():
+    rng = np.random.default_rng(5)
+    t = _small(9, [9, 7, 5, 4], 600, 0)
+    inds = t.inds()
+    for order in ([0, 1, 2, 3], [3, 1, 0, 2], [2, 3, 1, 0]):
+        F = A.count_fibers(t, order)
+        brute = [len({tuple(inds[list(order[:l + 1]), j]) for j in range(t.nnz)})
+                 for l in range(4)]
+        assert F == brute == O.count_fibers(inds, order)
+    del rng
+
+
+def test_c1_analytic_plan_is_candidate_2():
+    """At C1's dims the model predicts pops in ratio 13.41 : 13.94 : 10.93
+    (closed form for uniform coordinates) and picks candidate 2."""
+    t = A.synth_tensor(1, 400_000)
+    cands, ch = A.plan_analytic(t, 16)
+    assert ch == 2
+    assert [g[3] for g in cands[2].groups(4)] == ORDERS_C1[2]
+
+
+def test_plan_use_map():
+    p = A.plan_candidates(8, [1000] * 8)[9]  # 3 groups, memo modes 0, 3, 6
+    grp, tag = A.plan_use(p, 8)
+    assert grp == [0, 0, 0, 1, 1, 1, 2, 2]
+    assert tag == [0, 1, 1, 0, 1, 1, 0, 1]

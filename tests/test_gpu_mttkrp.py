@@ -146,3 +146,44 @@ def test_cpd_fit_matches_oracle(prec, graph):
                           float((vals ** 2).sum()))
     tol = 1e-9 if prec == A.FP64 else 1e-4
     assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= tol, (fits, ref)
+
+
+def test_device_fiber_counts_match_host():
+    """Radix sort + boundary scan on the device give the host's (and the
+    oracle's) fiber counts, including keys that need three 8-bit digits."""
+    spec = A.make_spec([70000, 300, 5, 40], 300000, dist=A.DIST_ZIPF, zipf_s=0.7, seed=31)
+    t = A.Tensor.synth(spec)
+    t.canonicalize()
+    inds = t.inds().astype(np.int64)
+    for order in ([0, 1, 2, 3], [3, 2, 1, 0], [2, 0, 3, 1], [1, 3, 0, 2]):
+        dev = A.count_fibers(t, order, device=0)
+        assert dev == A.count_fibers(t, order, device=-1) == O.count_fibers(inds, order)
+
+
+def test_device_csf_large_bit_exact():
+    """Device-built CSFs of a 2M-draw tensor equal the oracle's structure."""
+    t = A.synth_tensor(1, 2_000_000)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    cands, ch = A.plan_analytic(t, 16, device=0)
+    e = A.Engine(t, 16, cands[ch], precision=A.FP64)
+    for g, grp in enumerate(cands[ch].groups(4)):
+        F, fptr, fids, gv = e.get_csf(g, t.nnz)
+        Fo, fpo, fio, vo, _ = O.csf(inds, vals, grp[3])
+        assert F == Fo
+        for l in range(3):
+            assert np.array_equal(fptr[l], fpo[l])
+        for l in range(1, 4):
+            assert np.array_equal(fids[l], fio[l])
+        assert np.array_equal(gv, vo)
+    U = [A.rand_matrix(4, m, d, 16) for m, d in enumerate(t.dims)]
+    for m in range(4):
+        e.set_factor(m, U[m])
+    for m in range(4):
+        assert O.rel_fro(e.mttkrp(m), O.coo_mttkrp(inds, vals, U, m)) <= 1e-12
+
+
+def test_planner_device_equals_host():
+    t = A.synth_tensor(2, 300_000)
+    a, ca = A.plan_analytic(t, 32, device=0)
+    b, cb = A.plan_analytic(t, 32, device=-1)
+    assert ca == cb and [(p.pspace, p.pops) for p in a] == [(p.pspace, p.pops) for p in b]

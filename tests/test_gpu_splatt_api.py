@@ -1,0 +1,166 @@
+"""The SPLATT-compatible C API (include/splatt.h) on the GPU, through ctypes.
+
+splatt_csf_convert / splatt_mttkrp / splatt_cpd_als with every CSF
+allocation scheme, compared with the oracle on the same coordinates and
+factors (relative Frobenius error <= 1e-12 in FP64), plus the `splatt` CLI.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import adatm_amd as A
+from oracle import npref as O
+
+pytestmark = pytest.mark.gpu
+
+MAXM = 88
+
+
+class Kruskal(C.Structure):
+    _fields_ = [("rank", C.c_uint64), ("factors", C.POINTER(C.c_double) * MAXM),
+                ("lambda_", C.POINTER(C.c_double)), ("nmodes", C.c_uint64),
+                ("dims", C.c_uint64 * MAXM), ("fit", C.c_double)]
+
+
+def _lib():
+    L = A.lib()
+    L.splatt_default_opts.restype = C.POINTER(C.c_double)
+    L.splatt_free_opts.argtypes = [C.POINTER(C.c_double)]
+    L.splatt_csf_convert.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.POINTER(C.c_uint64)),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_double)]
+    L.splatt_mttkrp.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p,
+                                C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]
+    L.splatt_cpd_als.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_double),
+                                 C.POINTER(Kruskal)]
+    L.splatt_free_csf.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.splatt_free_kruskal.argtypes = [C.POINTER(Kruskal)]
+    return L
+
+
+def _convert(L, inds, vals, opts):
+    N = inds.shape[0]
+    cols = [np.ascontiguousarray(inds[m], dtype=np.uint64) for m in range(N)]
+    ip = (C.POINTER(C.c_uint64) * N)(*[c.ctypes.data_as(C.POINTER(C.c_uint64)) for c in cols])
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    out = C.c_void_p()
+    rc = L.splatt_csf_convert(N, v.shape[0], ip, v.ctypes.data_as(C.POINTER(C.c_double)),
+                              C.byref(out), opts)
+    assert rc == 1, A.lib().adatm_last_error()
+    return out
+
+
+def _mttkrp(L, csf, mode, U, opts):
+    R = U[0].shape[1]
+    mats = (C.POINTER(C.c_double) * len(U))(*[u.ctypes.data_as(C.POINTER(C.c_double)) for u in U])
+    out = np.zeros((U[mode].shape[0], R))
+    rc = L.splatt_mttkrp(mode, R, csf, mats, out.ctypes.data_as(C.POINTER(C.c_double)), opts)
+    assert rc == 1, A.lib().adatm_last_error()
+    return out
+
+
+def _tensor(seed, dims, nnz, empty_slices=True, dups=True):
+    rng = np.random.default_rng(seed)
+    inds = np.stack([rng.integers(0, d, nnz) for d in dims]).astype(np.int64)
+    if empty_slices:  # leave holes in every mode
+        for m, d in enumerate(dims):
+            inds[m] = np.where(inds[m] % 5 == 3, (inds[m] + 1) % d, inds[m])
+    if dups:
+        inds[:, nnz // 2:nnz // 2 + 50] = inds[:, :50]
+    vals = rng.random(nnz)
+    return inds, vals
+
+
+@pytest.mark.parametrize("alloc", [A.ADAPTIVE_TIME_EFFICIENT, A.CSF_ONEMODE,
+                                   A.CSF_TWOMODE, A.CSF_ALLMODE])
+@pytest.mark.parametrize("dims", [[30, 25, 20], [40, 30, 20, 12], [20, 18, 16, 14, 12],
+                                  [9, 8, 9, 8, 9, 8, 9]])
+def test_splatt_mttkrp_all_allocations(alloc, dims):
+    L = _lib()
+    opts = L.splatt_default_opts()
+    opts[A.OPT_CSF_ALLOC] = alloc
+    inds, vals = _tensor(len(dims) * 7 + alloc, dims, 4000)
+    csf = _convert(L, inds, vals, opts)
+    R = 12
+    rng = np.random.default_rng(1)
+    U = [np.ascontiguousarray(rng.random((d, R))) for d in dims]
+    for m in range(len(dims)):
+        ref = O.coo_mttkrp(inds, vals, U, m)
+        err = O.rel_fro(_mttkrp(L, csf, m, U, opts), ref)
+        assert err <= 1e-12, (alloc, m, err)
+    L.splatt_free_csf(csf, opts)
+    L.splatt_free_opts(opts)
+
+
+def test_memo_revalidated_when_factors_change():
+    """Reuse modes called out of sequence, with factors changed in between,
+    still match the brute force (the memo is rebuilt when stale)."""
+    L = _lib()
+    opts = L.splatt_default_opts()
+    dims = [25, 20, 15, 10, 8]
+    inds, vals = _tensor(3, dims, 6000)
+    csf = _convert(L, inds, vals, opts)
+    R = 8
+    rng = np.random.default_rng(2)
+    U = [np.ascontiguousarray(rng.random((d, R))) for d in dims]
+    order = [2, 1, 0, 4, 3, 1, 2, 2, 4]
+    for k, m in enumerate(order):
+        if k % 2 == 1:
+            j = int(rng.integers(0, len(dims)))
+            U[j][:] = rng.random(U[j].shape)
+        err = O.rel_fro(_mttkrp(L, csf, m, U, opts), O.coo_mttkrp(inds, vals, U, m))
+        assert err <= 1e-12, (k, m, err)
+    L.splatt_free_csf(csf, opts)
+
+
+@pytest.mark.parametrize("prec,tol", [(A.FP64, 1e-9), (A.FP32, 1e-4)])
+def test_splatt_cpd_als(prec, tol):
+    L = _lib()
+    opts = L.splatt_default_opts()
+    opts[A.OPT_NITER] = 8
+    opts[A.OPT_TOLERANCE] = 0.0
+    opts[A.OPT_RANDSEED] = 99
+    opts[A.OPT_VERBOSITY] = 0
+    opts[A.OPT_PRECISION] = prec
+    dims = [40, 30, 20, 12]
+    inds, vals = _tensor(5, dims, 5000)
+    csf = _convert(L, inds, vals, opts)
+    R = 6
+    k = Kruskal()
+    assert L.splatt_cpd_als(csf, R, opts, C.byref(k)) == 1
+    U0 = [A.rand_matrix(99, m, d, R) for m, d in enumerate(dims)]
+    fits, Af, lam = O.cpd_als(lambda m, F: O.coo_mttkrp(inds, vals, F, m), U0, 8,
+                              float((vals ** 2).sum()))
+    assert abs(k.fit - fits[-1]) <= tol
+    # unit columns, lambda absorbs the norms: the Kruskal tensors agree
+    got = [np.ctypeslib.as_array(k.factors[m], shape=(dims[m], R)) for m in range(4)]
+    for m in range(4):
+        np.testing.assert_allclose(np.linalg.norm(got[m], axis=0), 1.0, rtol=1e-6)
+    lam_got = np.ctypeslib.as_array(k.lambda_, shape=(R,))
+    X1 = np.einsum("r,ar,br,cr,dr->abcd", lam_got, *got)
+    X2 = np.einsum("r,ar,br,cr,dr->abcd", lam, *Af)
+    assert O.rel_fro(X1, X2) <= (1e-8 if prec == A.FP64 else 1e-3)
+    L.splatt_free_kruskal(C.byref(k))
+    L.splatt_free_csf(csf, opts)
+
+
+def test_cli_bench_and_cpd(tmp_path):
+    exe = str(A.CLI_PATH)
+    r = subprocess.run([exe, "bench", "synth:1:200000", "-a", "adatm", "-i", "2", "-r", "16"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "CSF-STORAGE" in r.stdout and "vCSF-STORAGE" in r.stdout
+    assert r.stdout.count("  mode ") == 8 and "its =   2" in r.stdout
+    r = subprocess.run([exe, "cpd", "synth:1:200000", "-r", "8", "-i", "5", "--tol", "0",
+                        "--seed", "3"], capture_output=True, text=True, timeout=300,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Final fit:" in r.stdout and r.stdout.count("  its = ") == 5
+    assert (tmp_path / "lambda.mat").exists() and (tmp_path / "mode4.mat").exists()
+    M = np.loadtxt(tmp_path / "mode1.mat")
+    assert M.shape[1] == 8
+    np.testing.assert_allclose(np.linalg.norm(M, axis=0), 1.0, rtol=1e-6)
