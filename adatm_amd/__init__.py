@@ -138,6 +138,34 @@ def lib() -> C.CDLL:
                                          P(C.c_double), P(C.c_uint64)]),
         "adatm_engine_get_csf": (C.c_int, [C.c_void_p, C.c_uint32, P(P(C.c_uint32)),
                                            P(P(C.c_uint32)), P(C.c_double)]),
+        "adatm_mlp_nparams": (C.c_size_t, []),
+        "adatm_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(C.c_void_p)]),
+        "adatm_mlp_destroy": (None, [C.c_void_p]),
+        "adatm_mlp_set_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_size_t]),
+        "adatm_mlp_get_params": (C.c_int, [C.c_void_p, P(C.c_float), C.c_size_t]),
+        "adatm_mlp_set_norm": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float),
+                                         C.c_float, C.c_float]),
+        "adatm_mlp_get_norm": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float),
+                                         P(C.c_float), P(C.c_float)]),
+        "adatm_mlp_loss_grad": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float),
+                                          C.c_uint32, P(C.c_float), P(C.c_float)]),
+        "adatm_mlp_step": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32,
+                                     C.c_float, P(C.c_float)]),
+        "adatm_mlp_train": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32,
+                                      C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                      P(C.c_float)]),
+        "adatm_mlp_predict": (C.c_int, [C.c_void_p, P(C.c_float), C.c_uint32,
+                                        P(C.c_float)]),
+        "adatm_mlp_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "adatm_mlp_load": (C.c_int, [C.c_int, C.c_char_p, P(C.c_void_p)]),
+        "adatm_plan_features": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
+                                          P(C.c_float)]),
+        "adatm_plan_features_from_counts": (C.c_int, [C.c_uint32, C.c_uint64,
+                                                      P(C.c_uint64), C.c_uint64, P(Plan),
+                                                      P(C.c_uint64), P(C.c_float)]),
+        "adatm_plan_learned": (C.c_int, [P(Coo), C.c_uint64, C.c_void_p, C.c_int,
+                                         P(Plan), C.c_int, P(C.c_int), P(C.c_int),
+                                         P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -317,6 +345,132 @@ def plan_use(plan: Plan, nmodes: int):
     t = np.zeros(MAX_NMODES, dtype=np.uint32)
     lib().adatm_plan_use(C.byref(plan), nmodes, _uptr(g), _uptr(t))
     return [int(x) for x in g[:nmodes]], [int(x) for x in t[:nmodes]]
+
+
+MLP_IN = 24
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def plan_features(t: Tensor, rank: int, plans, device: int = -1) -> np.ndarray:
+    """(len(plans), 24) features of the learned cost model."""
+    arr = (Plan * len(plans))(*plans)
+    f = np.zeros((len(plans), MLP_IN), dtype=np.float32)
+    check(lib().adatm_plan_features(C.byref(t.c), rank, arr, len(plans), device,
+                                    _fptr(f)), "plan_features")
+    return f
+
+
+def plan_features_from_counts(nmodes: int, nnz: int, dims, rank: int, plan: Plan,
+                              Fg) -> np.ndarray:
+    d = np.asarray(list(dims), dtype=np.uint64)
+    F = np.ascontiguousarray(np.asarray(Fg, dtype=np.uint64).reshape(-1))
+    f = np.zeros(MLP_IN, dtype=np.float32)
+    check(lib().adatm_plan_features_from_counts(nmodes, nnz, _u64ptr(d), rank,
+                                                C.byref(plan), _u64ptr(F), _fptr(f)),
+          "plan_features_from_counts")
+    return f
+
+
+class Mlp:
+    """The learned cost model (wraps adatm_mlp; weights on the GPU)."""
+
+    def __init__(self, device: int = 0, seed: int = 1, path=None):
+        self.h = C.c_void_p()
+        if path is not None:
+            check(lib().adatm_mlp_load(device, str(path).encode(), C.byref(self.h)),
+                  "mlp_load")
+        else:
+            check(lib().adatm_mlp_create(device, seed, C.byref(self.h)), "mlp_create")
+
+    @staticmethod
+    def nparams() -> int:
+        return int(lib().adatm_mlp_nparams())
+
+    def close(self):
+        if self.h:
+            lib().adatm_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams(), dtype=np.float32)
+        check(lib().adatm_mlp_get_params(self.h, _fptr(p), p.size), "mlp_get_params")
+        return p
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        check(lib().adatm_mlp_set_params(self.h, _fptr(p), p.size), "mlp_set_params")
+
+    def set_norm(self, xmean, xstd, ymean: float, ystd: float) -> None:
+        a, b = _f32(xmean), _f32(xstd)
+        check(lib().adatm_mlp_set_norm(self.h, _fptr(a), _fptr(b), ymean, ystd),
+              "mlp_set_norm")
+
+    def norm(self):
+        a = np.zeros(MLP_IN, dtype=np.float32)
+        b = np.zeros(MLP_IN, dtype=np.float32)
+        ym, ys = C.c_float(), C.c_float()
+        check(lib().adatm_mlp_get_norm(self.h, _fptr(a), _fptr(b), C.byref(ym),
+                                       C.byref(ys)), "mlp_get_norm")
+        return a, b, ym.value, ys.value
+
+    def loss_grad(self, X, y):
+        X, y = _f32(X), _f32(y)
+        g = np.zeros(self.nparams(), dtype=np.float32)
+        loss = C.c_float()
+        check(lib().adatm_mlp_loss_grad(self.h, _fptr(X), _fptr(y), len(y), _fptr(g),
+                                        C.byref(loss)), "mlp_loss_grad")
+        return loss.value, g
+
+    def step(self, X, y, lr: float) -> float:
+        X, y = _f32(X), _f32(y)
+        loss = C.c_float()
+        check(lib().adatm_mlp_step(self.h, _fptr(X), _fptr(y), len(y), lr,
+                                   C.byref(loss)), "mlp_step")
+        return loss.value
+
+    def train(self, X, y, epochs: int, batch: int = 64, lr: float = 1e-3,
+              seed: int = 1) -> np.ndarray:
+        X, y = _f32(X), _f32(y)
+        hist = np.zeros(epochs, dtype=np.float32)
+        check(lib().adatm_mlp_train(self.h, _fptr(X), _fptr(y), len(y), epochs, batch,
+                                    lr, seed, _fptr(hist)), "mlp_train")
+        return hist
+
+    def predict(self, X) -> np.ndarray:
+        X = _f32(X)
+        out = np.zeros(X.shape[0], dtype=np.float32)
+        check(lib().adatm_mlp_predict(self.h, _fptr(X), X.shape[0], _fptr(out)),
+              "mlp_predict")
+        return out
+
+    def save(self, path) -> None:
+        check(lib().adatm_mlp_save(self.h, str(path).encode()), "mlp_save")
+
+
+def plan_learned(t: Tensor, rank: int, model: "Mlp | None", device: int = 0):
+    """(candidates, chosen index, predictions) of the learned planner; the
+    analytic choice when model is None."""
+    arr = (Plan * 256)()
+    nc = C.c_int()
+    ch = C.c_int()
+    pred = np.zeros(256, dtype=np.float32)
+    check(lib().adatm_plan_learned(C.byref(t.c), rank, model.h if model else None,
+                                   device, arr, 256, C.byref(nc), C.byref(ch),
+                                   _fptr(pred)), "plan_learned")
+    return [arr[i] for i in range(nc.value)], ch.value, pred[: nc.value]
 
 
 # --------------------------------------------------------------------------

@@ -150,6 +150,19 @@ int adatm_k_scatter_rids(hipStream_t s, uint32_t const *node, uint64_t n,
                          uint32_t d, uint32_t const *const *par,
                          uint32_t const *const *fids, uint32_t *const *rid);
 
+/* learned cost model network (mlp.hip): FP32, W row-major I x O */
+int adatm_k_dense_fwd(hipStream_t s, float const *H, float const *W,
+                      float const *b, float *Z, float *Hout, uint32_t B,
+                      uint32_t I, uint32_t O, int relu);
+int adatm_k_mse(hipStream_t s, float const *y, float const *t, uint32_t B,
+                float *dy, float *part);
+int adatm_k_dense_bwd(hipStream_t s, float const *H, float const *dZ,
+                      float const *W, float const *Zprev, float *dH, float *dW,
+                      float *db, uint32_t B, uint32_t I, uint32_t O);
+int adatm_k_adam(hipStream_t s, float *p, float const *g, float *m, float *v,
+                 uint32_t n, float lr, float b1, float b2, float eps, float c1,
+                 float c2);
+
 #ifdef __cplusplus
 }
 #endif

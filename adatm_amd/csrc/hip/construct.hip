@@ -174,7 +174,9 @@ __global__ __launch_bounds__(CB) void k_rs_scatter(
     }
     uint32_t const before = cnt[w][d];
     rank[s] = ok ? before + (uint32_t)__popcll(peers & lt) : 0u;
-    /* lowest lane of the group advances the running count */
+    /* every lane of the wave has read its digit's count before the group's
+       lowest lane advances it (lockstep order, kept by the fence) */
+    __builtin_amdgcn_wave_barrier();
     if (ok && (peers & lt) == 0ull) cnt[w][d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();

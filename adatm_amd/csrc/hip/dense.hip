@@ -63,7 +63,7 @@ __global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
                                                   double const *__restrict__ lambda,
                                                   double *__restrict__ part)
 {
-  extern __shared__ double sh[]; /* GROWS * R */
+  HIP_DYNAMIC_SHARED(double, sh) /* GROWS * R */
   uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
   uint64_t const lo = (uint64_t)blockIdx.x * chunk;
   uint64_t const hi = lo + chunk < I ? lo + chunk : I;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
                                                   double *__restrict__ Vinv,
                                                   int *__restrict__ status)
 {
-  extern __shared__ double a[]; /* R x R, then R elimination factors */
+  HIP_DYNAMIC_SHARED(double, a) /* R x R, then R elimination factors */
   double *fac = a + (size_t)R * R;
   __shared__ double pinv;
   __shared__ int bad;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
                                                      uint32_t R,
                                                      double *__restrict__ part)
 {
-  extern __shared__ double sv[]; /* R*R Vinv, then 2*DB reduction scratch */
+  HIP_DYNAMIC_SHARED(double, sv) /* R*R Vinv, then 2*DB reduction scratch */
   for (uint32_t x = threadIdx.x; x < R * R; x += DB) sv[x] = Vinv[x];
   double *red = sv + R * R;
   __syncthreads();

@@ -84,6 +84,12 @@ int adatm_engine_precision(adatm_engine const *e);
 int adatm_count_fibers_host(adatm_coo const *t, uint32_t const *order,
                             uint64_t *nfibs);
 int adatm_coo_from_sptensor(sptensor_t const *tt, adatm_coo *t);
+/* analytic model of one plan from every group's fiber counts (no quirks) */
+void adatm_plan_model(uint32_t N, uint64_t nnz, uint64_t R, adatm_plan const *P,
+                      uint64_t const *Fg, uint64_t *pspace, uint64_t *pops);
+/* fiber counts of norders level orders (F: norders x N) in one device pass */
+int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
+                                   uint32_t norders, uint64_t *F, int device);
 
 /* ---- seconds since an arbitrary origin --------------------------------- */
 double adatm_wtime(void);

@@ -539,9 +539,6 @@ int adatm_sort_by_id(uint32_t const *ids, uint64_t n, uint64_t nrows,
   return SPLATT_SUCCESS;
 }
 
-int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
-                                   uint32_t norders, uint64_t *F, int device);
-
 /* Fiber counts on the device (radix sort + boundary scan). */
 int adatm_count_fibers_device(adatm_coo const *t, uint32_t const *order,
                               uint64_t *nfibs, int device)

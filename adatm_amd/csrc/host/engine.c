@@ -390,8 +390,9 @@ static int engine_alloc_common(adatm_engine *e)
   }
   if ((rc = dalloc(e, &e->M, e->maxdim * R * e->vsz)) != SPLATT_SUCCESS)
     return rc;
-  if ((rc = dalloc(e, (void **)&e->G, (size_t)N * R * R * sizeof(double))))
-    if (rc != SPLATT_SUCCESS) return rc;
+  if ((rc = dalloc(e, (void **)&e->G, (size_t)N * R * R * sizeof(double))) !=
+      SPLATT_SUCCESS)
+    return rc;
   if ((rc = dalloc(e, (void **)&e->lambda, R * sizeof(double))) != SPLATT_SUCCESS)
     return rc;
   if ((rc = dalloc(e, (void **)&e->Vinv, (size_t)R * R * sizeof(double))) !=
@@ -857,12 +858,14 @@ static int dwork_build(adatm_engine *e, uint32_t const *cp, uint64_t nseg,
     if (dscan(s, npiece, sbase, nseg, stemp, tot, &nslots) != SPLATT_SUCCESS) goto out;
     if (dscan(s, heavy, hbase, nseg, stemp, tot, &nheavy) != SPLATT_SUCCESS) goto out;
   }
-  if ((rc = dalloc(e, (void **)&d->kw.start, (nitems + 1) * 4)) ||
-      (rc = dalloc(e, (void **)&d->kw.seg, (nitems + 1) * 4)) ||
-      (rc = dalloc(e, (void **)&d->kw.slot, (nitems + 1) * 4)) ||
-      (rc = dalloc(e, (void **)&d->kw.hslot, (nheavy + 1) * 4)))
+  if ((rc = dalloc(e, (void **)&d->kw.start, (nitems + 1) * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&d->kw.seg, (nitems + 1) * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&d->kw.slot, (nitems + 1) * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&d->kw.hslot, (nheavy + 1) * 4)) != SPLATT_SUCCESS)
     goto out;
-  if (nheavy && (rc = dalloc(e, (void **)&d->kw.hseg, nheavy * 4))) goto out;
+  if (nheavy &&
+      (rc = dalloc(e, (void **)&d->kw.hseg, nheavy * 4)) != SPLATT_SUCCESS)
+    goto out;
   rc = SPLATT_ERROR_DEVICE;
   if (nseg &&
       adatm_k_items_emit(s, cp, nseg, tile, nitem, ibase, sbase, heavy, hbase,
@@ -883,6 +886,7 @@ static int dwork_build(adatm_engine *e, uint32_t const *cp, uint64_t nseg,
   }
   rc = SPLATT_SUCCESS;
 out:
+  hipStreamSynchronize(s); /* kernels above read the temporaries */
   hipFree(nitem); hipFree(npiece); hipFree(heavy); hipFree(ibase);
   hipFree(sbase); hipFree(hbase);
   return rc;

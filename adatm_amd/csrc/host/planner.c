@@ -30,8 +30,6 @@ int adatm_count_fibers_host(adatm_coo const *t, uint32_t const *order,
                             uint64_t *nfibs);
 int adatm_count_fibers_device(adatm_coo const *t, uint32_t const *order,
                               uint64_t *nfibs, int device);
-int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
-                                   uint32_t norders, uint64_t *F, int device);
 
 /* ---- quicksort of (mode, dim) pairs; tie order as adatm_sort.c:10-67 ---- */
 static void pair_qsort(pair *a, size_t lo, size_t hi)
@@ -388,6 +386,24 @@ static int predict(adatm_coo const *t, uint64_t R, adatm_plan *cands, int nc,
 out:
   free(cache);
   return rc;
+}
+
+/* The analytic model of one plan on fresh statistics of every group
+ * (Fg[g * N + l]); the reference's model without its statistics reuse. */
+void adatm_plan_model(uint32_t N, uint64_t nnz, uint64_t R, adatm_plan const *P,
+                      uint64_t const *Fg, uint64_t *pspace, uint64_t *pops)
+{
+  uint64_t sb = 0, so = 0;
+  for (uint32_t g = 0; g < P->n_grp; ++g) {
+    adatm_group const *G = P->grp + g;
+    uint64_t const *F = Fg + (size_t)g * N;
+    sb += pred_csf_bytes(N, nnz, F) +
+          pred_rcsf_bytes(R, G->begin_imten, G->n_imten, F, 0);
+    so += pred_csf_ops(N, R, F) +
+          pred_rcsf_ops(R, G->begin_imten, G->n_imten, num_remttkrp(P, g, N), F);
+  }
+  *pspace = sb;
+  *pops = so;
 }
 
 int adatm_plan_analytic(adatm_coo const *t, uint64_t rank, int ref_quirks,

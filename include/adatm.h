@@ -129,6 +129,60 @@ void adatm_plan_use(adatm_plan const *p, uint32_t nmodes, uint32_t *use_grp,
                     uint32_t *use_tag);
 
 /* ----------------------------------------------------------------------- *
+ * Learned planner (no counterpart in the reference, which has only the
+ * analytic model of adatm_bench.c:108-142): an MLP 24 -> 64 -> 64 -> 64 -> 1
+ * (ReLU, FP32) predicting log10(ms) of a plan's MTTKRP sequence on the GPU
+ * from adatm_plan_features(); trained and evaluated by HIP kernels.
+ * ----------------------------------------------------------------------- */
+#define ADATM_MLP_IN 24
+#define ADATM_MLP_HID 64
+
+typedef struct adatm_mlp adatm_mlp;
+
+size_t adatm_mlp_nparams(void); /* 9985: W1 b1 W2 b2 W3 b3 W4 b4, W row-major in x out */
+int adatm_mlp_create(int device, uint64_t seed, adatm_mlp **out);
+void adatm_mlp_destroy(adatm_mlp *m);
+int adatm_mlp_set_params(adatm_mlp *m, float const *p, size_t n); /* resets Adam */
+int adatm_mlp_get_params(adatm_mlp *m, float *p, size_t n);
+/* input standardisation x' = (x - xmean) / xstd, target y' = (y - ymean) / ystd */
+int adatm_mlp_set_norm(adatm_mlp *m, float const *xmean, float const *xstd,
+                       float ymean, float ystd);
+int adatm_mlp_get_norm(adatm_mlp *m, float *xmean, float *xstd, float *ymean,
+                       float *ystd);
+/* mean squared error of the standardised targets over n rows and its
+ * gradient w.r.t. the parameters (nparams floats) */
+int adatm_mlp_loss_grad(adatm_mlp *m, float const *X, float const *y, uint32_t n,
+                        float *grad, float *loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch (X, y) */
+int adatm_mlp_step(adatm_mlp *m, float const *X, float const *y, uint32_t n,
+                   float lr, float *loss);
+/* Adam on shuffled minibatches; sets the standardisation from (X, y) first.
+ * loss_hist (nullable, epochs entries) receives the mean batch loss. */
+int adatm_mlp_train(adatm_mlp *m, float const *X, float const *y, uint32_t n,
+                    uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                    float *loss_hist);
+int adatm_mlp_predict(adatm_mlp *m, float const *X, uint32_t n, float *y);
+int adatm_mlp_save(adatm_mlp *m, char const *path);
+int adatm_mlp_load(int device, char const *path, adatm_mlp **out);
+
+/* The 24 features of every candidate (nc x 24, row-major); fiber counts on
+ * the device when device >= 0. */
+int adatm_plan_features(adatm_coo const *t, uint64_t rank,
+                        adatm_plan const *cands, int nc, int device,
+                        float *feat);
+/* Same from known fiber counts: Fg[g * N + l] for every group g of p. */
+int adatm_plan_features_from_counts(uint32_t nmodes, uint64_t nnz,
+                                    uint64_t const *dims, uint64_t rank,
+                                    adatm_plan const *p, uint64_t const *Fg,
+                                    float *feat);
+/* Candidates as adatm_plan_analytic; the chosen one minimises the model's
+ * prediction (pred, nullable, receives them), or is the analytic choice when
+ * model is NULL. */
+int adatm_plan_learned(adatm_coo const *t, uint64_t rank, adatm_mlp *model,
+                       int device, adatm_plan *cands, int maxc, int *ncands,
+                       int *chosen, float *pred);
+
+/* ----------------------------------------------------------------------- *
  * Device engine: group CSFs, memo buffers, factors, output on one GPU.
  * ----------------------------------------------------------------------- */
 typedef struct adatm_engine adatm_engine;
