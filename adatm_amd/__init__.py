@@ -138,6 +138,8 @@ def lib() -> C.CDLL:
                                          P(C.c_double), P(C.c_uint64)]),
         "adatm_engine_get_csf": (C.c_int, [C.c_void_p, C.c_uint32, P(P(C.c_uint32)),
                                            P(P(C.c_uint32)), P(C.c_double)]),
+        "adatm_engine_launches": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int),
+                                            P(C.c_double), P(C.c_double)]),
         "adatm_mlp_nparams": (C.c_size_t, []),
         "adatm_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(C.c_void_p)]),
         "adatm_mlp_destroy": (None, [C.c_void_p]),
@@ -538,6 +540,20 @@ class Engine:
         check(lib().adatm_engine_time(self.h, what, nwarm, nrep, int(graph),
                                       C.byref(ms), _dptr(pk)), "engine_time")
         return ms.value, pk
+
+    def launches(self, max_n: int = 1024):
+        """(class, info, ms, algorithmic bytes) of every launch of the
+        instrumented pass of the last time() call; info = levels fused by a
+        reduction or the level of a scatter."""
+        cls = np.zeros(max_n, dtype=np.int32)
+        info = np.zeros(max_n, dtype=np.int32)
+        ms = np.zeros(max_n, dtype=np.float64)
+        by = np.zeros(max_n, dtype=np.float64)
+        ip = C.POINTER(C.c_int)
+        n = lib().adatm_engine_launches(self.h, max_n, cls.ctypes.data_as(ip),
+                                        info.ctypes.data_as(ip), _dptr(ms), _dptr(by))
+        return [(int(cls[i]), int(info[i]), float(ms[i]), float(by[i]))
+                for i in range(max(n, 0))]
 
     def run_timed(self, what: int, nsteps: int, graph: bool = False) -> float:
         ms = C.c_double()

@@ -35,7 +35,19 @@ typedef struct {
   uint32_t const *anc;   /* nitems*(K-1), NULL when K == 1 */
   uint32_t const *hseg;  /* nheavy */
   uint32_t const *hslot; /* nheavy+1 */
+  /* merge of split parents: chunk c sums carry slots [cstart[c], min(cstart[c]
+   * + ADATM_FIX_CHUNK, hslot[cseg[c]+1])); parent h owns chunks [hchunk[h],
+   * hchunk[h+1]); the nmulti parents of mlist own more than one chunk */
+  uint32_t nchunk, nmulti;
+  uint32_t const *cstart; /* nchunk */
+  uint32_t const *cseg;   /* nchunk */
+  uint32_t const *hchunk; /* nheavy+1 */
+  uint32_t const *mlist;  /* nmulti */
 } adatm_kwork;
+
+#ifndef ADATM_FIX_CHUNK
+#define ADATM_FIX_CHUNK 32u /* carry rows per merge chunk */
+#endif
 
 /* K fused CSF levels below a top level (relative level 0):
  * fptr[j] (j < K) = child pointer of relative level j, ids[j] / U[j]
@@ -53,7 +65,7 @@ typedef struct {
  * is the scalar xin[c] when leaf, else the row xin[c*R ...].  R <= 64. */
 int adatm_k_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
                    adatm_kchain const *ch, void const *xin, void *out,
-                   void *carry, uint32_t R);
+                   void *carry, void *carry2, uint32_t R);
 
 /* Reuse MTTKRP at CSF level d over level-d nodes sorted by output row:
  * out[i] = sum_{q in [rowptr[i], rowptr[i+1])} Y_q o Z[node[q]],
@@ -69,7 +81,12 @@ typedef struct {
 int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
                     uint32_t const *rowptr, adatm_kscat const *sc,
                     void const *Z, int zscalar, void *out, void *carry,
-                    uint32_t R);
+                    void *carry2, uint32_t R);
+
+/* Merge of the split parents of a work list after adatm_k_reduce /
+ * adatm_k_scatter wrote their carry rows (no-op without split parents). */
+int adatm_k_fixup(hipStream_t s, int prec, adatm_kwork const *w,
+                  void const *carry, void *carry2, void *out, uint32_t R);
 
 /* zero n bytes with a kernel (graph-capture safe) */
 int adatm_k_zero(hipStream_t s, void *p, size_t nbytes);
@@ -141,6 +158,12 @@ int adatm_k_items_tail(hipStream_t s, uint32_t const *cp, uint64_t nseg,
                        uint32_t nitems, uint32_t nheavy, uint32_t nslots,
                        uint32_t *start, uint32_t *seg, uint32_t *slot,
                        uint32_t *hslot);
+int adatm_k_fix_count(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
+                      uint32_t ch, uint32_t *nch, uint32_t *multi);
+int adatm_k_fix_emit(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
+                     uint32_t ch, uint32_t *hchunk, uint32_t nchunk,
+                     uint32_t const *multi, uint32_t const *mbase, uint32_t *cstart,
+                     uint32_t *cseg, uint32_t *mlist);
 int adatm_k_items_anc(hipStream_t s, uint32_t const *start, uint32_t nitems,
                       uint32_t lb, uint32_t lt, uint32_t const *const *par,
                       uint32_t *anc);

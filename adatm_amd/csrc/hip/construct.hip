@@ -373,6 +373,44 @@ __global__ void k_items_tail(uint32_t const *__restrict__ cp, uint64_t nseg,
   hslot[nheavy] = nslots;
 }
 
+/* split-parent merge lists: chunks of <= ch carry slots per split parent,
+ * and the parents spanning more than one chunk */
+__global__ void k_fix_count(uint32_t const *__restrict__ hslot, uint32_t nheavy,
+                            uint32_t ch, uint32_t *__restrict__ nch,
+                            uint32_t *__restrict__ multi)
+{
+  uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; h < nheavy; h += st) {
+    uint32_t const ns = hslot[h + 1] - hslot[h];
+    uint32_t const c = (ns + ch - 1) / ch;
+    nch[h] = c;
+    multi[h] = c > 1 ? 1u : 0u;
+  }
+}
+
+__global__ void k_fix_emit(uint32_t const *__restrict__ hslot, uint32_t nheavy,
+                           uint32_t ch, uint32_t const *__restrict__ hchunk,
+                           uint32_t const *__restrict__ multi,
+                           uint32_t const *__restrict__ mbase,
+                           uint32_t *__restrict__ cstart, uint32_t *__restrict__ cseg,
+                           uint32_t *__restrict__ mlist)
+{
+  uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; h < nheavy; h += st) {
+    uint32_t const c0 = hchunk[h];
+    uint32_t const c1 = hchunk[h + 1];
+    for (uint32_t c = c0; c < c1; ++c) {
+      cstart[c] = hslot[h] + (c - c0) * ch;
+      cseg[c] = (uint32_t)h;
+    }
+    if (multi[h]) mlist[mbase[h]] = (uint32_t)h;
+  }
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
 struct par_ptrs {
   uint32_t const *p[ADATM_KMAXL];
 };
@@ -604,6 +642,28 @@ int adatm_k_items_tail(hipStream_t s, uint32_t const *cp, uint64_t nseg,
 {
   hipLaunchKernelGGL(k_items_tail, dim3(1), dim3(1), 0, s, cp, nseg, nitems,
                      nheavy, nslots, start, seg, slot, hslot);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_fix_count(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
+                      uint32_t ch, uint32_t *nch, uint32_t *multi)
+{
+  if (nheavy == 0) return 0;
+  hipLaunchKernelGGL(k_fix_count, grid_for(nheavy), dim3(256), 0, s, hslot, nheavy,
+                     ch, nch, multi);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_fix_emit(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
+                     uint32_t ch, uint32_t *hchunk, uint32_t nchunk,
+                     uint32_t const *multi, uint32_t const *mbase, uint32_t *cstart,
+                     uint32_t *cseg, uint32_t *mlist)
+{
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, hchunk + nheavy, nchunk);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nheavy == 0) return herr(e);
+  hipLaunchKernelGGL(k_fix_emit, grid_for(nheavy), dim3(256), 0, s, hslot, nheavy, ch,
+                     (uint32_t const *)hchunk, multi, mbase, cstart, cseg, mlist);
   return herr(hipGetLastError());
 }
 

@@ -6,7 +6,7 @@
  * gather and each memo-row access of a group is one contiguous R*sizeof(T)
  * transaction.  Work items are balanced by the number of children they walk;
  * a parent longer than one item is cut into pieces whose partial rows go to
- * carry slots, summed in a fixed order by k_fixup, so results are
+ * carry slots, summed in a fixed order by k_fix_chunk / k_fix_multi, so results are
  * deterministic and no output needs zero-filling or atomics.
  *
  *   k_reduce<K>  Z_top(p) = sum over the level-`top+K` descendants c of p of
@@ -26,7 +26,8 @@
  *                1024 striped locks; here nodes are pre-sorted by output row
  *                with their ancestors' row ids, so every output row is one
  *                segment of a segmented reduction.
- *   k_fixup      one workgroup per split parent: tree sum of its carry rows.
+ *   k_fix_chunk  merge of split parents: a lane group per chunk of <= 32 carry
+ *                rows; k_fix_multi sums the chunk rows of parents with several.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -196,24 +197,71 @@ __global__ __launch_bounds__(256) void k_scatter(
   }
 }
 
-/* out[hseg[h]] = sum of carry rows [hslot[h], hslot[h+1]) in a fixed tree
- * order; one workgroup per split parent */
+/* Chunk c of split parent h = cseg[c]: the sum of its <= ADATM_FIX_CHUNK
+ * carry rows, eight independent partial sums combined in a fixed order; the
+ * parent's output row when it has one chunk, else chunk row c of carry2. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_fixup(uint32_t const *__restrict__ hseg,
-                                               uint32_t const *__restrict__ hslot,
-                                               T const *__restrict__ carry,
-                                               T *__restrict__ out, uint32_t R)
+__global__ __launch_bounds__(256) void k_fix_chunk(
+    uint32_t nchunk, uint32_t const *__restrict__ cstart,
+    uint32_t const *__restrict__ cseg, uint32_t const *__restrict__ hslot,
+    uint32_t const *__restrict__ hchunk, uint32_t const *__restrict__ hseg,
+    T const *__restrict__ carry, T *__restrict__ carry2, T *__restrict__ out,
+    uint32_t R, uint32_t gpw)
+{
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const grp = lane / R;
+  if (grp >= gpw) return;
+  uint32_t const col = lane - grp * R;
+  uint32_t const c =
+      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+  if (c >= nchunk) return;
+  uint32_t const h = cseg[c];
+  uint32_t const lo = cstart[c];
+  uint32_t const end = hslot[h + 1];
+  uint32_t const hi = (lo + ADATM_FIX_CHUNK < end) ? lo + ADATM_FIX_CHUNK : end;
+  T p[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) p[i] = (T)0;
+  uint32_t s = lo;
+  for (; s + 8 <= hi; s += 8) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] += carry[(size_t)(s + i) * R + col];
+  }
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+    if (s + i < hi) p[i] += carry[(size_t)(s + i) * R + col];
+  T const acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+  if (hchunk[h + 1] - hchunk[h] == 1)
+    out[(size_t)hseg[h] * R + col] = acc;
+  else
+    carry2[(size_t)c * R + col] = acc;
+}
+
+/* Split parents with several chunks (one workgroup each): tree sum of their
+ * chunk rows in a fixed order. */
+template <typename T>
+__global__ __launch_bounds__(256) void k_fix_multi(uint32_t const *__restrict__ mlist,
+                                                   uint32_t const *__restrict__ hchunk,
+                                                   uint32_t const *__restrict__ hseg,
+                                                   T const *__restrict__ carry2,
+                                                   T *__restrict__ out, uint32_t R)
 {
   __shared__ T red[256];
-  uint32_t const h = blockIdx.x;
+  uint32_t const h = mlist[blockIdx.x];
   uint32_t const rpi = 256u / R;
   uint32_t const sl = threadIdx.x / R;
   uint32_t const col = threadIdx.x - sl * R;
-  T acc = (T)0;
-  if (sl < rpi)
-    for (uint32_t s = hslot[h] + sl; s < hslot[h + 1]; s += rpi)
-      acc += carry[(size_t)s * R + col];
-  red[threadIdx.x] = acc;
+  T a0 = (T)0, a1 = (T)0;
+  if (sl < rpi) {
+    uint32_t const c1 = hchunk[h + 1];
+    uint32_t c = hchunk[h] + sl;
+    for (; c + rpi < c1; c += 2 * rpi) {
+      a0 += carry2[(size_t)c * R + col];
+      a1 += carry2[(size_t)(c + rpi) * R + col];
+    }
+    if (c < c1) a0 += carry2[(size_t)c * R + col];
+  }
+  red[threadIdx.x] = a0 + a1;
   __syncthreads();
   if (threadIdx.x < R) {
     T t = (T)0;
@@ -225,11 +273,18 @@ __global__ __launch_bounds__(256) void k_fixup(uint32_t const *__restrict__ hseg
 /* ---- launchers ----------------------------------------------------------- */
 template <typename T>
 hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
-                        void *out, uint32_t R)
+                        void *carry2, void *out, uint32_t R)
 {
-  if (w->nheavy == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_fixup<T>), dim3(w->nheavy), dim3(256), 0, s, w->hseg,
-                     w->hslot, (T const *)carry, (T *)out, R);
+  if (w->nchunk == 0) return hipSuccess;
+  uint32_t const gpw = 64u / R;
+  uint32_t const per_block = 4u * gpw;
+  hipLaunchKernelGGL((k_fix_chunk<T>), dim3((w->nchunk + per_block - 1) / per_block),
+                     dim3(256), 0, s, w->nchunk, w->cstart, w->cseg, w->hslot,
+                     w->hchunk, w->hseg, (T const *)carry, (T *)carry2, (T *)out, R, gpw);
+  hipError_t const e = hipGetLastError();
+  if (e != hipSuccess || w->nmulti == 0) return e;
+  hipLaunchKernelGGL((k_fix_multi<T>), dim3(w->nmulti), dim3(256), 0, s, w->mlist,
+                     w->hchunk, w->hseg, (T const *)carry2, (T *)out, R);
   return hipGetLastError();
 }
 
@@ -247,7 +302,7 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
 
 template <typename T, typename TU, bool LEAF>
 hipError_t reduce_k(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
-                    void const *xin, void *out, void *carry, uint32_t R)
+                    void const *xin, void *out, void *carry, void *carry2, uint32_t R)
 {
   if (w->nitems > 0) {
     switch (ch->K) {
@@ -260,10 +315,9 @@ hipError_t reduce_k(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
     case 7: go_reduce<T, TU, LEAF, 7>(s, w, ch, xin, out, carry, R); break;
     default: go_reduce<T, TU, LEAF, 8>(s, w, ch, xin, out, carry, R); break;
     }
-    hipError_t const e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
   }
-  return launch_fixup<T>(s, w, carry, out, R);
+  return hipSuccess;
 }
 
 template <typename T, typename TU, int D>
@@ -287,7 +341,7 @@ void go_scatter(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
 template <typename T, typename TU>
 hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
                      adatm_kscat const *sc, void const *Z, int zs, void *out,
-                     void *carry, uint32_t R)
+                     void *carry, void *carry2, uint32_t R)
 {
   if (w->nitems > 0) {
     switch (sc->d) {
@@ -300,10 +354,9 @@ hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr
     case 7: go_scatter<T, TU, 7>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
     default: go_scatter<T, TU, 8>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
     }
-    hipError_t const e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
   }
-  return launch_fixup<T>(s, w, carry, out, R);
+  return hipSuccess;
 }
 
 } // namespace
@@ -312,37 +365,47 @@ extern "C" {
 
 int adatm_k_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
                    adatm_kchain const *ch, void const *xin, void *out,
-                   void *carry, uint32_t R)
+                   void *carry, void *carry2, uint32_t R)
 {
   if (ch->K < 1 || ch->K > ADATM_KMAXFUSE || R < 1 || R > 64)
     return (int)hipErrorInvalidValue;
   hipError_t e;
   if (prec == KP_FP64)
-    e = leaf ? reduce_k<double, double, true>(s, w, ch, xin, out, carry, R)
-             : reduce_k<double, double, false>(s, w, ch, xin, out, carry, R);
+    e = leaf ? reduce_k<double, double, true>(s, w, ch, xin, out, carry, carry2, R)
+             : reduce_k<double, double, false>(s, w, ch, xin, out, carry, carry2, R);
   else if (prec == KP_FP32)
-    e = leaf ? reduce_k<float, float, true>(s, w, ch, xin, out, carry, R)
-             : reduce_k<float, float, false>(s, w, ch, xin, out, carry, R);
+    e = leaf ? reduce_k<float, float, true>(s, w, ch, xin, out, carry, carry2, R)
+             : reduce_k<float, float, false>(s, w, ch, xin, out, carry, carry2, R);
   else
-    e = leaf ? reduce_k<float, bf16raw, true>(s, w, ch, xin, out, carry, R)
-             : reduce_k<float, bf16raw, false>(s, w, ch, xin, out, carry, R);
+    e = leaf ? reduce_k<float, bf16raw, true>(s, w, ch, xin, out, carry, carry2, R)
+             : reduce_k<float, bf16raw, false>(s, w, ch, xin, out, carry, carry2, R);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int adatm_k_fixup(hipStream_t s, int prec, adatm_kwork const *w,
+                  void const *carry, void *carry2, void *out, uint32_t R)
+{
+  if (R < 1 || R > 64) return (int)hipErrorInvalidValue;
+  hipError_t const e = (prec == KP_FP64)
+                           ? launch_fixup<double>(s, w, carry, carry2, out, R)
+                           : launch_fixup<float>(s, w, carry, carry2, out, R);
   return e == hipSuccess ? 0 : (int)e;
 }
 
 int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
                     uint32_t const *rowptr, adatm_kscat const *sc,
                     void const *Z, int zscalar, void *out, void *carry,
-                    uint32_t R)
+                    void *carry2, uint32_t R)
 {
   if (sc->d < 1 || sc->d > ADATM_KMAXSCAT || R < 1 || R > 64)
     return (int)hipErrorInvalidValue;
   hipError_t e;
   if (prec == KP_FP64)
-    e = scatter_d<double, double>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+    e = scatter_d<double, double>(s, w, rowptr, sc, Z, zscalar, out, carry, carry2, R);
   else if (prec == KP_FP32)
-    e = scatter_d<float, float>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+    e = scatter_d<float, float>(s, w, rowptr, sc, Z, zscalar, out, carry, carry2, R);
   else
-    e = scatter_d<float, bf16raw>(s, w, rowptr, sc, Z, zscalar, out, carry, R);
+    e = scatter_d<float, bf16raw>(s, w, rowptr, sc, Z, zscalar, out, carry, carry2, R);
   return e == hipSuccess ? 0 : (int)e;
 }
 

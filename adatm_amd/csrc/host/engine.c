@@ -30,8 +30,12 @@
 #include "../hip/adatm_kern.h"
 
 #define MAXN ADATM_MAX_NMODES
-#define RED_TILE 128u
-#define SCAT_TILE 64u
+#ifndef RED_TILE
+#define RED_TILE 128u /* children per reduction work item */
+#endif
+#ifndef SCAT_TILE
+#define SCAT_TILE 64u /* nodes per scatter work item */
+#endif
 
 typedef struct {
   adatm_kwork kw; /* device pointers */
@@ -70,7 +74,7 @@ typedef struct {
   int zero_root;    /* output rows without a root node: zero M first */
 } dgroup;
 
-enum { KC_LEAF = 0, KC_LEVEL = 1, KC_SCATTER = 2, KC_DENSE = 3, KC_NCLASS = 4 };
+enum { KC_LEAF = 0, KC_LEVEL = 1, KC_SCATTER = 2, KC_DENSE = 3, KC_MERGE = 4, KC_NCLASS = 5 };
 
 struct adatm_engine {
   int device;
@@ -92,8 +96,8 @@ struct adatm_engine {
   uint32_t *iter;
   int *status;
   void *tmp[2];
-  void *carry;
-  size_t tmp_bytes, carry_bytes;
+  void *carry, *carry2;
+  size_t tmp_bytes, carry_bytes, carry2_bytes;
   double ttnormsq;
   size_t dev_bytes;
   uint32_t fits_cap;
@@ -104,11 +108,13 @@ struct adatm_engine {
   int instrument;
   hipEvent_t ev[2048];
   int ev_class[1024];
+  int ev_info[1024]; /* levels fused (reduce) or scatter level */
   double ev_bytes[1024];
   int nev;
 };
 
 /* ---- helpers ------------------------------------------------------------ */
+static int dwork_fixlists(adatm_engine *e, dwork *d);
 static int hip_fail(hipError_t e, char const *what)
 {
   adatm_set_error("%s: %s", what, hipGetErrorString(e));
@@ -192,6 +198,10 @@ static void free_work(dwork *d)
   hipFree((void *)d->kw.slot);
   hipFree((void *)d->kw.hseg);
   hipFree((void *)d->kw.hslot);
+  hipFree((void *)d->kw.cstart);
+  hipFree((void *)d->kw.cseg);
+  hipFree((void *)d->kw.hchunk);
+  hipFree((void *)d->kw.mlist);
   memset(d, 0, sizeof(*d));
 }
 
@@ -206,11 +216,12 @@ static void const *gather_factor(adatm_engine const *e, uint32_t mode)
 }
 
 /* ---- instrumentation ---------------------------------------------------- */
-static void ev_begin(adatm_engine *e, int cls, double bytes)
+static void ev_begin(adatm_engine *e, int cls, int info, double bytes)
 {
   if (!e->instrument || e->nev >= 1024) return;
   hipEventRecord(e->ev[2 * e->nev], e->stream);
   e->ev_class[e->nev] = cls;
+  e->ev_info[e->nev] = info;
   e->ev_bytes[e->nev] = bytes;
 }
 
@@ -253,6 +264,14 @@ static double bytes_scatter(adatm_engine const *e, dgroup const *g, uint32_t d)
   return b;
 }
 
+/* merge of split parents: every carry row read once, chunk rows of the
+ * multi-chunk parents written and read once, one output row per parent */
+static double merge_bytes(adatm_engine const *e, dwork const *d)
+{
+  double const row = (double)e->R * (double)e->vsz;
+  return ((double)d->nslots + 2.0 * d->kw.nchunk + d->kw.nheavy) * row;
+}
+
 /* ---- one MTTKRP ----------------------------------------------------------- */
 static dpass *find_pass(dgroup *g, uint32_t lb, uint32_t lt)
 {
@@ -279,11 +298,17 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
     ch.U[j] = gather_factor(e, g->order[lt + j]);
   }
   int const leaf = (lb == e->N - 1);
-  ev_begin(e, leaf ? KC_LEAF : KC_LEVEL, bytes_pass(e, g, lb, lt));
+  ev_begin(e, leaf ? KC_LEAF : KC_LEVEL, (int)ch.K, bytes_pass(e, g, lb, lt));
   KCHK(adatm_k_reduce(e->stream, e->prec, leaf, &P->w.kw, &ch, in, dst,
-                      e->carry, e->R),
+                      e->carry, e->carry2, e->R),
        "reduce");
   ev_end(e);
+  if (P->w.kw.nchunk) {
+    ev_begin(e, KC_MERGE, 0, merge_bytes(e, &P->w));
+    KCHK(adatm_k_fixup(e->stream, e->prec, &P->w.kw, e->carry, e->carry2, dst, e->R),
+         "merge");
+    ev_end(e);
+  }
   return SPLATT_SUCCESS;
 }
 
@@ -358,11 +383,17 @@ static int mttkrp_level(adatm_engine *e, dgroup *g, uint32_t d, int tag)
     sc.U[l] = gather_factor(e, g->order[l]);
   }
   int const zscalar = (d == N - 1); /* leaf level: Z = the values */
-  ev_begin(e, KC_SCATTER, bytes_scatter(e, g, d));
+  ev_begin(e, KC_SCATTER, (int)d, bytes_scatter(e, g, d));
   KCHK(adatm_k_scatter(e->stream, e->prec, &S->w.kw, S->rowptr, &sc, in,
-                       zscalar, e->M, e->carry, e->R),
+                       zscalar, e->M, e->carry, e->carry2, e->R),
        "scatter");
   ev_end(e);
+  if (S->w.kw.nchunk) {
+    ev_begin(e, KC_MERGE, 0, merge_bytes(e, &S->w));
+    KCHK(adatm_k_fixup(e->stream, e->prec, &S->w.kw, e->carry, e->carry2, e->M, e->R),
+         "merge");
+    ev_end(e);
+  }
   return SPLATT_SUCCESS;
 }
 
@@ -457,6 +488,7 @@ static int build_pass(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
   P->lb = lb;
   P->lt = lt;
   rc = upload_work(e, &w, anc, anc ? (size_t)w.nitems * (K - 1) : 0, &P->w);
+  if (rc == SPLATT_SUCCESS) rc = dwork_fixlists(e, &P->w);
   size_t const cb = (size_t)w.nslots * e->R * e->vsz;
   if (cb > e->carry_bytes) e->carry_bytes = cb;
   free(anc);
@@ -504,6 +536,7 @@ static int build_scatter(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
   if (rc == SPLATT_SUCCESS)
     rc = dupload(e, (void **)&S->rowptr, rowptr, (nrows + 1) * sizeof(uint32_t));
   if (rc == SPLATT_SUCCESS) rc = upload_work(e, &w, NULL, 0, &S->w);
+  if (rc == SPLATT_SUCCESS) rc = dwork_fixlists(e, &S->w);
   {
     size_t const cb = (size_t)w.nslots * e->R * e->vsz;
     if (cb > e->carry_bytes) e->carry_bytes = cb;
@@ -597,6 +630,9 @@ static int engine_finish(adatm_engine *e)
   if ((rc = dalloc(e, &e->tmp[0], e->tmp_bytes)) != SPLATT_SUCCESS) return rc;
   if ((rc = dalloc(e, &e->tmp[1], e->tmp_bytes)) != SPLATT_SUCCESS) return rc;
   if ((rc = dalloc(e, &e->carry, e->carry_bytes ? e->carry_bytes : 64)) !=
+      SPLATT_SUCCESS)
+    return rc;
+  if ((rc = dalloc(e, &e->carry2, e->carry2_bytes ? e->carry2_bytes : 64)) !=
       SPLATT_SUCCESS)
     return rc;
   return SPLATT_SUCCESS;
@@ -884,11 +920,61 @@ static int dwork_build(adatm_engine *e, uint32_t const *cp, uint64_t nseg,
     size_t const cb = (size_t)nslots * e->R * e->vsz;
     if (cb > e->carry_bytes) e->carry_bytes = cb;
   }
-  rc = SPLATT_SUCCESS;
+  rc = dwork_fixlists(e, d);
 out:
   hipStreamSynchronize(s); /* kernels above read the temporaries */
   hipFree(nitem); hipFree(npiece); hipFree(heavy); hipFree(ibase);
   hipFree(sbase); hipFree(hbase);
+  return rc;
+}
+
+/* Merge lists of the split parents of a work list (hslot on the device):
+ * chunks of at most ADATM_FIX_CHUNK carry slots, and the parents that span
+ * several chunks. */
+static int dwork_fixlists(adatm_engine *e, dwork *d)
+{
+  hipStream_t const s = e->stream;
+  uint32_t const nh = d->kw.nheavy;
+  d->kw.nchunk = 0;
+  d->kw.nmulti = 0;
+  if (nh == 0) return SPLATT_SUCCESS;
+  uint32_t *nch = NULL, *multi = NULL, *mbase = NULL, *stemp = NULL, *tot = NULL;
+  uint32_t nchunk = 0, nmulti = 0;
+  int rc = SPLATT_ERROR_NOMEMORY;
+  if (hipMalloc((void **)&nch, (size_t)nh * 4) ||
+      hipMalloc((void **)&multi, (size_t)nh * 4) ||
+      hipMalloc((void **)&mbase, (size_t)nh * 4) ||
+      hipMalloc((void **)&stemp, adatm_k_scan_temp_words(nh) * 4) ||
+      hipMalloc((void **)&tot, 16))
+    goto out;
+  if ((rc = dalloc(e, (void **)&d->kw.hchunk, ((size_t)nh + 1) * 4)) != SPLATT_SUCCESS)
+    goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  if (adatm_k_fix_count(s, d->kw.hslot, nh, ADATM_FIX_CHUNK, nch, multi)) goto out;
+  if (dscan(s, nch, (uint32_t *)d->kw.hchunk, nh, stemp, tot, &nchunk) != SPLATT_SUCCESS)
+    goto out;
+  if (dscan(s, multi, mbase, nh, stemp, tot, &nmulti) != SPLATT_SUCCESS) goto out;
+  if ((rc = dalloc(e, (void **)&d->kw.cstart, (size_t)nchunk * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&d->kw.cseg, (size_t)nchunk * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&d->kw.mlist, ((size_t)nmulti + 1) * 4)) != SPLATT_SUCCESS)
+    goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  if (adatm_k_fix_emit(s, d->kw.hslot, nh, ADATM_FIX_CHUNK, (uint32_t *)d->kw.hchunk,
+                       nchunk, multi, mbase, (uint32_t *)d->kw.cstart,
+                       (uint32_t *)d->kw.cseg, (uint32_t *)d->kw.mlist))
+    goto out;
+  d->kw.nchunk = nchunk;
+  d->kw.nmulti = nmulti;
+  {
+    size_t const cb = (size_t)nchunk * e->R * e->vsz;
+    if (cb > e->carry2_bytes) e->carry2_bytes = cb;
+  }
+  rc = SPLATT_SUCCESS;
+out:
+  hipStreamSynchronize(s);
+  hipFree(nch); hipFree(multi); hipFree(mbase); hipFree(stemp); hipFree(tot);
+  if (rc != SPLATT_SUCCESS && !adatm_last_error()[0])
+    adatm_set_error("split-parent merge lists failed");
   return rc;
 }
 
@@ -1214,6 +1300,7 @@ void adatm_engine_destroy(adatm_engine *e)
   hipFree(e->tmp[0]);
   hipFree(e->tmp[1]);
   hipFree(e->carry);
+  hipFree(e->carry2);
   if (e->ev[0])
     for (int i = 0; i < 2048; ++i)
       if (e->ev[i]) hipEventDestroy(e->ev[i]);
@@ -1345,7 +1432,7 @@ static int sweep_enqueue(adatm_engine *e)
   for (uint32_t m = 0; m < N; ++m) {
     int const rc = mttkrp_dev(e, m);
     if (rc != SPLATT_SUCCESS) return rc;
-    ev_begin(e, KC_DENSE, 0.0);
+    ev_begin(e, KC_DENSE, 0, 0.0);
     KCHK(adatm_k_solve_inv(e->stream, e->G, N, m, R, e->Vinv, e->status),
          "solve_inv");
     KCHK(adatm_k_update(e->stream, e->prec, e->M, e->Vinv, e->A[m], e->Abf[m],
@@ -1354,7 +1441,7 @@ static int sweep_enqueue(adatm_engine *e)
          "update");
     ev_end(e);
   }
-  ev_begin(e, KC_DENSE, 0.0);
+  ev_begin(e, KC_DENSE, 0, 0.0);
   KCHK(adatm_k_fit(e->stream, e->prec, e->A[N - 1], e->M, e->dims[N - 1],
                    e->lambda, e->G, N, R, e->ttnormsq, e->part, e->fits,
                    e->iter),
@@ -1522,6 +1609,25 @@ int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
     }
   }
   return SPLATT_SUCCESS;
+}
+
+/* per-launch records of the last instrumented pass of adatm_engine_time:
+ * class, levels fused / scatter level, device ms and algorithmic bytes of
+ * every launch, in order */
+int adatm_engine_launches(adatm_engine *e, int max, int *cls, int *info,
+                          double *ms, double *bytes)
+{
+  if (!e || max < 0) return -1;
+  int const n = e->nev < max ? e->nev : max;
+  for (int i = 0; i < n; ++i) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, e->ev[2 * i], e->ev[2 * i + 1]);
+    if (cls) cls[i] = e->ev_class[i];
+    if (info) info[i] = e->ev_info[i];
+    if (ms) ms[i] = t;
+    if (bytes) bytes[i] = e->ev_bytes[i];
+  }
+  return n;
 }
 
 /* ---- statistics --------------------------------------------------------------- */

@@ -27,8 +27,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = ("ms/CP-ALS sweep (full MTTKRP seq), learned vs plain plan; "
           "MTTKRP HBM GB/s")
-KC_NAMES = {0: "k_seg_reduce (leaf SpTTM)", 1: "k_seg_reduce (level Hadamard-reduce)",
-            2: "k_scatter (memo reuse)", 3: "dense ALS steps"}
+KC_NAMES = {0: "k_reduce (leaf SpTTM)", 1: "k_reduce (level Hadamard-reduce)",
+            2: "k_scatter (memo reuse)", 3: "dense ALS steps",
+            4: "k_fix_chunk/k_fix_multi (split-parent merge)"}
+MTTKRP_CLASSES = (0, 1, 2, 4)
+
+
+def launch_name(cls: int, info: int, leaf: bool = True) -> str:
+    if cls in (0, 1):
+        return (f"k_reduce<float, float, {'true' if cls == 0 else 'false'}, {info}> "
+                f"({'leaf SpTTM' if cls == 0 else 'level Hadamard-reduce'}, "
+                f"{info} level(s) fused)")
+    if cls == 2:
+        return f"k_scatter<float, float, {info}, ...> (memo reuse at level {info})"
+    return KC_NAMES[cls]
 
 
 def log(msg: str) -> None:
@@ -83,11 +95,12 @@ def main() -> int:
         fits = e.fits(args.steps)
         seq_ms, _ = e.time(0, 1, 5)
         _, pk = e.time(1, 0, 1, graph=False)
+        launch_log = e.launches()
         F, seq_bytes, seq_flops, dev_bytes = e.stats()
         res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=float(fits[-1]),
                          seq_ms=seq_ms, pk=pk, seq_bytes=seq_bytes,
                          seq_flops=seq_flops, dev_bytes=dev_bytes,
-                         nfibs=F.tolist())
+                         nfibs=F.tolist(), launches=launch_log)
         log(f"{name}: {ms / args.steps:.3f} ms/sweep, MTTKRP seq {seq_ms:.3f} ms, "
             f"fit {fits[-1]:.6f}, device {dev_bytes / 2**30:.1f} GiB")
         # MTTKRP error of this precision against the FP64 path (pinned to the
@@ -110,14 +123,13 @@ def main() -> int:
 
     r32 = res["fp32"]
     pk = r32["pk"]
-    times = {c: pk[c] for c in range(4)}
-    dom = max(range(3), key=lambda c: times[c])
-    launches = pk[8 + dom]
-    avg_ms = times[dom] / launches
-    bytes_per_launch = pk[16 + dom] / launches
+    times = {c: pk[c] for c in range(5)}
+    # dominant single launch of the MTTKRP kernels (one instrumented sweep)
+    mk = [x for x in r32["launches"] if x[0] in (0, 1, 2)]
+    dcls, dinfo, avg_ms, bytes_per_launch = max(mk, key=lambda x: x[2])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    mttkrp_ms = sum(times[c] for c in range(3))
-    mttkrp_bytes = sum(pk[16 + c] for c in range(3))
+    mttkrp_ms = sum(times[c] for c in MTTKRP_CLASSES)
+    mttkrp_bytes = sum(pk[16 + c] for c in MTTKRP_CLASSES)
 
     cpu = None
     if not args.no_cpu:
@@ -145,7 +157,7 @@ def main() -> int:
             "plan_index": ch,
         },
         "roofline": {
-            "bound": "hbm", "kernel": KC_NAMES[dom],
+            "bound": "hbm", "kernel": launch_name(dcls, dinfo),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
@@ -160,7 +172,9 @@ def main() -> int:
         "fit_fp32": r32["fit_last"],
         "fit_fp64": res["fp64"]["fit_last"] if "fp64" in res else None,
         "mttkrp_rel_err_fp32_vs_fp64": res.get("mttkrp_rel_err_fp32_vs_fp64"),
-        "kernel_ms": {KC_NAMES[c]: round(times[c], 4) for c in range(4)},
+        "kernel_ms": {KC_NAMES[c]: round(times[c], 4) for c in range(5)},
+        "launches_ms": [[launch_name(c, i), round(ms, 4), round(b / 1e9, 4)]
+                        for c, i, ms, b in r32["launches"] if c != 3],
     }
     print(json.dumps(out), flush=True)
     return 0

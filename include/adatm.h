@@ -218,10 +218,18 @@ int adatm_engine_cpd(adatm_engine *e, uint32_t niters, double tol,
  * 1 one full ALS sweep, after nwarm untimed repetitions (hipEvents on the
  * engine's stream).  per_kernel (nullable, 24 entries) then receives, from
  * one extra pass with an event pair around every launch, per kernel class
- * c (0 leaf SpTTM, 1 level Hadamard-reduce, 2 reuse scatter, 3 dense ALS):
+ * c (0 leaf SpTTM, 1 level Hadamard-reduce, 2 reuse scatter, 3 dense ALS,
+ * 4 merge of split parents):
  * [c] total device ms, [8+c] launch count, [16+c] algorithmic bytes. */
 int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
                       int use_graph, double *ms_median, double *per_kernel);
+
+/* Launch records of the last instrumented pass of adatm_engine_time (class
+ * as above, 4 = merge of split parents; info = levels fused by a reduction or
+ * the level of a scatter; device ms; algorithmic bytes).  Returns the number
+ * written (<= max). */
+int adatm_engine_launches(adatm_engine *e, int max, int *cls, int *info,
+                          double *ms, double *bytes);
 
 /* nsteps back-to-back MTTKRP sequences (what = 0) or ALS sweeps (what = 1,
  * after fresh Grams), bracketed by stream synchronisation; *ms_total is the
