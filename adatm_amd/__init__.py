@@ -534,11 +534,15 @@ class Engine:
                                      C.byref(done)), "cpd")
         return fits[: done.value]
 
-    def time(self, what: int, nwarm: int, nrep: int, graph: bool = False):
+    def time(self, what: int, nwarm: int, nrep: int, graph: bool = False,
+             per_kernel: bool = True):
+        """Median ms of `what` (0 MTTKRP sequence, 1 ALS sweep); with
+        per_kernel, also the per-class totals of one instrumented pass."""
         ms = C.c_double()
         pk = np.zeros(24, dtype=np.float64)
         check(lib().adatm_engine_time(self.h, what, nwarm, nrep, int(graph),
-                                      C.byref(ms), _dptr(pk)), "engine_time")
+                                      C.byref(ms), _dptr(pk) if per_kernel else None),
+              "engine_time")
         return ms.value, pk
 
     def launches(self, max_n: int = 1024):
