@@ -49,13 +49,15 @@ typedef struct {
 #define ADATM_FIX_CHUNK 32u /* carry rows per merge chunk */
 #endif
 
-/* K fused CSF levels below a top level (relative level 0):
- * fptr[j] (j < K) = child pointer of relative level j, ids[j] / U[j]
- * (1 <= j <= K) = node ids and factor of relative level j. */
+/* K fused CSF levels below a top level lt (relative level 0): ids[j] / U[j]
+ * (1 <= j <= K) = node ids and factor of relative level j; cfd[c] = the
+ * first (absolute) level at which bottom node c and c+1 have different
+ * ancestors (0 for the last node), so the nodes of levels >= cfd[c] end
+ * with child c. */
 typedef struct {
-  uint32_t K;
+  uint32_t K, lt;
   uint32_t const *orow; /* output row of each top node (NULL: the node) */
-  uint32_t const *fptr[ADATM_KMAXFUSE + 1];
+  uint8_t const *cfd;
   uint32_t const *ids[ADATM_KMAXFUSE + 1];
   void const *U[ADATM_KMAXFUSE + 1];
 } adatm_kchain;
@@ -145,6 +147,9 @@ int adatm_k_fptr(hipStream_t s, uint32_t const *start, uint64_t F,
 int adatm_k_parent(hipStream_t s, uint32_t const *start_next, uint64_t Fn,
                    uint32_t const *flags, uint32_t const *pos, uint32_t *par);
 int adatm_k_compose(hipStream_t s, uint32_t *cp, uint64_t n1, uint32_t const *fp);
+/* cfd of level lb from the parent arrays par[1..lb] (par[l]: level l -> l-1) */
+int adatm_k_cfd(hipStream_t s, uint32_t const *const *par, uint32_t lb, uint64_t F,
+                uint8_t *cfd);
 int adatm_k_items_count(hipStream_t s, uint32_t const *cp, uint64_t nseg,
                         uint32_t tile, uint32_t *nitem, uint32_t *npiece,
                         uint32_t *heavy);

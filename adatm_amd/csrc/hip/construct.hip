@@ -415,6 +415,28 @@ struct par_ptrs {
   uint32_t const *p[ADATM_KMAXL];
 };
 
+/* first level at which nodes c and c+1 of level lb have different ancestors */
+__global__ void k_cfd(par_ptrs par, uint32_t lb, uint64_t F, uint8_t *__restrict__ cfd)
+{
+  uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; c < F; c += st) {
+    uint32_t l = 0;
+    if (c + 1 < F) {
+      uint32_t x = (uint32_t)c, y = (uint32_t)c + 1;
+      l = lb;
+      while (l > 0) {
+        uint32_t const px = par.p[l][x], py = par.p[l][y];
+        if (px == py) break;
+        x = px;
+        y = py;
+        --l;
+      }
+    }
+    cfd[c] = (uint8_t)l;
+  }
+}
+
 /* anc[k*(K-1) + j-1] = level-(lt+j) ancestor of item k's first position */
 __global__ void k_items_anc(uint32_t const *__restrict__ start, uint32_t nitems,
                             uint32_t lb, uint32_t lt, par_ptrs par,
@@ -664,6 +686,17 @@ int adatm_k_fix_emit(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
   if (e != hipSuccess || nheavy == 0) return herr(e);
   hipLaunchKernelGGL(k_fix_emit, grid_for(nheavy), dim3(256), 0, s, hslot, nheavy, ch,
                      (uint32_t const *)hchunk, multi, mbase, cstart, cseg, mlist);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_cfd(hipStream_t s, uint32_t const *const *par, uint32_t lb, uint64_t F,
+                uint8_t *cfd)
+{
+  if (F == 0) return 0;
+  if (lb >= ADATM_KMAXL) return (int)hipErrorInvalidValue;
+  par_ptrs p;
+  for (uint32_t l = 0; l < ADATM_KMAXL; ++l) p.p[l] = (l >= 1 && l <= lb) ? par[l] : nullptr;
+  hipLaunchKernelGGL(k_cfd, grid_for(F), dim3(256), 0, s, p, lb, F, cfd);
   return herr(hipGetLastError());
 }
 

@@ -35,9 +35,16 @@
 #include "adatm_kern.h"
 #include "common.hpp"
 
-#define PF 4 /* children whose loads are issued together */
+#define PF 4 /* scatter terms whose loads are issued together */
 
 namespace {
+
+/* Children per batch: all of a batch's loads -- child rows and the rows of
+ * the intermediate nodes that close inside it -- are issued together. */
+template <int K>
+struct rb_of {
+  static constexpr int v = (K <= 3) ? 8 : 4;
+};
 
 template <typename T, typename TU, bool LEAF, int K>
 __global__ __launch_bounds__(256) void k_reduce(
@@ -47,6 +54,7 @@ __global__ __launch_bounds__(256) void k_reduce(
     T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
+  constexpr int RB = rb_of<K>::v;
   uint32_t const lane = threadIdx.x & 63u;
   uint32_t const grp = lane / R;
   if (grp >= gpw) return;
@@ -57,59 +65,90 @@ __global__ __launch_bounds__(256) void k_reduce(
 
   uint32_t const a = istart[k], b = istart[k + 1];
   uint32_t const slot = islot[k];
-  uint32_t node[K], endc[K];
+  uint32_t const lt = ch.lt;
+  uint8_t const *__restrict__ cfd = ch.cfd;
+  uint32_t node[K];
   node[0] = iseg[k];
 #pragma unroll
   for (int j = 1; j < K; ++j) node[j] = ianc[(size_t)k * (K - 1) + (j - 1)];
-#pragma unroll
-  for (int j = 0; j < K; ++j) endc[j] = ch.fptr[j][node[j] + 1];
   T acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = (T)0;
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
 
-  for (uint32_t c0 = a; c0 < b; c0 += PF) {
-    T xv[PF], uv[PF];
+  for (uint32_t c0 = a; c0 < b; c0 += RB) {
+    uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
+    /* batch metadata: child ids, first differing level after each child
+       (the item's last child closes every level), leaf values */
+    uint32_t id[RB], f[RB];
+    T xv[RB];
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      uint32_t const c = (c0 + i < b) ? c0 + i : b - 1;
-      uint32_t const id = idb[c];
-      uv[i] = (T)ldu(Ub + (size_t)id * R + col);
-      xv[i] = LEAF ? xin[c] : xin[(size_t)c * R + col];
+    for (int i = 0; i < RB; ++i) {
+      uint32_t const c = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
+      id[i] = idb[c];
+      f[i] = (c0 + i + 1 >= b) ? 0u : (uint32_t)cfd[c];
+      if (LEAF) xv[i] = xin[c];
     }
+    /* closes of the intermediate levels in this batch: the q-th close of
+       relative level j is node node[j] + q */
+    uint32_t ncl[K];
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      uint32_t const c = c0 + i;
-      if (c < b) {
+    for (int j = 1; j < K; ++j) {
+      ncl[j] = 0;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+        ncl[j] += ((uint32_t)i < nb && f[i] <= lt + j) ? 1u : 0u;
+    }
+    uint32_t cid[K][RB];
+#pragma unroll
+    for (int j = 1; j < K; ++j)
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+        cid[j][q] = ((uint32_t)q < ncl[j]) ? ch.ids[j][node[j] + q] : 0u;
+    T uv[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      uv[i] = (T)ldu(Ub + (size_t)id[i] * R + col);
+      if (!LEAF) {
+        uint32_t const c = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
+        xv[i] = xin[(size_t)c * R + col];
+      }
+    }
+    T uc[K][RB];
+#pragma unroll
+    for (int j = 1; j < K; ++j)
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+        uc[j][q] = ((uint32_t)q < ncl[j])
+                       ? (T)ldu((TU const *)ch.U[j] + (size_t)cid[j][q] * R + col)
+                       : (T)0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      if ((uint32_t)i < nb) {
         acc[K - 1] += xv[i] * uv[i];
-        bool const last = (c + 1 == b);
-        uint32_t nxt = c + 1;
         bool go = true;
 #pragma unroll
-        for (int j = K - 1; j >= 0; --j) {
-          if (go && (last || nxt == endc[j])) {
-            if (j > 0) {
-              /* node[j] complete (or the item ends inside it): fold it into
-                 its parent with its own factor row */
-              uint32_t const nid = ch.ids[j][node[j]];
-              T const u = (T)ldu((TU const *)ch.U[j] + (size_t)nid * R + col);
-              acc[j - 1] += acc[j] * u;
-              acc[j] = (T)0;
-              nxt = ++node[j];
-              if (!last) endc[j] = ch.fptr[j][nxt + 1];
-            } else {
-              T *dst = (slot != NONE32)
-                           ? carry + (size_t)slot * R
-                           : out + (size_t)(ch.orow ? ch.orow[node[0]] : node[0]) * R;
-              dst[col] = acc[0];
-              acc[0] = (T)0;
-              ++node[0];
-              if (!last) endc[0] = ch.fptr[0][node[0] + 1];
-            }
+        for (int j = K - 1; j >= 1; --j) {
+          if (go && f[i] <= lt + j) {
+            /* node[j] complete (or the item ends inside it): fold it into
+               its parent with its own factor row */
+            acc[j - 1] += acc[j] * uc[j][0];
+            acc[j] = (T)0;
+#pragma unroll
+            for (int q = 0; q + 1 < RB; ++q) uc[j][q] = uc[j][q + 1];
+            ++node[j];
           } else {
             go = false;
           }
+        }
+        if (go && f[i] <= lt) {
+          T *dst = (slot != NONE32)
+                       ? carry + (size_t)slot * R
+                       : out + (size_t)(ch.orow ? ch.orow[node[0]] : node[0]) * R;
+          dst[col] = acc[0];
+          acc[0] = (T)0;
+          ++node[0];
         }
       }
     }

@@ -66,6 +66,7 @@ typedef struct {
   void *vals;
   uint32_t *ids[MAXN];
   uint32_t *ptr[MAXN];
+  uint8_t *cfd[MAXN]; /* first differing level after each node (pass bottoms) */
   void *memo[MAXN];
   dpass pass[MAXPASS];
   uint32_t npass;
@@ -240,16 +241,16 @@ static double bytes_pass(adatm_engine const *e, dgroup const *g, uint32_t lb,
 {
   double const z = (double)e->vsz, w = (double)e->wsz, R = (double)e->R;
   double b;
-  if (lb == e->N - 1) /* leaf values and leaf ids */
-    b = (double)g->F[lb] * (z + 4.0);
-  else /* memo rows and their node ids */
-    b = (double)g->F[lb] * (R * z + 4.0);
-  for (uint32_t l = lt + 1; l < lb; ++l) /* intermediate ids + pointers */
-    b += (double)g->F[l] * 4.0 + ((double)g->F[l] + 1.0) * 4.0;
-  b += ((double)g->F[lt] + 1.0) * 4.0;
+  if (lb == e->N - 1) /* leaf values, leaf ids, first-differing levels */
+    b = (double)g->F[lb] * (z + 4.0 + 1.0);
+  else /* memo rows, their node ids, first-differing levels */
+    b = (double)g->F[lb] * (R * z + 4.0 + 1.0);
+  for (uint32_t l = lt + 1; l < lb; ++l) /* intermediate node ids */
+    b += (double)g->F[l] * 4.0;
   for (uint32_t l = lt + 1; l <= lb; ++l)
     b += (double)e->dims[g->order[l]] * R * w;
-  b += (double)g->F[lt] * R * z;
+  b += (double)g->F[lt] * R * z; /* output rows */
+  if (lt == 0 && g->orow) b += (double)g->F[0] * 4.0;
   return b;
 }
 
@@ -291,8 +292,9 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
   adatm_kchain ch;
   memset(&ch, 0, sizeof(ch));
   ch.K = lb - lt;
+  ch.lt = lt;
   ch.orow = (lt == 0) ? g->orow : NULL;
-  for (uint32_t j = 0; j < ch.K; ++j) ch.fptr[j] = g->ptr[lt + j];
+  ch.cfd = g->cfd[lb];
   for (uint32_t j = 1; j <= ch.K; ++j) {
     ch.ids[j] = g->ids[lt + j];
     ch.U[j] = gather_factor(e, g->order[lt + j]);
@@ -451,6 +453,29 @@ static int build_pass(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
 {
   if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
   if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
+  if (!g->cfd[lb]) {
+    /* first level at which node c and c+1 of level lb differ */
+    uint64_t const F = h->F[lb];
+    uint8_t *f = (uint8_t *)malloc(F ? F : 1);
+    if (!f) return SPLATT_ERROR_NOMEMORY;
+    #pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < (int64_t)F; ++c) {
+      uint32_t l = 0;
+      if ((uint64_t)c + 1 < F) {
+        uint32_t x = (uint32_t)c, y = (uint32_t)c + 1;
+        l = lb;
+        while (l > 0 && h->par[l][x] != h->par[l][y]) {
+          x = h->par[l][x];
+          y = h->par[l][y];
+          --l;
+        }
+      }
+      f[c] = (uint8_t)l;
+    }
+    int const rc = dupload(e, (void **)&g->cfd[lb], f, F);
+    free(f);
+    if (rc != SPLATT_SUCCESS) return rc;
+  }
   uint64_t const np = h->F[lt];
   uint32_t *cp = (uint32_t *)malloc((np + 1) * sizeof(uint32_t));
   if (!cp) return SPLATT_ERROR_NOMEMORY;
@@ -991,6 +1016,12 @@ static int build_pass_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t lb,
   if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
   if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
   hipStream_t const s = e->stream;
+  if (!g->cfd[lb]) {
+    int const rc = dalloc(e, (void **)&g->cfd[lb], g->F[lb]);
+    if (rc != SPLATT_SUCCESS) return rc;
+    KCHK(adatm_k_cfd(s, (uint32_t const *const *)B->par, lb, g->F[lb], g->cfd[lb]),
+         "first differing levels");
+  }
   uint64_t const np = g->F[lt];
   uint32_t *cp = NULL;
   HIPCHK(hipMalloc((void **)&cp, (np + 1) * 4), "alloc cp");
@@ -1274,6 +1305,7 @@ void adatm_engine_destroy(adatm_engine *e)
     for (uint32_t l = 0; l < MAXN; ++l) {
       hipFree(g->ids[l]);
       hipFree(g->ptr[l]);
+      hipFree(g->cfd[l]);
       hipFree(g->memo[l]);
       dscat *S = g->scat + l;
       hipFree(S->node);
