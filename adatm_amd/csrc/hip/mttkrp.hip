@@ -77,19 +77,25 @@ __global__ __launch_bounds__(256) void k_reduce(
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
 
-  for (uint32_t c0 = a; c0 < b; c0 += RB) {
+  /* batch metadata (child ids, first differing level after each child --
+     the item's last child closes every level -- and the child values or
+     input rows), loaded one batch ahead */
+  uint32_t id[RB], f[RB];
+  T xv[RB];
+  auto load_meta = [&](uint32_t c0, uint32_t *mid, uint32_t *mf, T *mx) {
     uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
-    /* batch metadata: child ids, first differing level after each child
-       (the item's last child closes every level), leaf values */
-    uint32_t id[RB], f[RB];
-    T xv[RB];
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       uint32_t const c = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
-      id[i] = idb[c];
-      f[i] = (c0 + i + 1 >= b) ? 0u : (uint32_t)cfd[c];
-      if (LEAF) xv[i] = xin[c];
+      mid[i] = idb[c];
+      mf[i] = (c0 + i + 1 >= b) ? 0u : (uint32_t)cfd[c];
+      mx[i] = LEAF ? xin[c] : xin[(size_t)c * R + col];
     }
+  };
+  load_meta(a, id, f, xv);
+
+  for (uint32_t c0 = a; c0 < b; c0 += RB) {
+    uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
     /* closes of the intermediate levels in this batch: the q-th close of
        relative level j is node node[j] + q */
     uint32_t ncl[K];
@@ -100,6 +106,9 @@ __global__ __launch_bounds__(256) void k_reduce(
       for (int i = 0; i < RB; ++i)
         ncl[j] += ((uint32_t)i < nb && f[i] <= lt + j) ? 1u : 0u;
     }
+    /* issue order: closing-node ids, child rows, closing-node rows, next
+       batch's metadata -- so waiting for this batch never waits for the
+       prefetch */
     uint32_t cid[K][RB];
 #pragma unroll
     for (int j = 1; j < K; ++j)
@@ -108,13 +117,7 @@ __global__ __launch_bounds__(256) void k_reduce(
         cid[j][q] = ((uint32_t)q < ncl[j]) ? ch.ids[j][node[j] + q] : 0u;
     T uv[RB];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      uv[i] = (T)ldu(Ub + (size_t)id[i] * R + col);
-      if (!LEAF) {
-        uint32_t const c = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
-        xv[i] = xin[(size_t)c * R + col];
-      }
-    }
+    for (int i = 0; i < RB; ++i) uv[i] = (T)ldu(Ub + (size_t)id[i] * R + col);
     T uc[K][RB];
 #pragma unroll
     for (int j = 1; j < K; ++j)
@@ -123,6 +126,10 @@ __global__ __launch_bounds__(256) void k_reduce(
         uc[j][q] = ((uint32_t)q < ncl[j])
                        ? (T)ldu((TU const *)ch.U[j] + (size_t)cid[j][q] * R + col)
                        : (T)0;
+    uint32_t nid[RB], nf[RB];
+    T nxv[RB];
+    uint32_t const c1 = c0 + RB;
+    if (c1 < b) load_meta(c1, nid, nf, nxv);
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       if ((uint32_t)i < nb) {
@@ -150,6 +157,14 @@ __global__ __launch_bounds__(256) void k_reduce(
           acc[0] = (T)0;
           ++node[0];
         }
+      }
+    }
+    if (c1 < b) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        id[i] = nid[i];
+        f[i] = nf[i];
+        xv[i] = nxv[i];
       }
     }
   }

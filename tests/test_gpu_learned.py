@@ -1,0 +1,122 @@
+"""Learned planner on the GPU: the HIP forward / backward / Adam kernels
+against the FP64 oracle network, training, persistence, and the planner
+entry points."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import adatm_amd as A
+from oracle import mlpref as M
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _identity_norm(m):
+    m.set_norm(np.zeros(24), np.ones(24), 0.0, 1.0)
+
+
+def _data(n=200, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 24)).astype(np.float32)
+    y = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 3]).astype(np.float32)
+    return X, y
+
+
+def test_forward_and_gradient_match_oracle():
+    print("mlp forward/grad vs oracle", flush=True)
+    m = A.Mlp(device=0, seed=5)
+    _identity_norm(m)
+    p = M.he_init(2).astype(np.float32)
+    m.set_params(p)
+    assert np.array_equal(m.params(), p)
+    X, y = _data(97)                       # not a multiple of the block size
+    pred = m.predict(X)
+    ref, _ = M.forward(p.astype(np.float64), X)
+    # FP32 inner products of length <= 64 per layer
+    assert np.max(np.abs(pred - ref)) <= 1e-4 * max(1.0, np.max(np.abs(ref)))
+    loss, g = m.loss_grad(X, y)
+    rl, rg = M.loss_grad(p.astype(np.float64), X, y)
+    assert abs(loss - rl) <= 1e-5 * max(1.0, rl)
+    # FP32 gradient against FP64: relative to the gradient's scale
+    assert np.linalg.norm(g - rg) <= 1e-4 * np.linalg.norm(rg), np.linalg.norm(g - rg)
+
+
+def test_adam_step_matches_oracle():
+    print("mlp adam step vs oracle", flush=True)
+    m = A.Mlp(device=0, seed=5)
+    _identity_norm(m)
+    p = M.he_init(4).astype(np.float32)
+    m.set_params(p)
+    X, y = _data(64, seed=1)
+    P = p.astype(np.float64)
+    mm = np.zeros_like(P)
+    vv = np.zeros_like(P)
+    for step in (1, 2, 3):
+        m.step(X, y, 1e-3)
+        _, g = M.loss_grad(P, X, y)
+        P, mm, vv = M.adam_step(P, g, mm, vv, step, 1e-3)
+    # Adam moves every weight by about lr per step; FP32 vs FP64 gradients
+    # may flip the sign of near-zero components, so compare the bulk
+    d = np.abs(m.params() - P)
+    assert np.median(d) <= 1e-6 and np.quantile(d, 0.99) <= 2e-4, (np.median(d), d.max())
+
+
+def test_training_reduces_loss_and_roundtrips(tmp_path):
+    print("mlp training", flush=True)
+    X, y = _data(512, seed=2)
+    m = A.Mlp(device=0, seed=3)
+    hist = m.train(X, y, epochs=60, batch=64, lr=3e-3, seed=9)
+    assert hist[-1] < 0.25 * hist[0], hist[[0, -1]]
+    # the same seeds give the same network (deterministic kernels and order)
+    m2 = A.Mlp(device=0, seed=3)
+    m2.train(X, y, epochs=60, batch=64, lr=3e-3, seed=9)
+    assert np.array_equal(m.params(), m2.params())
+    path = tmp_path / "cm.bin"
+    m.save(path)
+    m3 = A.Mlp(device=0, path=path)
+    assert np.array_equal(m3.params(), m.params())
+    assert np.array_equal(m3.predict(X), m.predict(X))
+    # predictions de-standardise with the training-set statistics
+    xm, xs, ym, ys = m.norm()
+    assert np.allclose(xm, X.mean(axis=0), atol=1e-5)
+    assert abs(ym - y.mean()) <= 1e-5
+    P = m.params().astype(np.float64)
+    ref, _ = M.forward(P, (X - xm) / xs)
+    assert np.allclose(m.predict(X), ref * ys + ym, atol=1e-4)
+
+
+def _tensor(N, ndraws=30000, seed=21):
+    dims = [300, 200, 150, 100, 80, 60, 50, 40][:N]
+    spec = A.make_spec(dims, ndraws, dist=A.DIST_ZIPF, zipf_s=0.9, seed=seed)
+    t = A.Tensor.synth(spec)
+    t.canonicalize()
+    return t
+
+
+@pytest.mark.parametrize("N", [4, 6, 8])
+def test_features_device_equal_host(N):
+    print(f"features N={N}", flush=True)
+    t = _tensor(N)
+    cands = A.plan_candidates(N, t.dims)
+    assert np.array_equal(A.plan_features(t, 32, cands, device=0),
+                          A.plan_features(t, 32, cands, device=-1))
+
+
+def test_plan_learned_without_model_is_analytic():
+    t = _tensor(5)
+    cands, ch = A.plan_analytic(t, 16, device=0)
+    c2, ch2, _ = A.plan_learned(t, 16, None, device=0)
+    assert ch2 == ch and len(c2) == len(cands)
+
+
+def test_plan_learned_picks_min_prediction():
+    t = _tensor(6)
+    m = A.Mlp(device=0, seed=1)
+    cands, ch, pred = A.plan_learned(t, 16, m, device=0)
+    f = A.plan_features(t, 16, cands, device=0)
+    assert np.allclose(pred, m.predict(f))
+    assert ch == int(np.argmin(pred))
