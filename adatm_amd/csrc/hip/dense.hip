@@ -115,6 +115,104 @@ __global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
   }
 }
 
+/* Gram of FP32 rows on the matrix cores: v_mfma_f32_32x32x2f32 takes two
+ * rows of A per instruction.  With a = b = A[r0 + lane/32][c0 + lane%32],
+ * the instruction's A operand is A^T (32 x 2, element [m][k] from lane
+ * m + 32k) and its B operand A (2 x 32), so the 32 x 32 accumulator gathers
+ * A^T A over the rows it has seen; accumulator register i of lane l holds
+ * element (row (i%4) + 4(l/32) + 8(i/4), column l%32).  R <= 32 uses one
+ * tile, R <= 64 the three tiles (0,0), (0,1), (1,1) of the symmetric result;
+ * columns >= R are zero.  Each block sums its waves' tiles in a fixed order
+ * and writes one FP64 partial R x R (summed across blocks in FP64 by
+ * k_sum_parts).  With lambda, the rows are first divided by it and written
+ * back (and to the bf16 copy), as k_gram_part does. */
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
+                                                  bf16raw *__restrict__ Abf,
+                                                  uint64_t I, uint32_t R,
+                                                  double const *__restrict__ lambda,
+                                                  double *__restrict__ part)
+{
+  __shared__ float red[3][3][1024];
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const w = threadIdx.x >> 6;
+  uint32_t const c0 = lane & 31u, c1 = 32u + (lane & 31u);
+  bool const two = R > 32;
+  uint64_t const chunk = ((I + gridDim.x - 1) / gridDim.x + 7) & ~(uint64_t)7;
+  uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+  uint64_t const hi = lo + chunk < I ? lo + chunk : I;
+  double const l0 = (lambda && c0 < R) ? lambda[c0] : 1.0;
+  double const l1 = (lambda && two && c1 < R) ? lambda[c1] : 1.0;
+  floatx16 a00, a01, a11;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    a00[i] = 0.f;
+    a01[i] = 0.f;
+    a11[i] = 0.f;
+  }
+  for (uint64_t rp = lo + 2u * w; rp < hi; rp += 8u) {
+    uint64_t const r = rp + (lane >> 5);
+    float v0 = 0.f, v1 = 0.f;
+    if (r < hi) {
+      size_t const g0 = (size_t)r * R + c0, g1 = (size_t)r * R + c1;
+      if (c0 < R) {
+        v0 = A[g0];
+        if (lambda) {
+          v0 = (float)((double)v0 / l0);
+          A[g0] = v0;
+          if (Abf) Abf[g0].v = to_bf16(v0);
+        }
+      }
+      if (two && c1 < R) {
+        v1 = A[g1];
+        if (lambda) {
+          v1 = (float)((double)v1 / l1);
+          A[g1] = v1;
+          if (Abf) Abf[g1].v = to_bf16(v1);
+        }
+      }
+    }
+    a00 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0, v0, a00, 0, 0, 0);
+    if (two) {
+      a01 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0, v1, a01, 0, 0, 0);
+      a11 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1, v1, a11, 0, 0, 0);
+    }
+  }
+  if (w > 0) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      red[w - 1][0][i * 64 + lane] = a00[i];
+      red[w - 1][1][i * 64 + lane] = a01[i];
+      red[w - 1][2][i * 64 + lane] = a11[i];
+    }
+  }
+  __syncthreads();
+  if (w != 0) return;
+  for (uint32_t ww = 0; ww < 3; ++ww)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      a00[i] += red[ww][0][i * 64 + lane];
+      a01[i] += red[ww][1][i * 64 + lane];
+      a11[i] += red[ww][2][i * 64 + lane];
+    }
+  double *myp = part + (size_t)blockIdx.x * R * R;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    uint32_t const row = (uint32_t)(i % 4) + 4u * (lane >> 5) + 8u * (uint32_t)(i / 4);
+    uint32_t const col = lane & 31u;
+    if (row < R && col < R) myp[(size_t)row * R + col] = (double)a00[i];
+    if (two) {
+      if (row < R && 32u + col < R) {
+        myp[(size_t)row * R + 32u + col] = (double)a01[i];
+        myp[(size_t)(32u + col) * R + row] = (double)a01[i];
+      }
+      if (32u + row < R && 32u + col < R)
+        myp[(size_t)(32u + row) * R + 32u + col] = (double)a11[i];
+    }
+  }
+}
+
 __global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ part,
                                                   uint32_t nblk, uint32_t len,
                                                   double *__restrict__ out)
@@ -336,9 +434,14 @@ hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
                      double const *lambda, double *part, double *G_out)
 {
   uint32_t const nb = dense_blocks(I);
-  size_t const lds = (size_t)GROWS * R * sizeof(double);
-  hipLaunchKernelGGL((k_gram_part<T>), dim3(nb), dim3(DB), lds, s, A, Abf, I,
-                     R, lambda, part);
+  if constexpr (sizeof(T) == 4) {
+    hipLaunchKernelGGL(k_gram_mfma, dim3(nb), dim3(DB), 0, s, (float *)A, Abf, I, R,
+                       lambda, part);
+  } else {
+    size_t const lds = (size_t)GROWS * R * sizeof(double);
+    hipLaunchKernelGGL((k_gram_part<T>), dim3(nb), dim3(DB), lds, s, A, Abf, I,
+                       R, lambda, part);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint32_t const len = R * R;

@@ -45,6 +45,12 @@ template <int K>
 struct rb_of {
   static constexpr int v = (K <= 3) ? 8 : 4;
 };
+/* single-level passes load the next batch's metadata while the current one
+ * computes; multi-level passes keep the registers for occupancy instead */
+template <int K>
+struct pipe_of {
+  static constexpr bool v = (K == 1);
+};
 
 template <typename T, typename TU, bool LEAF, int K>
 __global__ __launch_bounds__(256) void k_reduce(
@@ -92,10 +98,12 @@ __global__ __launch_bounds__(256) void k_reduce(
       mx[i] = LEAF ? xin[c] : xin[(size_t)c * R + col];
     }
   };
-  load_meta(a, id, f, xv);
+  constexpr bool PIPE = pipe_of<K>::v;
+  if (PIPE) load_meta(a, id, f, xv);
 
   for (uint32_t c0 = a; c0 < b; c0 += RB) {
     uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
+    if (!PIPE) load_meta(c0, id, f, xv);
     /* closes of the intermediate levels in this batch: the q-th close of
        relative level j is node node[j] + q */
     uint32_t ncl[K];
@@ -117,19 +125,19 @@ __global__ __launch_bounds__(256) void k_reduce(
         cid[j][q] = ((uint32_t)q < ncl[j]) ? ch.ids[j][node[j] + q] : 0u;
     T uv[RB];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) uv[i] = (T)ldu(Ub + (size_t)id[i] * R + col);
+    for (int i = 0; i < RB; ++i) uv[i] = (T)ldu(Ub + (id[i] * R + col));
     T uc[K][RB];
 #pragma unroll
     for (int j = 1; j < K; ++j)
 #pragma unroll
       for (int q = 0; q < RB; ++q)
         uc[j][q] = ((uint32_t)q < ncl[j])
-                       ? (T)ldu((TU const *)ch.U[j] + (size_t)cid[j][q] * R + col)
+                       ? (T)ldu((TU const *)ch.U[j] + (cid[j][q] * R + col))
                        : (T)0;
     uint32_t nid[RB], nf[RB];
     T nxv[RB];
     uint32_t const c1 = c0 + RB;
-    if (c1 < b) load_meta(c1, nid, nf, nxv);
+    if (PIPE && c1 < b) load_meta(c1, nid, nf, nxv);
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       if ((uint32_t)i < nb) {
@@ -159,7 +167,7 @@ __global__ __launch_bounds__(256) void k_reduce(
         }
       }
     }
-    if (c1 < b) {
+    if (PIPE && c1 < b) {
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
         id[i] = nid[i];

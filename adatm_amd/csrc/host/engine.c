@@ -676,6 +676,12 @@ static int engine_begin(adatm_engine *e, uint32_t N, uint64_t const *dims,
   }
   if (precision < SPLATT_ADATM_FP64 || precision > SPLATT_ADATM_BF16ROWS)
     return SPLATT_ERROR_BADINPUT;
+  for (uint32_t m = 0; m < N; ++m)
+    if (dims[m] * rank >= (1ull << 32)) { /* factor rows are addressed in 32 bits */
+      adatm_set_error("engine: mode %u has %llu x %llu factor entries (limit 2^32)", m,
+                      (unsigned long long)dims[m], (unsigned long long)rank);
+      return SPLATT_ERROR_BADINPUT;
+    }
   e->device = device;
   e->N = N;
   e->R = (uint32_t)rank;

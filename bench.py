@@ -58,7 +58,13 @@ def main() -> int:
     ap.add_argument("--cpu-draws", type=int, default=6_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-cal", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (FP32 elementwise "
+                         "kernel, 3 times): the reference point that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_cal:
+        return copy_calibration(args.copy_cal)
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -177,6 +183,24 @@ def main() -> int:
                         for c, i, ms, b in r32["launches"] if c != 3],
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_calibration(nbytes: int) -> int:
+    """A plain read-once / write-once kernel of known size under the same
+    counters: FETCH_SIZE and WRITE_SIZE of one dispatch, divided by nbytes,
+    give the factor that turns the counters into bytes."""
+    import torch
+
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4,
+                      "kernel": "torch elementwise mul (read n, write n)"}), flush=True)
     return 0
 
 

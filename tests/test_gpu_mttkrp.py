@@ -148,6 +148,24 @@ def test_cpd_fit_matches_oracle(prec, graph):
     assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= tol, (fits, ref)
 
 
+@pytest.mark.parametrize("R", [10, 40, 64])
+def test_cpd_fit_ranks_fp32(R):
+    """FP32 sweeps at ranks that pad one (R=10) or use two (R=40, 64) 32-wide
+    Gram tiles on the matrix cores match the FP64 oracle's fits."""
+    print(f"cpd ranks R={R}", flush=True)
+    t = _tensor([90, 80, 70, 66], 40000, A.DIST_ZIPF, seed=23)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    U = _factors(t, R, seed=78)
+    cands, ch = A.plan_analytic(t, R)
+    e = A.Engine(t, R, cands[ch], precision=A.FP32)
+    for m in range(4):
+        e.set_factor(m, U[m])
+    fits = e.cpd(6, tol=0.0, graph=True)
+    ref, _, _ = O.cpd_als(lambda m, F: O.coo_mttkrp(inds, vals, F, m), U, 6,
+                          float((vals ** 2).sum()))
+    assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= 1e-4, (fits, ref)
+
+
 def test_device_fiber_counts_match_host():
     """Radix sort + boundary scan on the device give the host's (and the
     oracle's) fiber counts, including keys that need three 8-bit digits."""
