@@ -32,7 +32,8 @@ VALS_U01, VALS_LOGNORMAL, VALS_PM1 = 0, 1, 2
 OPT_NTHREADS, OPT_TOLERANCE, OPT_NITER, OPT_VERBOSITY, OPT_RANDSEED = 0, 1, 2, 3, 4
 OPT_CSF_ALLOC, OPT_TILE, OPT_TILEDEPTH, OPT_DECOMP, OPT_COMM = 5, 6, 7, 8, 9
 OPT_PRECISION, OPT_PLANNER, OPT_GRAPH, OPT_REFQUIRKS = 10, 11, 12, 13
-OPT_SPACE_LIMIT, OPT_DEVICE, OPT_NOPTIONS = 14, 15, 16
+OPT_SPACE_LIMIT, OPT_DEVICE, OPT_RANK_HINT, OPT_NOPTIONS = 14, 15, 16, 17
+PLAN_ANALYTIC, PLAN_LEARNED, PLAN_EXHAUSTIVE = 0, 1, 2
 CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
 ADAPTIVE, ADAPTIVE_TIME_EFFICIENT, ADAPTIVE_SPACE_EFFICIENT = 3, 4, 5
 
@@ -168,6 +169,10 @@ def lib() -> C.CDLL:
         "adatm_plan_learned": (C.c_int, [P(Coo), C.c_uint64, C.c_void_p, C.c_int,
                                          P(Plan), C.c_int, P(C.c_int), P(C.c_int),
                                          P(C.c_float)]),
+        "adatm_plan_exhaustive": (C.c_int, [P(Coo), C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                            C.c_int, P(Plan), C.c_int, P(C.c_int),
+                                            P(C.c_int), P(C.c_double)]),
+        "adatm_costmodel_path": (C.c_int, [C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -473,6 +478,25 @@ def plan_learned(t: Tensor, rank: int, model: "Mlp | None", device: int = 0):
                                    device, arr, 256, C.byref(nc), C.byref(ch),
                                    _fptr(pred)), "plan_learned")
     return [arr[i] for i in range(nc.value)], ch.value, pred[: nc.value]
+
+
+def plan_exhaustive(t: Tensor, rank: int, precision: int = FP32, device: int = 0,
+                    nwarm: int = 2, nrep: int = 5):
+    """(candidates, fastest index, ms per candidate) timing one CP-ALS sweep
+    (hipGraph) of every candidate on the device."""
+    arr = (Plan * 256)()
+    nc = C.c_int()
+    ch = C.c_int()
+    ms = np.zeros(256, dtype=np.float64)
+    check(lib().adatm_plan_exhaustive(C.byref(t.c), rank, precision, device, nwarm, nrep,
+                                      arr, 256, C.byref(nc), C.byref(ch), _dptr(ms)),
+          "plan_exhaustive")
+    return [arr[i] for i in range(nc.value)], ch.value, ms[: nc.value].copy()
+
+
+def costmodel_path():
+    buf = C.create_string_buffer(4096)
+    return buf.value.decode() if lib().adatm_costmodel_path(buf, 4096) else None
 
 
 # --------------------------------------------------------------------------

@@ -80,7 +80,7 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
         list(ex.map(_run, steps))
     lib = LIBDIR / LIBNAME
     tmp = LIBDIR / (LIBNAME + ".tmp")
-    _run([hipcc, "-shared", "-o", str(tmp), *map(str, objs), "-lgomp", "-lm",
+    _run([hipcc, "-shared", "-o", str(tmp), *map(str, objs), "-lgomp", "-lm", "-ldl",
           f"--offload-arch={ARCH}"])
     os.replace(tmp, lib)
     cli = CSRC / "host" / "cli.c"

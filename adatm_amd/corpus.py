@@ -62,12 +62,8 @@ def measure(k: int, nwarm: int = 2, nrep: int = 5, device: int = 0) -> dict:
     t.canonicalize()
     cands, ch = A.plan_analytic(t, R, ref_quirks=True, device=device)
     feats = A.plan_features(t, R, cands, device=device)
-    ms = np.zeros(len(cands))
-    for c, plan in enumerate(cands):
-        e = A.Engine(t, R, plan, precision=A.FP32, device=device)
-        e.init_factors(spec.seed + 1)
-        ms[c], _ = e.time(1, nwarm, nrep, graph=True, per_kernel=False)
-        e.close()
+    cx, _, ms = A.plan_exhaustive(t, R, A.FP32, device, nwarm, nrep)
+    assert [p.groups(t.nmodes) for p in cx] == [p.groups(t.nmodes) for p in cands]
     rec = dict(k=k, N=t.nmodes, nnz=t.nnz, R=R, dims=list(t.dims), nc=len(cands),
                analytic=ch, ms=ms, feats=feats,
                pops=np.array([p.pops for p in cands], dtype=np.float64),

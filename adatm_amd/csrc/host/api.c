@@ -116,6 +116,7 @@ double *splatt_default_opts(void)
   o[SPLATT_OPTION_ADATM_REFQUIRKS] = 1;
   o[SPLATT_OPTION_ADATM_SPACE_LIMIT] = (double)SPLATT_ADATM_SLIMIT;
   o[SPLATT_OPTION_ADATM_DEVICE] = 0;
+  o[SPLATT_OPTION_ADATM_RANK_HINT] = 16;
   return o;
 }
 
@@ -309,12 +310,30 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
   } else if (alloc == SPLATT_ADAPTIVE || alloc == SPLATT_ADAPTIVE_TIME_EFFICIENT) {
     adatm_plan cands[256];
     int nc = 0, ch = 0;
-    /* the analytic choice (min pops) does not depend on the rank: every
-       product count is proportional to it; rank 1 only loosens the space
-       filter, which the engine re-checks against the device memory */
-    rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
-                             opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands,
-                             256, &nc, &ch);
+    int const planner = (int)opts[SPLATT_OPTION_ADATM_PLANNER];
+    uint64_t const hint = opts[SPLATT_OPTION_ADATM_RANK_HINT] >= 1
+                              ? (uint64_t)opts[SPLATT_OPTION_ADATM_RANK_HINT]
+                              : 16;
+    if (planner == SPLATT_ADATM_PLAN_LEARNED) {
+      adatm_mlp *model = NULL;
+      char path[4096];
+      if (adatm_costmodel_path(path, sizeof(path)) &&
+          adatm_mlp_load(r->device, path, &model) != SPLATT_SUCCESS)
+        model = NULL; /* no usable model: the analytic choice */
+      rc = adatm_plan_learned(t, hint, model, r->device, cands, 256, &nc, &ch, NULL);
+      adatm_mlp_destroy(model);
+    } else if (planner == SPLATT_ADATM_PLAN_EXHAUSTIVE) {
+      int const prec = (int)opts[SPLATT_OPTION_ADATM_PRECISION];
+      rc = adatm_plan_exhaustive(t, hint, prec, r->device, 1, 3, cands, 256, &nc, &ch,
+                                 NULL);
+    } else {
+      /* the analytic choice (min pops) does not depend on the rank: every
+         product count is proportional to it; rank 1 only loosens the space
+         filter, which the engine re-checks against the device memory */
+      rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
+                               opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands,
+                               256, &nc, &ch);
+    }
     if (rc == SPLATT_SUCCESS) {
       r->plan = cands[ch];
       adatm_plan_use(&r->plan, N, r->use_grp, r->use_tag);

@@ -9,6 +9,8 @@
  * by the HIP kernels of mlp.hip on timings the engine itself produces on
  * synthetic tensors; without a model the analytic choice is used.
  */
+#define _GNU_SOURCE
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -689,4 +691,66 @@ int adatm_plan_learned(adatm_coo const *t, uint64_t R, adatm_mlp *model,
   free(feat);
   free(y);
   return rc;
+}
+
+/* Exhaustive search: every candidate built on the device and one CP-ALS
+ * sweep (hipGraph) timed, median of nrep after nwarm; a candidate that
+ * cannot be built (device memory) gets +inf.  ms (nullable) receives the
+ * timings. */
+int adatm_plan_exhaustive(adatm_coo const *t, uint64_t R, int precision,
+                          int device, int nwarm, int nrep, adatm_plan *cands,
+                          int maxc, int *ncands, int *chosen, double *ms)
+{
+  int ach = 0;
+  int rc = adatm_plan_analytic(t, R, 1, (double)SPLATT_ADATM_SLIMIT, device, cands,
+                               maxc, ncands, &ach);
+  if (rc != SPLATT_SUCCESS) return rc;
+  int best = -1;
+  double bt = INFINITY;
+  for (int c = 0; c < *ncands; ++c) {
+    adatm_engine *e = NULL;
+    double m = INFINITY;
+    if (adatm_engine_create(t, R, cands + c, precision, device, &e) == SPLATT_SUCCESS) {
+      if (adatm_engine_init_factors(e, 1) != SPLATT_SUCCESS ||
+          adatm_engine_time(e, 1, nwarm, nrep, 1, &m, NULL) != SPLATT_SUCCESS)
+        m = INFINITY;
+      adatm_engine_destroy(e);
+    }
+    if (ms) ms[c] = m;
+    if (m < bt) {
+      bt = m;
+      best = c;
+    }
+  }
+  if (best < 0) {
+    adatm_set_error("exhaustive planner: no candidate could be built on device %d",
+                    device);
+    return SPLATT_ERROR_NOMEMORY;
+  }
+  adatm_set_error("%s", "");
+  *chosen = best;
+  return SPLATT_SUCCESS;
+}
+
+/* The cost model file: $ADATM_COSTMODEL, else planner/costmodel.bin next to
+ * the library's directory (lib/../planner).  Returns 0 when there is none. */
+int adatm_costmodel_path(char *buf, size_t n)
+{
+  char const *env = getenv("ADATM_COSTMODEL");
+  if (env && env[0]) {
+    snprintf(buf, n, "%s", env);
+    return 1;
+  }
+  Dl_info info;
+  if (!dladdr((void *)&adatm_costmodel_path, &info) || !info.dli_fname) return 0;
+  char dir[4096];
+  snprintf(dir, sizeof(dir), "%s", info.dli_fname);
+  char *slash = strrchr(dir, '/');
+  if (!slash) return 0;
+  *slash = 0;
+  snprintf(buf, n, "%s/../planner/costmodel.bin", dir);
+  FILE *f = fopen(buf, "rb");
+  if (!f) return 0;
+  fclose(f);
+  return 1;
 }

@@ -2,16 +2,17 @@
 
 Workload (the configuration the metric is quoted on): 5-mode tensor
 100000 x 50000 x 20000 x 5000 x 1000, 100M coordinates drawn per mode from
-Zipf(1.1), lognormal(0,1) values, duplicates summed, rank 32; FP32 factors
-and memoised intermediates with FP64 solve and fit; analytic memoization plan;
-every ALS sweep (N memoised MTTKRPs, Grams, (hadamard of Grams)^-1, column
-normalisation, fit) captured in one hipGraph.  A step is one ALS sweep.
+Zipf(1.1), lognormal(0,1) values, duplicates summed, rank 32.  A step is one
+ALS sweep (N memoised MTTKRPs, Grams, (hadamard of Grams)^-1, column
+normalisation, fit) captured in one hipGraph.
 
-Printed: one JSON line with ms per sweep (value), the FP64 sweep on the same
-plan for comparison, the roofline of the dominant kernel (algorithmic bytes
-over its average launch time, measured here with HIP events on the engine's
-stream), and the C/OpenMP FP64 oracle timed on the host cores on a bounded
-sample of the same workload.
+Printed: one JSON line whose value is ms per sweep of the learned plan (the
+trained cost model's choice, adatm_amd/planner/costmodel.bin; the analytic
+plan when there is no model) with FP32 factors and memo and FP64 solve/fit,
+next to the plain analytic plan swept in FP64; the roofline of the dominant
+MTTKRP launch (algorithmic bytes over its duration, measured here with HIP
+events on the engine's stream); and the C/OpenMP FP64 oracle timed on the
+host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -31,6 +32,9 @@ KC_NAMES = {0: "k_reduce (leaf SpTTM)", 1: "k_reduce (level Hadamard-reduce)",
             2: "k_scatter (memo reuse)", 3: "dense ALS steps",
             4: "k_fix_chunk/k_fix_multi (split-parent merge)"}
 MTTKRP_CLASSES = (0, 1, 2, 4)
+WORKLOAD_TEXT = {1: "uniform coordinates, U(0,1) values",
+                 2: "Zipf(1.1) coordinates per mode, lognormal values",
+                 3: "100000 Gaussian clusters sigma 20, U(-1,1) values"}
 
 
 def launch_name(cls: int, info: int, leaf: bool = True) -> str:
@@ -58,6 +62,9 @@ def main() -> int:
     ap.add_argument("--cpu-draws", type=int, default=6_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
+    ap.add_argument("--bf16", action="store_true",
+                    help="also sweep with BF16 factor-row storage (FP32 accumulation)")
     ap.add_argument("--copy-cal", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (FP32 elementwise "
                          "kernel, 3 times): the reference point that scales the "
@@ -68,7 +75,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (the sharded design is described in DESIGN.md)", file=sys.stderr)
+              "supported (multi-GPU sharding by root subtrees is designed, not built)",
+              file=sys.stderr)
         return 2
 
     import numpy as np
@@ -86,20 +94,39 @@ def main() -> int:
     cands, ch = A.plan_analytic(t, R, ref_quirks=True, device=0)
     plan = cands[ch]
     log(f"analytic plan {ch}: {plan.groups(t.nmodes)} ({time.time() - t1:.1f}s)")
+    model_path = args.model or os.path.join(ROOT, "adatm_amd", "planner", "costmodel.bin")
+    learned = None
+    if os.path.exists(model_path):
+        t1 = time.time()
+        model = A.Mlp(device=0, path=model_path)
+        lc, lch, pred = A.plan_learned(t, R, model, device=0)
+        model.close()
+        learned = (lch, lc[lch])
+        log(f"learned plan {lch}: {lc[lch].groups(t.nmodes)} (predicted log10 ms "
+            f"{[round(float(x), 3) for x in pred]}, {time.time() - t1:.1f}s)")
+    else:
+        log(f"no trained cost model at {model_path}: learned plan = analytic plan")
+    lplan_idx, lplan = learned if learned else (ch, plan)
 
+    runs = [("learned_fp32", lplan, A.FP32)]
+    if args.bf16:
+        runs.append(("learned_bf16", lplan, A.BF16ROWS))
+    if lplan_idx != ch:
+        runs.append(("analytic_fp32", plan, A.FP32))
+    if not args.no_fp64:
+        runs.append(("plain_fp64", plan, A.FP64))
     res = {}
-    for name, prec in (("fp32", A.FP32), ("fp64", A.FP64)):
-        if name == "fp64" and args.no_fp64:
-            continue
+    Mref = {}
+    for name, pl, prec in runs:
         t2 = time.time()
-        e = A.Engine(t, R, plan, precision=prec, device=0)
+        e = A.Engine(t, R, pl, precision=prec, device=0)
         e.init_factors(spec.seed + 1)
         log(f"{name} engine built ({time.time() - t2:.1f}s)")
         if args.warmup > 0:
             e.run_timed(1, args.warmup, graph=True)
         ms = e.run_timed(1, args.steps, graph=True)
         fits = e.fits(args.steps)
-        seq_ms, _ = e.time(0, 1, 5)
+        seq_ms, _ = e.time(0, 1, 5, per_kernel=False)
         _, pk = e.time(1, 0, 1, graph=False)
         launch_log = e.launches()
         F, seq_bytes, seq_flops, dev_bytes = e.stats()
@@ -109,25 +136,22 @@ def main() -> int:
                          nfibs=F.tolist(), launches=launch_log)
         log(f"{name}: {ms / args.steps:.3f} ms/sweep, MTTKRP seq {seq_ms:.3f} ms, "
             f"fit {fits[-1]:.6f}, device {dev_bytes / 2**30:.1f} GiB")
-        # MTTKRP error of this precision against the FP64 path (pinned to the
-        # oracle by the GPU parity tests), same factors
-        if name == "fp32":
-            e.init_factors(spec.seed + 1)
-            M32 = [e.mttkrp(m) for m in range(t.nmodes)]
-            res[name]["M"] = M32
-        else:
-            e.init_factors(spec.seed + 1)
-            errs = []
-            for m in range(t.nmodes):
-                M64 = e.mttkrp(m)
-                if "fp32" in res:
-                    d = np.linalg.norm(res["fp32"]["M"][m] - M64) / np.linalg.norm(M64)
-                    errs.append(float(d))
-            res["mttkrp_rel_err_fp32_vs_fp64"] = errs
+        # MTTKRP of every mode from the same factors, to compare precisions
+        e.init_factors(spec.seed + 1)
+        Mref[name] = [e.mttkrp(m) for m in range(t.nmodes)]
         e.close()
-    res.get("fp32", {}).pop("M", None)
+    errs = errs_bf16 = None
+    if "plain_fp64" in Mref:
+        def rel(name):
+            return [float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                    for a, b in zip(Mref[name], Mref["plain_fp64"])]
+        errs = rel("learned_fp32")
+        if "learned_bf16" in Mref:
+            errs_bf16 = rel("learned_bf16")
+    Mref.clear()
 
-    r32 = res["fp32"]
+    r32 = res["learned_fp32"]
+    r64 = res.get("plain_fp64")
     pk = r32["pk"]
     times = {c: pk[c] for c in range(5)}
     # dominant single launch of the MTTKRP kernels (one instrumented sweep)
@@ -157,10 +181,12 @@ def main() -> int:
         "config": {
             "workload": (f"C{args.config}: {t.nmodes}-mode "
                          f"{'x'.join(str(d) for d in spec.dims[:t.nmodes])}, "
-                         f"{ndraws} Zipf(1.1) draws -> {t.nnz} nnz, lognormal values, "
-                         f"R={R}, analytic plan, one hipGraph per ALS sweep"),
-            "nnz": t.nnz, "rank": R, "plan": plan.groups(t.nmodes),
-            "plan_index": ch,
+                         f"{ndraws} draws -> {t.nnz} nnz ({WORKLOAD_TEXT.get(args.config, '')}), "
+                         f"R={R}, one hipGraph per ALS sweep"),
+            "nnz": t.nnz, "rank": R,
+            "planner": "learned (MLP cost model)" if learned else "analytic (no trained model)",
+            "learned_plan": lplan.groups(t.nmodes), "learned_plan_index": lplan_idx,
+            "analytic_plan": plan.groups(t.nmodes), "analytic_plan_index": ch,
         },
         "roofline": {
             "bound": "hbm", "kernel": launch_name(dcls, dinfo),
@@ -169,15 +195,22 @@ def main() -> int:
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
-        "fp64_ms_per_sweep": round(res["fp64"]["ms_per_sweep"], 4) if "fp64" in res else None,
-        "speedup_fp32_vs_fp64": (round(res["fp64"]["ms_per_sweep"] / r32["ms_per_sweep"], 3)
-                                 if "fp64" in res else None),
+        "plain_fp64_ms_per_sweep": round(r64["ms_per_sweep"], 4) if r64 else None,
+        "analytic_fp32_ms_per_sweep": (round(res["analytic_fp32"]["ms_per_sweep"], 4)
+                                       if "analytic_fp32" in res else
+                                       round(r32["ms_per_sweep"], 4)),
+        "speedup_learned_fp32_vs_plain_fp64": (round(r64["ms_per_sweep"] / r32["ms_per_sweep"], 3)
+                                               if r64 else None),
         "mttkrp_seq_ms": round(r32["seq_ms"], 4),
         "mttkrp_GBps": round(mttkrp_bytes / (mttkrp_ms * 1e-3) / 1e9, 1),
         "mttkrp_hbm_frac": round(mttkrp_bytes / (mttkrp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "fit_fp32": r32["fit_last"],
-        "fit_fp64": res["fp64"]["fit_last"] if "fp64" in res else None,
-        "mttkrp_rel_err_fp32_vs_fp64": res.get("mttkrp_rel_err_fp32_vs_fp64"),
+        "fit_learned_fp32": r32["fit_last"],
+        "fit_plain_fp64": r64["fit_last"] if r64 else None,
+        "mttkrp_rel_err_fp32_vs_fp64": errs,
+        "bf16_rows_ms_per_sweep": (round(res["learned_bf16"]["ms_per_sweep"], 4)
+                                   if "learned_bf16" in res else None),
+        "fit_bf16_rows": res["learned_bf16"]["fit_last"] if "learned_bf16" in res else None,
+        "mttkrp_rel_err_bf16_rows_vs_fp64": errs_bf16,
         "kernel_ms": {KC_NAMES[c]: round(times[c], 4) for c in range(5)},
         "launches_ms": [[launch_name(c, i), round(ms, 4), round(b / 1e9, 4)]
                         for c, i, ms, b in r32["launches"] if c != 3],

@@ -175,6 +175,15 @@ int adatm_plan_features_from_counts(uint32_t nmodes, uint64_t nnz,
                                     uint64_t const *dims, uint64_t rank,
                                     adatm_plan const *p, uint64_t const *Fg,
                                     float *feat);
+/* Every candidate timed on the device (one CP-ALS sweep in a hipGraph at
+ * `precision`, median of nrep after nwarm); the chosen one is the fastest.
+ * ms (nullable, one per candidate) receives the timings (+inf: not built). */
+int adatm_plan_exhaustive(adatm_coo const *t, uint64_t rank, int precision,
+                          int device, int nwarm, int nrep, adatm_plan *cands,
+                          int maxc, int *ncands, int *chosen, double *ms);
+/* Path of the trained cost model ($ADATM_COSTMODEL, else
+ * planner/costmodel.bin beside the library's lib/ directory); 0 if none. */
+int adatm_costmodel_path(char *buf, size_t n);
 /* Candidates as adatm_plan_analytic; the chosen one minimises the model's
  * prediction (pred, nullable, receives them), or is the analytic choice when
  * model is NULL. */

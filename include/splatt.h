@@ -72,6 +72,9 @@ typedef enum {
                                     statistics reuse (adatm_bench.c:246-262) */
   SPLATT_OPTION_ADATM_SPACE_LIMIT, /* bytes; reference SLIMIT = 90e9 */
   SPLATT_OPTION_ADATM_DEVICE,    /* GPU ordinal */
+  SPLATT_OPTION_ADATM_RANK_HINT, /* rank the learned / exhaustive planners
+                                    plan for (CSFs are built before the rank
+                                    of a decomposition is known) */
   SPLATT_OPTION_NOPTIONS
 } splatt_option_type;
 

@@ -120,3 +120,13 @@ def test_plan_learned_picks_min_prediction():
     f = A.plan_features(t, 16, cands, device=0)
     assert np.allclose(pred, m.predict(f))
     assert ch == int(np.argmin(pred))
+
+
+def test_plan_exhaustive_times_every_candidate():
+    print("exhaustive planner", flush=True)
+    t = _tensor(5)
+    cands, ch, ms = A.plan_exhaustive(t, 16, A.FP32, 0, 1, 3)
+    ac, _ = A.plan_analytic(t, 16, device=0)
+    assert [c.groups(5) for c in cands] == [c.groups(5) for c in ac]
+    assert np.isfinite(ms).all() and (ms > 0).all()
+    assert ch == int(np.argmin(ms))

@@ -148,6 +148,27 @@ def test_splatt_cpd_als(prec, tol):
     L.splatt_free_csf(csf, opts)
 
 
+@pytest.mark.parametrize("planner", [A.PLAN_LEARNED, A.PLAN_EXHAUSTIVE])
+def test_planner_options(planner, tmp_path, monkeypatch):
+    """The learned (analytic fallback without a model) and exhaustive planners
+    behind splatt_csf_convert give CSF sets whose MTTKRPs match the oracle."""
+    print(f"planner option {planner}", flush=True)
+    monkeypatch.setenv("ADATM_COSTMODEL", str(tmp_path / "missing.bin"))
+    L = _lib()
+    inds, vals = _tensor(5, [40, 30, 20, 12], 6000)
+    opts = L.splatt_default_opts()
+    opts[A.OPT_PLANNER] = planner
+    opts[A.OPT_RANK_HINT] = 8
+    csf = _convert(L, inds, vals, opts)
+    rng = np.random.default_rng(1)
+    U = [np.ascontiguousarray(rng.random((d, 8))) for d in [40, 30, 20, 12]]
+    for m in range(4):
+        ref = O.coo_mttkrp(inds, vals, U, m)
+        assert O.rel_fro(_mttkrp(L, csf, m, U, opts), ref) <= 1e-12
+    L.splatt_free_csf(csf, opts)
+    L.splatt_free_opts(opts)
+
+
 def test_cli_bench_and_cpd(tmp_path):
     exe = str(A.CLI_PATH)
     r = subprocess.run([exe, "bench", "synth:1:200000", "-a", "adatm", "-i", "2", "-r", "16"],
