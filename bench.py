@@ -47,6 +47,27 @@ def launch_name(cls: int, info: int, leaf: bool = True) -> str:
     return KC_NAMES[cls]
 
 
+def template_name(cls: int, info: int, nmodes: int) -> str:
+    """The kernel's template instantiation as rocprofv3 names it (FP32)."""
+    if cls in (0, 1):
+        return f"k_reduce<float, float, {'true' if cls == 0 else 'false'}, {info}>"
+    return f"k_scatter<float, float, {info}, {'true' if info == nmodes - 1 else 'false'}>"
+
+
+def measured_traffic(config: int, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"c{config}_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("TRAFFIC_JSON "):
+                t = json.loads(line[len("TRAFFIC_JSON "):]).get(kernel)
+                return (t["fetch_bytes"] + t["write_bytes"]) if t else None
+    except OSError:
+        return None
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -191,7 +212,9 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "kernel": launch_name(dcls, dinfo),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(args.config, template_name(dcls, dinfo, t.nmodes)),
+            "traffic_source": f"profiles/c{args.config}_hbm_traffic.md (rocprofv3 --pmc, copy-calibrated)",
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
