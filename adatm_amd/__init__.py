@@ -122,6 +122,11 @@ def lib() -> C.CDLL:
         "adatm_engine_create": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
                                           P(C.c_void_p)]),
         "adatm_engine_destroy": (None, [C.c_void_p]),
+        "adatm_engine_create_tiled": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
+                                                C.c_uint32, C.c_uint32, P(C.c_void_p)]),
+        "adatm_autotune_tiles": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
+                                           C.c_int, P(C.c_uint32), P(C.c_uint32),
+                                           P(C.c_double)]),
         "adatm_engine_set_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_get_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_get_lambda": (C.c_int, [C.c_void_p, P(C.c_double)]),
@@ -494,6 +499,18 @@ def plan_exhaustive(t: Tensor, rank: int, precision: int = FP32, device: int = 0
     return [arr[i] for i in range(nc.value)], ch.value, ms[: nc.value].copy()
 
 
+def autotune_tiles(t: Tensor, rank: int, plan: Plan, precision: int = FP32,
+                   device: int = 0, nrep: int = 3):
+    """((reduction tile, scatter tile), best ms per sweep) by timing."""
+    r = C.c_uint32()
+    s = C.c_uint32()
+    ms = C.c_double()
+    check(lib().adatm_autotune_tiles(C.byref(t.c), rank, C.byref(plan), precision, device,
+                                     nrep, C.byref(r), C.byref(s), C.byref(ms)),
+          "autotune_tiles")
+    return (r.value, s.value), ms.value
+
+
 def costmodel_path():
     buf = C.create_string_buffer(4096)
     return buf.value.decode() if lib().adatm_costmodel_path(buf, 4096) else None
@@ -504,15 +521,16 @@ class Engine:
     """A memoization plan resident on one GPU (wraps adatm_engine)."""
 
     def __init__(self, t: Tensor, rank: int, plan: Plan, precision: int = FP64,
-                 device: int = 0):
+                 device: int = 0, tiles=(0, 0)):
         self.h = C.c_void_p()
         self.N = t.nmodes
         self.R = int(rank)
         self.dims = t.dims
         self.plan = plan
         self.precision = precision
-        check(lib().adatm_engine_create(C.byref(t.c), rank, C.byref(plan), precision,
-                                        device, C.byref(self.h)), "engine_create")
+        check(lib().adatm_engine_create_tiled(C.byref(t.c), rank, C.byref(plan), precision,
+                                              device, int(tiles[0]), int(tiles[1]),
+                                              C.byref(self.h)), "engine_create")
 
     def close(self) -> None:
         if self.h:

@@ -87,6 +87,7 @@ struct adatm_engine {
   uint64_t nnz;
   uint64_t dims[MAXN];
   uint64_t maxdim;
+  uint32_t red_tile, scat_tile; /* children / nodes per work item */
   adatm_plan plan;
   uint32_t use_grp[MAXN], use_tag[MAXN], level_of[MAXN][MAXN];
   dgroup g[MAXN];
@@ -487,7 +488,7 @@ static int build_pass(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
     cp[p] = (uint32_t)v;
   }
   adatm_segwork w;
-  int rc = adatm_segwork_build(cp, np, RED_TILE, &w);
+  int rc = adatm_segwork_build(cp, np, e->red_tile, &w);
   free(cp);
   if (rc != SPLATT_SUCCESS) return rc;
   uint32_t const K = lb - lt;
@@ -551,7 +552,7 @@ static int build_scatter(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
       rid[(size_t)(l - 1) * Fd + q] = h->fids[l - 1][v];
     }
   }
-  rc = adatm_segwork_build(rowptr, nrows, SCAT_TILE, &w);
+  rc = adatm_segwork_build(rowptr, nrows, e->scat_tile, &w);
   if (rc != SPLATT_SUCCESS) goto out;
   S->d = d;
   rc = dupload(e, (void **)&S->node, perm, Fd * sizeof(uint32_t));
@@ -685,6 +686,8 @@ static int engine_begin(adatm_engine *e, uint32_t N, uint64_t const *dims,
   e->device = device;
   e->N = N;
   e->R = (uint32_t)rank;
+  if (!e->red_tile) e->red_tile = RED_TILE;
+  if (!e->scat_tile) e->scat_tile = SCAT_TILE;
   e->prec = precision;
   e->vsz = (precision == SPLATT_ADATM_FP64) ? 8 : 4;
   e->wsz = (precision == SPLATT_ADATM_FP64) ? 8
@@ -1038,7 +1041,7 @@ static int build_pass_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t lb,
   dpass *P = g->pass + g->npass;
   P->lb = lb;
   P->lt = lt;
-  int rc = dwork_build(e, cp, np, RED_TILE, &P->w, B->stemp, B->tot);
+  int rc = dwork_build(e, cp, np, e->red_tile, &P->w, B->stemp, B->tot);
   hipFree(cp);
   if (rc != SPLATT_SUCCESS) return rc;
   uint32_t const K = lb - lt;
@@ -1098,7 +1101,7 @@ static int build_scatter_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t d)
   KCHK(adatm_k_scatter_rids(s, S->node, Fd, d, (uint32_t const *const *)B->par,
                             fids, (uint32_t *const *)S->rid),
        "scatter rids");
-  rc = dwork_build(e, S->rowptr, nrows, SCAT_TILE, &S->w, B->stemp, B->tot);
+  rc = dwork_build(e, S->rowptr, nrows, e->scat_tile, &S->w, B->stemp, B->tot);
   HIPCHK(hipStreamSynchronize(s), "scatter build sync");
   hipFree(keys);
   hipFree(sortmp);
@@ -1266,13 +1269,28 @@ int adatm_engine_create(adatm_coo const *t, uint64_t rank,
                         adatm_plan const *plan, int precision, int device,
                         adatm_engine **out)
 {
+  return adatm_engine_create_tiled(t, rank, plan, precision, device, 0, 0, out);
+}
+
+int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
+                              adatm_plan const *plan, int precision, int device,
+                              uint32_t red_tile, uint32_t scat_tile,
+                              adatm_engine **out)
+{
   *out = NULL;
   uint32_t const N = t->nmodes;
   if (N < 3 || N > MAXN || t->nnz == 0 || t->nnz >= (1ull << 32))
     return SPLATT_ERROR_BADINPUT;
+  if ((red_tile && (red_tile < 8 || red_tile > 4096)) ||
+      (scat_tile && (scat_tile < 8 || scat_tile > 4096))) {
+    adatm_set_error("engine: work-item tiles must be in [8, 4096]");
+    return SPLATT_ERROR_BADINPUT;
+  }
   adatm_set_error("%s", "");
   adatm_engine *e = (adatm_engine *)calloc(1, sizeof(adatm_engine));
   if (!e) return SPLATT_ERROR_NOMEMORY;
+  e->red_tile = red_tile;
+  e->scat_tile = scat_tile;
   int rc = engine_begin(e, N, t->dims, t->nnz, rank, plan, precision, device,
                         NULL, NULL);
   dcoo c;
@@ -1832,4 +1850,39 @@ uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
 {
   if (!e || grp >= e->plan.n_grp || level >= e->N) return 0;
   return e->g[grp].memo[level] ? e->g[grp].F[level] * e->R * e->vsz : 0;
+}
+
+/* Work-item tiles of one plan chosen by timing: the reduction tile over
+ * {64, 128, 256, 512} with the scatter tile at its default, then the scatter
+ * tile over {32, 64, 128, 256} with the best reduction tile; each candidate
+ * is a fresh engine timed on nrep CP-ALS sweeps (hipGraph, median). */
+int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                         int precision, int device, int nrep, uint32_t *red_tile,
+                         uint32_t *scat_tile, double *best_ms)
+{
+  static uint32_t const RT[4] = {64, 128, 256, 512}, ST[4] = {32, 64, 128, 256};
+  uint32_t br = RED_TILE, bs = SCAT_TILE;
+  double bt = INFINITY;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < 4; ++i) {
+      uint32_t const r = pass == 0 ? RT[i] : br, sc = pass == 0 ? SCAT_TILE : ST[i];
+      if (pass == 1 && sc == SCAT_TILE) continue; /* timed in the first pass */
+      adatm_engine *e = NULL;
+      int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, r, sc, &e);
+      if (rc != SPLATT_SUCCESS) return rc;
+      double ms = INFINITY;
+      rc = adatm_engine_init_factors(e, 1);
+      if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
+      adatm_engine_destroy(e);
+      if (rc != SPLATT_SUCCESS) return rc;
+      if (ms < bt) {
+        bt = ms;
+        br = r;
+        bs = sc;
+      }
+    }
+  *red_tile = br;
+  *scat_tile = bs;
+  if (best_ms) *best_ms = bt;
+  return SPLATT_SUCCESS;
 }

@@ -199,6 +199,17 @@ typedef struct adatm_engine adatm_engine;
 int adatm_engine_create(adatm_coo const *t, uint64_t rank,
                         adatm_plan const *plan, int precision, int device,
                         adatm_engine **out);
+/* The same with the work-item tiles given: bottom children per reduction
+ * item and nodes per scatter item (0 = default 128 / 64; range 8..4096). */
+int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
+                              adatm_plan const *plan, int precision, int device,
+                              uint32_t red_tile, uint32_t scat_tile,
+                              adatm_engine **out);
+/* Autotuner: the tiles of one plan chosen by timing CP-ALS sweeps (nrep,
+ * median) of engines built with each candidate. */
+int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                         int precision, int device, int nrep, uint32_t *red_tile,
+                         uint32_t *scat_tile, double *best_ms);
 void adatm_engine_destroy(adatm_engine *e);
 
 /* Factor matrices are exchanged in FP64 row-major (dims[m] x rank). */

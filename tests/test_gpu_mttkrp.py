@@ -166,6 +166,27 @@ def test_cpd_fit_ranks_fp32(R):
     assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= 1e-4, (fits, ref)
 
 
+def test_tiles_and_autotuner():
+    """Work-item tiles change only the summation grouping: every tile gives
+    the oracle's MTTKRP; the autotuner's choice is one of its candidates."""
+    print("tiles and autotuner", flush=True)
+    t = _tensor([300, 200, 100, 50], 60000, A.DIST_ZIPF, seed=29)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    R = 16
+    U = _factors(t, R, seed=5)
+    cands, ch = A.plan_analytic(t, R)
+    ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(4)]
+    for tiles in ((8, 8), (64, 32), (512, 256), (4096, 4096)):
+        e = A.Engine(t, R, cands[ch], precision=A.FP64, tiles=tiles)
+        for m in range(4):
+            e.set_factor(m, U[m])
+        for m in range(4):
+            assert O.rel_fro(e.mttkrp(m), ref[m]) <= 1e-12, (tiles, m)
+        e.close()
+    (rt, st), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
+    assert rt in (64, 128, 256, 512) and st in (32, 64, 128, 256) and ms > 0
+
+
 def test_device_fiber_counts_match_host():
     """Radix sort + boundary scan on the device give the host's (and the
     oracle's) fiber counts, including keys that need three 8-bit digits."""
