@@ -146,6 +146,10 @@ def lib() -> C.CDLL:
                                            P(P(C.c_uint32)), P(C.c_double)]),
         "adatm_engine_launches": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int),
                                             P(C.c_double), P(C.c_double)]),
+        "adatm_batch_create": (C.c_int, [P(C.c_void_p), C.c_int, P(C.c_void_p)]),
+        "adatm_batch_destroy": (None, [C.c_void_p]),
+        "adatm_batch_cpd": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double), P(C.c_double)]),
+        "adatm_batch_sequential": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_mlp_nparams": (C.c_size_t, []),
         "adatm_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(C.c_void_p)]),
         "adatm_mlp_destroy": (None, [C.c_void_p]),
@@ -638,3 +642,38 @@ class Engine:
         vals = np.zeros(nnz, dtype=np.float64)
         check(lib().adatm_engine_get_csf(self.h, g, pf, pi, _dptr(vals)), "get_csf")
         return F, fptr, fids, vals
+
+
+class Batch:
+    """Batched CP-ALS over independent engines (wraps adatm_batch); keeps
+    the engines alive while it exists."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        self.h = C.c_void_p()
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        check(lib().adatm_batch_create(arr, len(self.engines), C.byref(self.h)),
+              "batch_create")
+
+    def cpd(self, niters: int):
+        """(device ms of niters batched sweeps, fits (n, niters))."""
+        ms = C.c_double()
+        fits = np.zeros((len(self.engines), niters), dtype=np.float64)
+        check(lib().adatm_batch_cpd(self.h, niters, C.byref(ms), _dptr(fits)), "batch_cpd")
+        return ms.value, fits
+
+    def sequential(self, niters: int) -> float:
+        ms = C.c_double()
+        check(lib().adatm_batch_sequential(self.h, niters, C.byref(ms)), "batch_sequential")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().adatm_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -91,6 +91,9 @@ void adatm_plan_model(uint32_t N, uint64_t nnz, uint64_t R, adatm_plan const *P,
 int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
                                    uint32_t norders, uint64_t *F, int device);
 
+/* sum of squares in a fixed association (thread-count independent) */
+double adatm_sumsq(double const *v, uint64_t n);
+
 /* ---- seconds since an arbitrary origin --------------------------------- */
 double adatm_wtime(void);
 

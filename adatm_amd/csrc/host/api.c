@@ -364,9 +364,7 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
     free(r);
     return rc;
   }
-  double ss = 0.0;
-  for (uint64_t j = 0; j < t->nnz; ++j) ss += t->vals[j] * t->vals[j];
-  r->ttnormsq = ss;
+  r->ttnormsq = adatm_sumsq(t->vals, t->nnz);
   r->csf = arr;
   r->n_csf = ng;
   reg_add(r);

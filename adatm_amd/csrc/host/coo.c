@@ -570,3 +570,23 @@ int adatm_coo_alloc(uint32_t nmodes, uint64_t nnz, uint64_t const *dims,
   }
   return SPLATT_SUCCESS;
 }
+
+/* Sum of squares in a fixed association (1024 contiguous blocks summed in
+ * order), so the value does not depend on the thread count or timing. */
+double adatm_sumsq(double const *v, uint64_t n)
+{
+  enum { NB = 1024 };
+  double part[NB];
+  uint64_t const chunk = (n + NB - 1) / NB;
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < NB; ++b) {
+    uint64_t const lo = (uint64_t)b * chunk;
+    uint64_t const hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0;
+    for (uint64_t j = lo; j < hi; ++j) s += v[j] * v[j];
+    part[b] = s;
+  }
+  double s = 0.0;
+  for (int b = 0; b < NB; ++b) s += part[b];
+  return s;
+}

@@ -1300,10 +1300,7 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
     rc = engine_add_group_dev(e, gi, &c);
   dcoo_free(&c);
   if (rc == SPLATT_SUCCESS) rc = engine_finish(e);
-  double ss = 0.0;
-  #pragma omp parallel for reduction(+ : ss)
-  for (int64_t j = 0; j < (int64_t)t->nnz; ++j) ss += t->vals[j] * t->vals[j];
-  e->ttnormsq = ss;
+  e->ttnormsq = adatm_sumsq(t->vals, t->nnz);
   if (rc != SPLATT_SUCCESS) {
     char msg[512];
     snprintf(msg, sizeof(msg), "%s", adatm_last_error());
@@ -1884,5 +1881,168 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
   *red_tile = br;
   *scat_tile = bs;
   if (best_ms) *best_ms = bt;
+  return SPLATT_SUCCESS;
+}
+
+/* ======================================================================== *
+ * Batched CP-ALS: many independent engines on one device, every instance's
+ * ALS sweep captured in one hipGraph.  Each instance runs on its own stream,
+ * forked from and joined to the batch stream inside the capture, so the
+ * graph's branches are independent and small instances run side by side.
+ * ======================================================================== */
+struct adatm_batch {
+  int n, device;
+  adatm_engine **e;
+  hipStream_t s;
+  hipEvent_t fork, *join;
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+  int gvalid;
+};
+
+void adatm_batch_destroy(adatm_batch *b)
+{
+  if (!b) return;
+  if (b->gvalid) {
+    hipGraphExecDestroy(b->gexec);
+    hipGraphDestroy(b->graph);
+  }
+  if (b->join)
+    for (int i = 0; i < b->n; ++i)
+      if (b->join[i]) hipEventDestroy(b->join[i]);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->s) hipStreamDestroy(b->s);
+  free(b->join);
+  free(b->e);
+  free(b);
+}
+
+int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
+{
+  *out = NULL;
+  if (!engines || n < 1) return SPLATT_ERROR_BADINPUT;
+  for (int i = 0; i < n; ++i)
+    if (!engines[i] || engines[i]->device != engines[0]->device) {
+      adatm_set_error("batch: engine %d is missing or on another device", i);
+      return SPLATT_ERROR_BADINPUT;
+    }
+  adatm_batch *b = (adatm_batch *)calloc(1, sizeof(adatm_batch));
+  if (!b) return SPLATT_ERROR_NOMEMORY;
+  b->n = n;
+  b->device = engines[0]->device;
+  b->e = (adatm_engine **)malloc((size_t)n * sizeof(adatm_engine *));
+  b->join = (hipEvent_t *)calloc((size_t)n, sizeof(hipEvent_t));
+  if (!b->e || !b->join) {
+    adatm_batch_destroy(b);
+    return SPLATT_ERROR_NOMEMORY;
+  }
+  memcpy(b->e, engines, (size_t)n * sizeof(adatm_engine *));
+  int rc = SPLATT_SUCCESS;
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    rc = hip_fail(hipGetLastError(), "batch stream");
+  for (int i = 0; rc == SPLATT_SUCCESS && i < n; ++i)
+    if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess)
+      rc = hip_fail(hipGetLastError(), "batch event");
+  if (rc != SPLATT_SUCCESS) {
+    adatm_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return SPLATT_SUCCESS;
+}
+
+static int batch_graph(adatm_batch *b)
+{
+  if (b->gvalid) return SPLATT_SUCCESS;
+  HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal), "begin capture");
+  int rc = SPLATT_SUCCESS;
+  if (hipEventRecord(b->fork, b->s) != hipSuccess) rc = SPLATT_ERROR_DEVICE;
+  for (int i = 0; rc == SPLATT_SUCCESS && i < b->n; ++i) {
+    adatm_engine *e = b->e[i];
+    if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) {
+      rc = SPLATT_ERROR_DEVICE;
+      break;
+    }
+    rc = sweep_enqueue(e);
+    if (rc == SPLATT_SUCCESS &&
+        (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
+         hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess))
+      rc = SPLATT_ERROR_DEVICE;
+  }
+  hipGraph_t g = NULL;
+  hipError_t const ec = hipStreamEndCapture(b->s, &g);
+  if (rc != SPLATT_SUCCESS || ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (rc == SPLATT_SUCCESS) rc = hip_fail(ec, "end capture");
+    else if (!adatm_last_error()[0]) adatm_set_error("batch capture failed");
+    return rc;
+  }
+  hipError_t const ei = hipGraphInstantiate(&b->gexec, g, NULL, NULL, 0);
+  if (ei != hipSuccess) {
+    hipGraphDestroy(g);
+    return hip_fail(ei, "batch graph instantiate");
+  }
+  b->graph = g;
+  b->gvalid = 1;
+  return SPLATT_SUCCESS;
+}
+
+/* niters batched sweeps from every instance's current factors: one graph
+ * launch per sweep.  *ms_total = device time of the sweeps (events on the
+ * batch stream); fits (nullable, n x niters row-major) = every instance's
+ * fit after every sweep. */
+int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *fits)
+{
+  if (!b || niters == 0) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(b->device), "hipSetDevice");
+  for (int i = 0; i < b->n; ++i) {
+    if (niters > b->e[i]->fits_cap) return SPLATT_ERROR_BADINPUT;
+    int const rc = cpd_prepare(b->e[i]);
+    if (rc != SPLATT_SUCCESS) return rc;
+    HIPCHK(hipStreamSynchronize(b->e[i]->stream), "prepare sync");
+  }
+  int rc = batch_graph(b);
+  if (rc != SPLATT_SUCCESS) return rc;
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0), "event");
+  HIPCHK(hipEventCreate(&t1), "event");
+  hipEventRecord(t0, b->s);
+  for (uint32_t it = 0; it < niters; ++it)
+    if (hipGraphLaunch(b->gexec, b->s) != hipSuccess) {
+      rc = hip_fail(hipGetLastError(), "batch graph launch");
+      break;
+    }
+  hipEventRecord(t1, b->s);
+  hipError_t const es = hipEventSynchronize(t1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, t0, t1);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  if (rc != SPLATT_SUCCESS) return rc;
+  HIPCHK(es, "batch sync");
+  if (ms_total) *ms_total = ms;
+  if (fits)
+    for (int i = 0; i < b->n; ++i)
+      HIPCHK(hipMemcpy(fits + (size_t)i * niters, b->e[i]->fits, niters * sizeof(double),
+                       hipMemcpyDeviceToHost),
+             "batch fits");
+  return SPLATT_SUCCESS;
+}
+
+/* The unbatched comparison: each instance's niters sweeps (its own graph)
+ * one instance after another, synchronised between instances. */
+int adatm_batch_sequential(adatm_batch *b, uint32_t niters, double *ms_total)
+{
+  if (!b || niters == 0) return SPLATT_ERROR_BADINPUT;
+  double total = 0.0;
+  for (int i = 0; i < b->n; ++i) {
+    double ms = 0.0;
+    int const rc = adatm_engine_run_timed(b->e[i], 1, niters, 1, &ms);
+    if (rc != SPLATT_SUCCESS) return rc;
+    total += ms;
+  }
+  *ms_total = total;
   return SPLATT_SUCCESS;
 }

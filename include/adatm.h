@@ -282,6 +282,21 @@ int adatm_engine_stats(adatm_engine *e, uint64_t *nfibs, double *seq_bytes,
 int adatm_engine_get_csf(adatm_engine *e, uint32_t grp, uint32_t **fptr,
                          uint32_t **fids, double *vals);
 
+/* ----------------------------------------------------------------------- *
+ * Batched CP-ALS of many independent engines on one device (no counterpart
+ * in the reference): one hipGraph holds every instance's ALS sweep, each
+ * instance on its own forked stream, so one graph launch runs a batched sweep.
+ * The batch borrows the engines (destroy them after the batch).
+ * ----------------------------------------------------------------------- */
+typedef struct adatm_batch adatm_batch;
+int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out);
+void adatm_batch_destroy(adatm_batch *b);
+/* niters batched sweeps from the current factors; *ms_total = device time;
+ * fits (nullable, n x niters) = every instance's fit after every sweep */
+int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *fits);
+/* each instance's niters sweeps (its own graph), one instance after another */
+int adatm_batch_sequential(adatm_batch *b, uint32_t niters, double *ms_total);
+
 /* Last device error string (thread-local), "" when none. */
 char const *adatm_last_error(void);
 

@@ -187,6 +187,37 @@ def test_tiles_and_autotuner():
     assert rt in (64, 128, 256, 512) and st in (32, 64, 128, 256) and ms > 0
 
 
+def test_batched_cpd_matches_each_instance():
+    """Uneven instances (orders 4-6, ranks 8/16/32) swept together in one
+    hipGraph: every replay's fits equal each engine's own eager sweeps."""
+    print("batched cpd", flush=True)
+    specs = [([60, 50, 40, 30], 8000, A.DIST_UNIFORM, 8),
+             ([80, 70, 60, 50, 40], 12000, A.DIST_ZIPF, 16),
+             ([40, 35, 30, 25, 20, 15], 6000, A.DIST_UNIFORM, 32),
+             ([100, 90, 80, 70], 20000, A.DIST_ZIPF, 16)]
+    eng, solo = [], []
+    for k, (dims, nd, dist, R) in enumerate(specs):
+        t = _tensor(dims, nd, dist, seed=60 + k)
+        cands, ch = A.plan_analytic(t, R)
+        for lst in (eng, solo):
+            e = A.Engine(t, R, cands[ch], precision=A.FP32)
+            e.init_factors(7 + k)
+            lst.append(e)
+    b = A.Batch(eng)
+    ms, fits = b.cpd(5)
+    assert ms > 0
+    for k, e in enumerate(solo):
+        ref = np.asarray(e.cpd(5, tol=0.0, graph=False))
+        assert np.array_equal(fits[k], ref), (k, fits[k], ref)
+    # a second batched run from the updated factors replays the same graph
+    ms2, fits2 = b.cpd(3)
+    for k, e in enumerate(solo):
+        ref = np.asarray(e.cpd(3, tol=0.0, graph=False))
+        assert np.array_equal(fits2[k], ref), (k, fits2[k], ref)
+    assert b.sequential(2) > 0
+    b.close()
+
+
 def test_device_fiber_counts_match_host():
     """Radix sort + boundary scan on the device give the host's (and the
     oracle's) fiber counts, including keys that need three 8-bit digits."""
