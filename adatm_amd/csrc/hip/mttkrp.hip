@@ -104,25 +104,32 @@ __global__ __launch_bounds__(256) void k_reduce(
   for (uint32_t c0 = a; c0 < b; c0 += RB) {
     uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
     if (!PIPE) load_meta(c0, id, f, xv);
-    /* closes of the intermediate levels in this batch: the q-th close of
-       relative level j is node node[j] + q */
-    uint32_t ncl[K];
+    /* close flags: child i ends the node of relative level j when
+       f[i] <= lt + j (nested: a level closes only if every deeper one does).
+       The node of level j current at child i is node[j] plus the closes of
+       level j before i, so each closing child loads its own node's id and
+       factor row: all of a batch's loads are issued together, indexed by the
+       child, and the folds below are selects. */
+    uint32_t ci[K][RB];
 #pragma unroll
-    for (int j = 1; j < K; ++j) {
-      ncl[j] = 0;
+    for (int j = 0; j < K; ++j) {
+      uint32_t cnt = 0;
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
-        ncl[j] += ((uint32_t)i < nb && f[i] <= lt + j) ? 1u : 0u;
+      for (int i = 0; i < RB; ++i) {
+        ci[j][i] = node[j] + cnt;
+        cnt += ((uint32_t)i < nb && f[i] <= lt + j) ? 1u : 0u;
+      }
+      node[j] += cnt;
     }
-    /* issue order: closing-node ids, child rows, closing-node rows, next
-       batch's metadata -- so waiting for this batch never waits for the
-       prefetch */
+    /* loads are unconditional: ci[j][i] of an existing child is the node
+       holding it, a valid node; past the item's end the last child's is
+       reused */
     uint32_t cid[K][RB];
 #pragma unroll
     for (int j = 1; j < K; ++j)
 #pragma unroll
-      for (int q = 0; q < RB; ++q)
-        cid[j][q] = ((uint32_t)q < ncl[j]) ? ch.ids[j][node[j] + q] : 0u;
+      for (int i = 0; i < RB; ++i)
+        cid[j][i] = ch.ids[j][(uint32_t)i < nb ? ci[j][i] : ci[j][0]];
     T uv[RB];
 #pragma unroll
     for (int i = 0; i < RB; ++i) uv[i] = (T)ldu(Ub + (id[i] * R + col));
@@ -130,42 +137,37 @@ __global__ __launch_bounds__(256) void k_reduce(
 #pragma unroll
     for (int j = 1; j < K; ++j)
 #pragma unroll
-      for (int q = 0; q < RB; ++q)
-        uc[j][q] = ((uint32_t)q < ncl[j])
-                       ? (T)ldu((TU const *)ch.U[j] + (cid[j][q] * R + col))
-                       : (T)0;
+      for (int i = 0; i < RB; ++i)
+        uc[j][i] = (T)ldu((TU const *)ch.U[j] + (cid[j][i] * R + col));
     uint32_t nid[RB], nf[RB];
     T nxv[RB];
     uint32_t const c1 = c0 + RB;
     if (PIPE && c1 < b) load_meta(c1, nid, nf, nxv);
+    auto step = [&](int i) {
+      acc[K - 1] += xv[i] * uv[i];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      if ((uint32_t)i < nb) {
-        acc[K - 1] += xv[i] * uv[i];
-        bool go = true;
-#pragma unroll
-        for (int j = K - 1; j >= 1; --j) {
-          if (go && f[i] <= lt + j) {
-            /* node[j] complete (or the item ends inside it): fold it into
-               its parent with its own factor row */
-            acc[j - 1] += acc[j] * uc[j][0];
-            acc[j] = (T)0;
-#pragma unroll
-            for (int q = 0; q + 1 < RB; ++q) uc[j][q] = uc[j][q + 1];
-            ++node[j];
-          } else {
-            go = false;
-          }
-        }
-        if (go && f[i] <= lt) {
-          T *dst = (slot != NONE32)
-                       ? carry + (size_t)slot * R
-                       : out + (size_t)(ch.orow ? ch.orow[node[0]] : node[0]) * R;
-          dst[col] = acc[0];
-          acc[0] = (T)0;
-          ++node[0];
-        }
+      for (int j = K - 1; j >= 1; --j) {
+        /* fold node ci[j][i] into its parent with its own factor row */
+        bool const c = f[i] <= lt + j;
+        T const t = acc[j] * uc[j][i];
+        acc[j - 1] = c ? acc[j - 1] + t : acc[j - 1];
+        acc[j] = c ? (T)0 : acc[j];
       }
+      if (f[i] <= lt) {
+        T *dst = (slot != NONE32)
+                     ? carry + (size_t)slot * R
+                     : out + (size_t)(ch.orow ? ch.orow[ci[0][i]] : ci[0][i]) * R;
+        dst[col] = acc[0];
+        acc[0] = (T)0;
+      }
+    };
+    if (nb == (uint32_t)RB) {
+#pragma unroll
+      for (int i = 0; i < RB; ++i) step(i);
+    } else {
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+        if ((uint32_t)i < nb) step(i);
     }
     if (PIPE && c1 < b) {
 #pragma unroll

@@ -85,3 +85,26 @@ def test_features_host(lib, N):
         Fg = [A.count_fibers(t, plan.grp[g].o_opt[:N]) for g in range(plan.n_grp)]
         fc = A.plan_features_from_counts(N, t.nnz, t.dims, R, plan, Fg)
         assert np.array_equal(fc, f[c])
+
+
+def test_committed_corpus_is_consistent():
+    """The committed planner corpus: every tensor's candidates are complete
+    (Appendix A counts for its order), timings positive, the analytic choice
+    is a valid candidate, and the held-out split has 80 tensors."""
+    import glob
+    import os
+    from adatm_amd import corpus as K
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    paths = sorted(glob.glob(os.path.join(root, "adatm_amd", "planner", "corpus_c4_*.npz")))
+    if not paths:
+        pytest.skip("no committed corpus")
+    d = K.load(paths)
+    counts = {3: 1, 4: 3, 5: 4, 6: 7, 7: 8, 8: 11}
+    assert (d["ms"] > 0).all() and np.isfinite(d["X"]).all()
+    for k, N, nnz, R, nc, an in d["meta"]:
+        assert nc == counts[int(N)] == (d["tensor"] == k).sum()
+        assert 0 <= an < nc and R in (8, 16, 32, 64)
+        spec, Rs = K.corpus_spec(int(k))
+        assert spec.nmodes == N and Rs == R
+    assert len(np.unique(d["tensor"])) == K.NTENSORS
+    assert len(np.unique(K.split(d, False)["tensor"])) == K.NTENSORS - K.NTRAIN

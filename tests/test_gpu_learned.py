@@ -130,3 +130,32 @@ def test_plan_exhaustive_times_every_candidate():
     assert [c.groups(5) for c in cands] == [c.groups(5) for c in ac]
     assert np.isfinite(ms).all() and (ms > 0).all()
     assert ch == int(np.argmin(ms))
+
+
+PLANNER = os.path.join(ROOT, "adatm_amd", "planner")
+
+
+def test_trained_model_gain_on_heldout():
+    """The committed cost model, run through the HIP forward kernels, picks
+    plans for the 80 held-out corpus tensors; scored against the exhaustive
+    timings of the corpus, its mean regret is below the analytic model's and
+    within the 10% target."""
+    from adatm_amd import corpus as K
+    import glob
+    paths = sorted(glob.glob(os.path.join(PLANNER, "corpus_c4_*.npz")))
+    model = os.path.join(PLANNER, "costmodel.bin")
+    if not paths or not os.path.exists(model):
+        pytest.skip("no committed corpus / model")
+    print("trained model on held-out corpus", flush=True)
+    d = K.load(paths)
+    te = K.split(d, False)
+    assert len(np.unique(te["tensor"])) == K.NTENSORS - K.NTRAIN
+    m = A.Mlp(device=0, path=model)
+    pred = m.predict(te["X"])
+    learned = K.regrets(te, K.model_choice(te, pred))
+    analytic = K.regrets(te, K.analytic_choice(te))
+    print(f"held-out regret: learned mean {learned.mean():.4f} "
+          f"(median {np.median(learned):.4f}, max {learned.max():.4f}); "
+          f"analytic mean {analytic.mean():.4f}", flush=True)
+    assert learned.mean() <= 0.10
+    assert learned.mean() < analytic.mean()
