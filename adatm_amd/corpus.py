@@ -129,7 +129,7 @@ def split(d: dict, train: bool) -> dict:
     return out
 
 
-def train(d: dict, epochs: int = 400, batch: int = 64, lr: float = 2e-3,
+def train(d: dict, epochs: int = 400, batch: int = 64, lr: float = 3e-3,
           seed: int = 1, device: int = 0) -> "A.Mlp":
     tr = split(d, True)
     m = A.Mlp(device=device, seed=seed)
