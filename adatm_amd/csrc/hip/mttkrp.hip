@@ -180,18 +180,11 @@ __global__ __launch_bounds__(256) void k_reduce(
   }
 }
 
-template <typename T, typename TU, int D, bool ZS>
-__device__ __forceinline__ T scatter_term(uint32_t q, adatm_kscat const &sc,
-                                          T const *__restrict__ Z, uint32_t R,
-                                          uint32_t col)
-{
-  T y = (T)ldu((TU const *)sc.U[0] + (size_t)sc.rid[0][q] * R + col);
-#pragma unroll
-  for (int l = 1; l < D; ++l)
-    y = y * (T)ldu((TU const *)sc.U[l] + (size_t)sc.rid[l][q] * R + col);
-  uint32_t const n = sc.node[q];
-  return (ZS ? Z[n] : Z[(size_t)n * R + col]) * y;
-}
+/* Reuse MTTKRP of level D: nodes pre-sorted by output row, taken in batches
+ * of SB whose node ids and ancestor row ids are loaded one batch ahead; the
+ * batch's Z rows and ancestor factor rows are then gathered together.  A
+ * batch in which no output row ends is summed without per-node checks. */
+#define SB 8
 
 template <typename T, typename TU, int D, bool ZS>
 __global__ __launch_bounds__(256) void k_scatter(
@@ -210,55 +203,79 @@ __global__ __launch_bounds__(256) void k_scatter(
   if (k >= nitems) return;
   uint32_t const a = istart[k], b = istart[k + 1];
   uint32_t const slot = islot[k];
-  if (slot != NONE32) {
-    T acc = (T)0;
-    uint32_t q = a;
-    for (; q + PF <= b; q += PF) {
-      T v[PF];
-#pragma unroll
-      for (int i = 0; i < PF; ++i) v[i] = scatter_term<T, TU, D, ZS>(q + i, sc, Z, R, col);
-#pragma unroll
-      for (int i = 0; i < PF; ++i) acc += v[i];
-    }
-    for (; q < b; ++q) acc += scatter_term<T, TU, D, ZS>(q, sc, Z, R, col);
-    carry[(size_t)slot * R + col] = acc;
-    return;
-  }
+  bool const heavy = slot != NONE32;
   /* light item: whole rows iseg[k] .. iseg[k+1]-1, positions [a, b) */
   uint32_t p = iseg[k];
-  uint32_t const pe = iseg[k + 1];
-  uint32_t e = rowptr[p + 1];
-  while (e == a) { /* leading empty rows */
-    out[(size_t)p * R + col] = (T)0;
-    if (++p == pe) return;
+  uint32_t const pe = heavy ? p : iseg[k + 1];
+  uint32_t e = b; /* end of the current output row */
+  if (!heavy) {
     e = rowptr[p + 1];
-  }
-  T acc = (T)0;
-  for (uint32_t c0 = a; c0 < b; c0 += PF) {
-    T v[PF];
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      uint32_t const c = (c0 + i < b) ? c0 + i : b - 1;
-      v[i] = scatter_term<T, TU, D, ZS>(c, sc, Z, R, col);
+    while (e == a) { /* leading empty rows */
+      out[(size_t)p * R + col] = (T)0;
+      if (++p == pe) return;
+      e = rowptr[p + 1];
     }
+  }
+  uint32_t nd[SB], rr[D][SB];
+  auto load_idx = [&](uint32_t c0, uint32_t *mn, uint32_t (*mr)[SB]) {
+    uint32_t const nb = (b - c0 < (uint32_t)SB) ? b - c0 : (uint32_t)SB;
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      uint32_t const c = c0 + i;
-      if (c < b) {
-        acc += v[i];
-        if (c + 1 == e) {
-          out[(size_t)p * R + col] = acc;
-          acc = (T)0;
-          /* next non-empty row; empty rows in between get zeros */
-          while (++p < pe) {
-            e = rowptr[p + 1];
-            if (e != c + 1) break;
-            out[(size_t)p * R + col] = (T)0;
+    for (int i = 0; i < SB; ++i) {
+      uint32_t const q = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
+      mn[i] = sc.node[q];
+#pragma unroll
+      for (int l = 0; l < D; ++l) mr[l][i] = sc.rid[l][q];
+    }
+  };
+  load_idx(a, nd, rr);
+  T acc = (T)0;
+  for (uint32_t c0 = a; c0 < b; c0 += SB) {
+    uint32_t const nb = (b - c0 < (uint32_t)SB) ? b - c0 : (uint32_t)SB;
+    T v[SB];
+#pragma unroll
+    for (int i = 0; i < SB; ++i) {
+      T y = (T)ldu((TU const *)sc.U[0] + (rr[0][i] * R + col));
+#pragma unroll
+      for (int l = 1; l < D; ++l)
+        y = y * (T)ldu((TU const *)sc.U[l] + (rr[l][i] * R + col));
+      v[i] = (ZS ? Z[nd[i]] : Z[(size_t)nd[i] * R + col]) * y;
+    }
+    uint32_t const c1 = c0 + SB;
+    if (c1 < b) load_idx(c1, nd, rr);
+    if (nb == (uint32_t)SB && (heavy || e >= c0 + SB)) {
+      /* no row ends before the batch's last node (a piece has no row end) */
+      T s01 = v[0] + v[1], s23 = v[2] + v[3], s45 = v[4] + v[5], s67 = v[6] + v[7];
+      acc += (s01 + s23) + (s45 + s67);
+      if (!heavy && e == c0 + SB) { /* the row ends with the batch */
+        out[(size_t)p * R + col] = acc;
+        acc = (T)0;
+        while (++p < pe) {
+          e = rowptr[p + 1];
+          if (e != c0 + SB) break;
+          out[(size_t)p * R + col] = (T)0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SB; ++i) {
+        uint32_t const c = c0 + i;
+        if ((uint32_t)i < nb) {
+          acc += v[i];
+          if (!heavy && c + 1 == e) {
+            out[(size_t)p * R + col] = acc;
+            acc = (T)0;
+            /* next non-empty row; empty rows in between get zeros */
+            while (++p < pe) {
+              e = rowptr[p + 1];
+              if (e != c + 1) break;
+              out[(size_t)p * R + col] = (T)0;
+            }
           }
         }
       }
     }
   }
+  if (heavy) carry[(size_t)slot * R + col] = acc;
 }
 
 /* Chunk c of split parent h = cseg[c]: the sum of its <= ADATM_FIX_CHUNK

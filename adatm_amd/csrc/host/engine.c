@@ -1850,18 +1850,19 @@ uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
 }
 
 /* Work-item tiles of one plan chosen by timing: the reduction tile over
- * {64, 128, 256, 512} with the scatter tile at its default, then the scatter
- * tile over {32, 64, 128, 256} with the best reduction tile; each candidate
- * is a fresh engine timed on nrep CP-ALS sweeps (hipGraph, median). */
+ * {32, 64, 128, 256, 512} with the scatter tile at its default, then the
+ * scatter tile over {16, 32, 64, 128, 256} with the best reduction tile; each
+ * candidate is a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
+ * median). */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, double *best_ms)
 {
-  static uint32_t const RT[4] = {64, 128, 256, 512}, ST[4] = {32, 64, 128, 256};
+  static uint32_t const RT[5] = {32, 64, 128, 256, 512}, ST[5] = {16, 32, 64, 128, 256};
   uint32_t br = RED_TILE, bs = SCAT_TILE;
   double bt = INFINITY;
   for (int pass = 0; pass < 2; ++pass)
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
       uint32_t const r = pass == 0 ? RT[i] : br, sc = pass == 0 ? SCAT_TILE : ST[i];
       if (pass == 1 && sc == SCAT_TILE) continue; /* timed in the first pass */
       adatm_engine *e = NULL;

@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default work-item tiles instead of the autotuner's")
+    ap.add_argument("--tiles", default="", metavar="RED,SCAT",
+                    help="use these work-item tiles for every engine (no autotuning)")
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
     ap.add_argument("--bf16", action="store_true",
@@ -144,7 +148,13 @@ def main() -> int:
     Mref = {}
     for name, pl, prec in runs:
         t2 = time.time()
-        e = A.Engine(t, R, pl, precision=prec, device=0)
+        tiles = (0, 0)
+        if args.tiles:
+            tiles = tuple(int(x) for x in args.tiles.split(","))
+        elif not args.no_autotune:
+            tiles, tms = A.autotune_tiles(t, R, pl, prec, 0, 3)
+            log(f"{name}: autotuned tiles {tiles} ({tms:.3f} ms/sweep, {time.time() - t2:.1f}s)")
+        e = A.Engine(t, R, pl, precision=prec, device=0, tiles=tiles)
         e.init_factors(spec.seed + 1)
         log(f"{name} engine built ({time.time() - t2:.1f}s)")
         if args.warmup > 0:
@@ -155,7 +165,7 @@ def main() -> int:
         _, pk = e.time(1, 0, 1, graph=False)
         launch_log = e.launches()
         F, seq_bytes, seq_flops, dev_bytes = e.stats()
-        res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=float(fits[-1]),
+        res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=float(fits[-1]), tiles=list(tiles),
                          seq_ms=seq_ms, pk=pk, seq_bytes=seq_bytes,
                          seq_flops=seq_flops, dev_bytes=dev_bytes,
                          nfibs=F.tolist(), launches=launch_log)
@@ -212,6 +222,7 @@ def main() -> int:
             "planner": "learned (MLP cost model)" if learned else "analytic (no trained model)",
             "learned_plan": lplan.groups(t.nmodes), "learned_plan_index": lplan_idx,
             "analytic_plan": plan.groups(t.nmodes), "analytic_plan_index": ch,
+            "tiles": {k: v["tiles"] for k, v in res.items()},
         },
         "roofline": {
             "bound": "hbm", "kernel": launch_name(dcls, dinfo),
@@ -285,7 +296,7 @@ def batched_c5(args) -> int:
         t = A.Tensor.synth(spec)
         t.canonicalize()
         cands, ch = A.plan_analytic(t, R, device=0)
-        tl, _ = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
+        tl = (0, 0) if args.no_autotune else A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)[0]
         e = A.Engine(t, R, cands[ch], precision=A.FP32, tiles=tl)
         e.init_factors(spec.seed + 1)
         f = A.Engine(t, R, cands[ch], precision=A.FP64)
