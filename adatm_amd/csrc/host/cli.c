@@ -1,7 +1,7 @@
 /*
  * `splatt` command-line tool: the AdaTM subcommands on one MI355X.
  *
- *   splatt bench TENSOR -a adatm [-i NITERS] [-r RANK] [-t NTHREADS] [--precision P]
+ *   splatt bench TENSOR -a adatm [-i NITERS] [-r RANK] [-t NTHREADS] [--precision P] [--planner PL]
  *   splatt cpd   TENSOR [-i NITERS] [--tol TOL] [-r RANK] [-t NTHREADS]
  *                [--nowrite] [-v] [--precision P] [--seed S]
  *
@@ -108,13 +108,30 @@ static void print_header(adatm_coo const *t, char const *name)
   printf(" NNZ=%llu\n\n", (unsigned long long)t->nnz);
 }
 
-static int plan_for(adatm_coo const *t, uint64_t R, adatm_plan *plan)
+/* planner: 0 analytic (the reference's), 1 learned (cost model beside the
+ * library or $ADATM_COSTMODEL; analytic without one), 2 exhaustive timing */
+static int plan_for(adatm_coo const *t, uint64_t R, int planner, int prec,
+                    adatm_plan *plan)
 {
   adatm_plan cands[256];
-  int nc = 0, ch = 0;
-  if (adatm_plan_analytic(t, R, 1, (double)SPLATT_ADATM_SLIMIT, 0, cands, 256,
-                          &nc, &ch) != SPLATT_SUCCESS)
-    return -1;
+  int nc = 0, ch = 0, rc;
+  if (planner == SPLATT_ADATM_PLAN_LEARNED) {
+    adatm_mlp *model = NULL;
+    char path[4096];
+    if (adatm_costmodel_path(path, sizeof(path)) &&
+        adatm_mlp_load(0, path, &model) != SPLATT_SUCCESS)
+      model = NULL;
+    printf("Planner: %s\n", model ? "learned cost model" : "analytic (no cost model)");
+    rc = adatm_plan_learned(t, R, model, 0, cands, 256, &nc, &ch, NULL);
+    adatm_mlp_destroy(model);
+  } else if (planner == SPLATT_ADATM_PLAN_EXHAUSTIVE) {
+    printf("Planner: exhaustive timing\n");
+    rc = adatm_plan_exhaustive(t, R, prec, 0, 1, 3, cands, 256, &nc, &ch, NULL);
+  } else {
+    rc = adatm_plan_analytic(t, R, 1, (double)SPLATT_ADATM_SLIMIT, 0, cands, 256, &nc,
+                             &ch);
+  }
+  if (rc != SPLATT_SUCCESS) return -1;
   printf("Number of configurations: %d\n", nc);
   char b[32];
   printf("Storage limitation: %s\n\n", bytes_str((double)SPLATT_ADATM_SLIMIT, b));
@@ -146,7 +163,7 @@ struct bench_args {
   int adatm;
   char *alg;
   uint32_t niters, nthreads, rank;
-  int prec;
+  int prec, planner;
 };
 
 static struct argp_option bench_opts[] = {
@@ -156,6 +173,13 @@ static struct argp_option bench_opts[] = {
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 16)", 0},
     {"precision", 'p', "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
     {0}};
+
+static int parse_planner(char const *s)
+{
+  if (!strcmp(s, "learned")) return SPLATT_ADATM_PLAN_LEARNED;
+  if (!strcmp(s, "exhaustive")) return SPLATT_ADATM_PLAN_EXHAUSTIVE;
+  return SPLATT_ADATM_PLAN_ANALYTIC;
+}
 
 static int parse_prec(char const *s)
 {
@@ -176,6 +200,7 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
   case 't': a->nthreads = (uint32_t)atoi(arg); break;
   case 'r': a->rank = (uint32_t)atoi(arg); break;
   case 'p': a->prec = parse_prec(arg); break;
+  case 'P': a->planner = parse_planner(arg); break;
   case ARGP_KEY_ARG:
     if (a->tensor) argp_usage(st);
     a->tensor = arg;
@@ -190,7 +215,8 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
 
 static int cmd_bench(int argc, char **argv)
 {
-  struct bench_args a = {NULL, 0, NULL, 5, 1, 16, SPLATT_ADATM_FP64};
+  struct bench_args a = {NULL, 0, NULL, 5, 1, 16, SPLATT_ADATM_FP64,
+                         SPLATT_ADATM_PLAN_ANALYTIC};
   struct argp ap = {bench_opts, bench_parse, "TENSOR -a adatm",
                     "splatt-bench -- benchmark the memoized MTTKRP sequence.", 0, 0, 0};
   argp_parse(&ap, argc, argv, ARGP_IN_ORDER, 0, &a);
@@ -208,7 +234,7 @@ static int cmd_bench(int argc, char **argv)
   printf("Benchmarking ---------------------------------------------------\n");
   printf("RANK=%u ITS=%u\n", a.rank, a.niters);
   adatm_plan plan;
-  if (plan_for(&t, a.rank, &plan) != 0) return EXIT_FAILURE;
+  if (plan_for(&t, a.rank, a.planner, a.prec, &plan) != 0) return EXIT_FAILURE;
   adatm_engine *e = NULL;
   if (adatm_engine_create(&t, a.rank, &plan, a.prec, 0, &e) != SPLATT_SUCCESS) {
     fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
@@ -240,6 +266,7 @@ struct cpd_args {
   double tol;
   int write, verbose, prec;
   uint64_t seed;
+  int planner;
 };
 
 #define OPT_TOL 254
@@ -254,6 +281,7 @@ static struct argp_option cpd_opts[] = {
     {"verbose", 'v', 0, 0, "turn on verbose output (default: no)", 0},
     {"precision", 'p', "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
     {"seed", OPT_SEED, "SEED", 0, "factor initialisation seed (default: time)", 0},
+    {"planner", 'P', "PLANNER", 0, "analytic | learned | exhaustive (default: analytic)", 0},
     {0}};
 
 static error_t cpd_parse(int key, char *arg, struct argp_state *st)
@@ -269,6 +297,7 @@ static error_t cpd_parse(int key, char *arg, struct argp_state *st)
   case 'v': a->verbose += 1; break;
   case 'p': a->prec = parse_prec(arg); break;
   case OPT_SEED: a->seed = strtoull(arg, NULL, 10); break;
+  case 'P': a->planner = parse_planner(arg); break;
   case ARGP_KEY_ARG:
     if (a->tensor) argp_usage(st);
     a->tensor = arg;
@@ -296,7 +325,7 @@ static void write_matrix(char const *fname, double const *A, uint64_t I,
 static int cmd_cpd(int argc, char **argv)
 {
   struct cpd_args a = {NULL, 50, 10, 0, 1e-5, 1, 0, SPLATT_ADATM_FP64,
-                       (uint64_t)time(NULL)};
+                       (uint64_t)time(NULL), SPLATT_ADATM_PLAN_ANALYTIC};
   struct argp ap = {cpd_opts, cpd_parse, "TENSOR",
                     "splatt-cpd -- Compute the CPD of a sparse tensor.", 0, 0, 0};
   argp_parse(&ap, argc, argv, ARGP_IN_ORDER, 0, &a);
@@ -308,7 +337,7 @@ static int cmd_cpd(int argc, char **argv)
   }
   print_header(&t, a.tensor);
   adatm_plan plan;
-  if (plan_for(&t, a.rank, &plan) != 0) return EXIT_FAILURE;
+  if (plan_for(&t, a.rank, a.planner, a.prec, &plan) != 0) return EXIT_FAILURE;
   adatm_engine *e = NULL;
   if (adatm_engine_create(&t, a.rank, &plan, a.prec, 0, &e) != SPLATT_SUCCESS) {
     fprintf(stderr, "SPLATT: %s\n", adatm_last_error());

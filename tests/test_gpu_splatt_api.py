@@ -185,3 +185,9 @@ def test_cli_bench_and_cpd(tmp_path):
     M = np.loadtxt(tmp_path / "mode1.mat")
     assert M.shape[1] == 8
     np.testing.assert_allclose(np.linalg.norm(M, axis=0), 1.0, rtol=1e-6)
+    for planner, label in (("learned", "Planner: "), ("exhaustive", "Planner: exhaustive")):
+        r = subprocess.run([exe, "bench", "synth:1:100000", "-a", "adatm", "-i", "1", "-r", "8",
+                            "--precision", "fp32", "--planner", planner],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert label in r.stdout and "its =   1" in r.stdout
