@@ -3,7 +3,7 @@
  *
  *   splatt bench TENSOR -a adatm [-i NITERS] [-r RANK] [-t NTHREADS] [--precision P] [--planner PL]
  *   splatt cpd   TENSOR [-i NITERS] [--tol TOL] [-r RANK] [-t NTHREADS]
- *                [--nowrite] [-v] [--precision P] [--seed S]
+ *                [--nowrite] [-v] [--precision P] [--seed S] [--planner PL]
  *
  * Flags, defaults and log lines follow the reference's drivers:
  * src/cmds/cmd_bench.c:75-87, 203-287 with src/adatm_bench.c:516-655
@@ -172,6 +172,7 @@ static struct argp_option bench_opts[] = {
     {"threads", 't', "NTHREADS", 0, "host threads (default: 1)", 0},
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 16)", 0},
     {"precision", 'p', "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
+    {"planner", 'P', "PLANNER", 0, "analytic | learned | exhaustive (default: analytic)", 0},
     {0}};
 
 static int parse_planner(char const *s)
