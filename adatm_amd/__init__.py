@@ -126,7 +126,8 @@ def lib() -> C.CDLL:
                                                 C.c_uint32, C.c_uint32, P(C.c_void_p)]),
         "adatm_autotune_tiles": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
                                            C.c_int, P(C.c_uint32), P(C.c_uint32),
-                                           P(C.c_double)]),
+                                           P(C.c_uint32), P(C.c_double)]),
+        "adatm_engine_set_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
         "adatm_engine_set_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_get_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_get_lambda": (C.c_int, [C.c_void_p, P(C.c_double)]),
@@ -505,14 +506,16 @@ def plan_exhaustive(t: Tensor, rank: int, precision: int = FP32, device: int = 0
 
 def autotune_tiles(t: Tensor, rank: int, plan: Plan, precision: int = FP32,
                    device: int = 0, nrep: int = 3):
-    """((reduction tile, scatter tile), best ms per sweep) by timing."""
+    """((reduction tile, scatter tile, children per batch), best ms per
+    sweep) by timing; pass the triple as Engine(..., tiles=...)."""
     r = C.c_uint32()
     s = C.c_uint32()
+    b = C.c_uint32()
     ms = C.c_double()
     check(lib().adatm_autotune_tiles(C.byref(t.c), rank, C.byref(plan), precision, device,
-                                     nrep, C.byref(r), C.byref(s), C.byref(ms)),
+                                     nrep, C.byref(r), C.byref(s), C.byref(b), C.byref(ms)),
           "autotune_tiles")
-    return (r.value, s.value), ms.value
+    return (r.value, s.value, b.value), ms.value
 
 
 def costmodel_path():
@@ -535,6 +538,11 @@ class Engine:
         check(lib().adatm_engine_create_tiled(C.byref(t.c), rank, C.byref(plan), precision,
                                               device, int(tiles[0]), int(tiles[1]),
                                               C.byref(self.h)), "engine_create")
+        if len(tiles) > 2 and tiles[2]:
+            self.set_batch(int(tiles[2]))
+
+    def set_batch(self, batch: int) -> None:
+        check(lib().adatm_engine_set_batch(self.h, batch), "engine_set_batch")
 
     def close(self) -> None:
         if self.h:

@@ -39,12 +39,11 @@
 
 namespace {
 
-/* Children per batch: all of a batch's loads -- child rows and the rows of
- * the intermediate nodes that close inside it -- are issued together. */
-template <int K>
-struct rb_of {
-  static constexpr int v = (K <= 3) ? 8 : 4;
-};
+/* Children per batch (RB): all of a batch's loads -- child rows and the rows
+ * of the intermediate nodes that close inside it -- are issued together.
+ * Larger batches keep more loads in flight, smaller ones need fewer
+ * registers; the launch picks RB (adatm_kchain.rb, tuned per plan) from
+ * {4, 8, 16} for K <= 2, {4, 8} for K = 3 and 4 for deeper passes. */
 /* single-level passes load the next batch's metadata while the current one
  * computes; multi-level passes keep the registers for occupancy instead */
 template <int K>
@@ -52,7 +51,7 @@ struct pipe_of {
   static constexpr bool v = (K == 1);
 };
 
-template <typename T, typename TU, bool LEAF, int K>
+template <typename T, typename TU, bool LEAF, int K, int RB>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -60,7 +59,6 @@ __global__ __launch_bounds__(256) void k_reduce(
     T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
-  constexpr int RB = rb_of<K>::v;
   uint32_t const lane = threadIdx.x & 63u;
   uint32_t const grp = lane / R;
   if (grp >= gpw) return;
@@ -369,16 +367,37 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
   return hipGetLastError();
 }
 
-template <typename T, typename TU, bool LEAF, int K>
-void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
-               void const *xin, void *out, void *carry, uint32_t R)
+template <typename T, typename TU, bool LEAF, int K, int RB>
+void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
+                  void const *xin, void *out, void *carry, uint32_t R)
 {
   uint32_t const gpw = 64u / R;
   uint32_t const per_block = 4u * gpw;
   dim3 const grid((w->nitems + per_block - 1) / per_block);
-  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K>), grid, dim3(256), 0, s,
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB>), grid, dim3(256), 0, s,
                      w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
                      (T const *)xin, (T *)out, (T *)carry, R, gpw);
+}
+
+template <typename T, typename TU, bool LEAF, int K>
+void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
+               void const *xin, void *out, void *carry, uint32_t R)
+{
+  if constexpr (K <= 2) {
+    if (ch->rb == 16)
+      go_reduce_rb<T, TU, LEAF, K, 16>(s, w, ch, xin, out, carry, R);
+    else if (ch->rb == 4)
+      go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+    else
+      go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
+  } else if constexpr (K == 3) {
+    if (ch->rb == 4)
+      go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+    else
+      go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
+  } else {
+    go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+  }
 }
 
 template <typename T, typename TU, bool LEAF>

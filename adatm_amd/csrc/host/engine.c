@@ -88,6 +88,7 @@ struct adatm_engine {
   uint64_t dims[MAXN];
   uint64_t maxdim;
   uint32_t red_tile, scat_tile; /* children / nodes per work item */
+  uint32_t batch;               /* children per batch of the reductions (0: default) */
   adatm_plan plan;
   uint32_t use_grp[MAXN], use_tag[MAXN], level_of[MAXN][MAXN];
   dgroup g[MAXN];
@@ -294,6 +295,7 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
   memset(&ch, 0, sizeof(ch));
   ch.K = lb - lt;
   ch.lt = lt;
+  ch.rb = e->batch;
   ch.orow = (lt == 0) ? g->orow : NULL;
   ch.cfd = g->cfd[lb];
   for (uint32_t j = 1; j <= ch.K; ++j) {
@@ -1849,14 +1851,27 @@ uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
   return e->g[grp].memo[level] ? e->g[grp].F[level] * e->R * e->vsz : 0;
 }
 
-/* Work-item tiles of one plan chosen by timing: the reduction tile over
- * {32, 64, 128, 256, 512} with the scatter tile at its default, then the
- * scatter tile over {16, 32, 64, 128, 256} with the best reduction tile; each
- * candidate is a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
- * median). */
+int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
+{
+  if (!e || (batch != 0 && batch != 4 && batch != 8 && batch != 16))
+    return SPLATT_ERROR_BADINPUT;
+  if (batch != e->batch && e->gvalid) { /* the captured sweep holds the old one */
+    hipGraphExecDestroy(e->gexec);
+    hipGraphDestroy(e->graph);
+    e->gvalid = 0;
+  }
+  e->batch = batch;
+  return SPLATT_SUCCESS;
+}
+
+/* Work-item tiles and reduction batch of one plan chosen by timing: the
+ * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
+ * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
+ * reduction tile, each a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
+ * median); then, on the best engine, the children per batch over {4, 8, 16}. */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
-                         uint32_t *scat_tile, double *best_ms)
+                         uint32_t *scat_tile, uint32_t *batch, double *best_ms)
 {
   static uint32_t const RT[5] = {32, 64, 128, 256, 512}, ST[5] = {16, 32, 64, 128, 256};
   uint32_t br = RED_TILE, bs = SCAT_TILE;
@@ -1879,8 +1894,28 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
         bs = sc;
       }
     }
+  uint32_t bb = 0;
+  {
+    adatm_engine *e = NULL;
+    int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, br, bs, &e);
+    if (rc != SPLATT_SUCCESS) return rc;
+    rc = adatm_engine_init_factors(e, 1);
+    static uint32_t const BT[2] = {4, 16};
+    for (int i = 0; rc == SPLATT_SUCCESS && i < 2; ++i) {
+      double ms = INFINITY;
+      rc = adatm_engine_set_batch(e, BT[i]);
+      if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
+      if (rc == SPLATT_SUCCESS && ms < bt) {
+        bt = ms;
+        bb = BT[i];
+      }
+    }
+    adatm_engine_destroy(e);
+    if (rc != SPLATT_SUCCESS) return rc;
+  }
   *red_tile = br;
   *scat_tile = bs;
+  if (batch) *batch = bb;
   if (best_ms) *best_ms = bt;
   return SPLATT_SUCCESS;
 }

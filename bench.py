@@ -86,8 +86,9 @@ def main() -> int:
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default work-item tiles instead of the autotuner's")
-    ap.add_argument("--tiles", default="", metavar="RED,SCAT",
-                    help="use these work-item tiles for every engine (no autotuning)")
+    ap.add_argument("--tiles", default="", metavar="RED,SCAT[,BATCH]",
+                    help="use these work-item tiles (and reduction batch) for every engine "
+                         "(no autotuning)")
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
     ap.add_argument("--bf16", action="store_true",
@@ -148,7 +149,7 @@ def main() -> int:
     Mref = {}
     for name, pl, prec in runs:
         t2 = time.time()
-        tiles = (0, 0)
+        tiles = (0, 0, 0)
         if args.tiles:
             tiles = tuple(int(x) for x in args.tiles.split(","))
         elif not args.no_autotune:
@@ -296,15 +297,16 @@ def batched_c5(args) -> int:
         t = A.Tensor.synth(spec)
         t.canonicalize()
         cands, ch = A.plan_analytic(t, R, device=0)
-        tl = (0, 0) if args.no_autotune else A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)[0]
+        tl = (0, 0, 0) if args.no_autotune else A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)[0]
         e = A.Engine(t, R, cands[ch], precision=A.FP32, tiles=tl)
         e.init_factors(spec.seed + 1)
-        f = A.Engine(t, R, cands[ch], precision=A.FP64)
-        f.init_factors(spec.seed + 1)
+        if not args.no_fp64:
+            f = A.Engine(t, R, cands[ch], precision=A.FP64)
+            f.init_factors(spec.seed + 1)
+            e64.append(f)
         seq_bytes += e.stats()[1]
         nnz_total += t.nnz
         eng.append(e)
-        e64.append(f)
         tiles.append(tl)
     log(f"C5: {n} instances, {nnz_total} nnz in total, built and tuned in "
         f"{time.time() - t0:.1f}s")
@@ -314,8 +316,10 @@ def batched_c5(args) -> int:
         e.init_factors(c5_spec(k)[0].seed + 1)
     ms, fits = b.cpd(iters)
     seq_ms = b.sequential(2) / 2.0
-    fits64 = np.array([f.cpd(iters, tol=0.0, graph=True)[-1] for f in e64])
-    dfit = float(np.max(np.abs(fits[:, -1] - fits64)))
+    dfit = None
+    if e64:
+        fits64 = np.array([f.cpd(iters, tol=0.0, graph=True)[-1] for f in e64])
+        dfit = float(np.max(np.abs(fits[:, -1] - fits64)))
     per = ms / iters
     out = {
         "metric": f"ms per batched CP-ALS sweep ({n} uneven instances, one hipGraph "

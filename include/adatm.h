@@ -205,11 +205,16 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
                               adatm_plan const *plan, int precision, int device,
                               uint32_t red_tile, uint32_t scat_tile,
                               adatm_engine **out);
-/* Autotuner: the tiles of one plan chosen by timing CP-ALS sweeps (nrep,
- * median) of engines built with each candidate. */
+/* Children per batch of the fused reductions (4, 8, 16; 0 = default 8, or 4
+ * for passes fusing more than 3 levels; 16 applies to passes of <= 2 levels,
+ * 8 at most to 3).  A launch-time choice: no rebuild. */
+int adatm_engine_set_batch(adatm_engine *e, uint32_t batch);
+/* Autotuner: the tiles and batch of one plan chosen by timing CP-ALS sweeps
+ * (nrep, median) of engines built with each candidate; batch (nullable) gets
+ * the value for adatm_engine_set_batch (0 = default). */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
-                         uint32_t *scat_tile, double *best_ms);
+                         uint32_t *scat_tile, uint32_t *batch, double *best_ms);
 void adatm_engine_destroy(adatm_engine *e);
 
 /* Factor matrices are exchanged in FP64 row-major (dims[m] x rank). */

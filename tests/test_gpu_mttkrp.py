@@ -176,15 +176,16 @@ def test_tiles_and_autotuner():
     U = _factors(t, R, seed=5)
     cands, ch = A.plan_analytic(t, R)
     ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(4)]
-    for tiles in ((8, 8), (64, 32), (512, 256), (4096, 4096)):
+    for tiles in ((8, 8, 4), (64, 32, 16), (512, 256, 8), (4096, 4096, 0)):
         e = A.Engine(t, R, cands[ch], precision=A.FP64, tiles=tiles)
         for m in range(4):
             e.set_factor(m, U[m])
         for m in range(4):
             assert O.rel_fro(e.mttkrp(m), ref[m]) <= 1e-12, (tiles, m)
         e.close()
-    (rt, st), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
-    assert rt in (32, 64, 128, 256, 512) and st in (16, 32, 64, 128, 256) and ms > 0
+    (rt, st, bt), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
+    assert rt in (32, 64, 128, 256, 512) and st in (16, 32, 64, 128, 256)
+    assert bt in (0, 4, 16) and ms > 0
 
 
 def test_batched_cpd_matches_each_instance():
