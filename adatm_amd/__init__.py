@@ -119,6 +119,8 @@ def lib() -> C.CDLL:
         "adatm_plan_analytic": (C.c_int, [P(Coo), C.c_uint64, C.c_int, C.c_double, C.c_int,
                                           P(Plan), C.c_int, P(C.c_int), P(C.c_int)]),
         "adatm_plan_use": (None, [P(Plan), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
+        "adatm_plan_space_efficient": (C.c_int, [P(Plan), C.c_uint32, P(Plan),
+                                                 P(C.c_uint32), P(C.c_uint32)]),
         "adatm_engine_create": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
                                           P(C.c_void_p)]),
         "adatm_engine_destroy": (None, [C.c_void_p]),
@@ -355,6 +357,15 @@ def plan_analytic(t: Tensor, rank: int, ref_quirks: bool = True,
                                     device, arr, 256, C.byref(nc), C.byref(ch)),
           "plan_analytic")
     return [arr[i] for i in range(nc.value)], ch.value
+
+
+def plan_space_efficient(plan: Plan, nmodes: int):
+    """(kept plan, use groups, use tags) of SPACE_EFFICIENT allocation."""
+    out = Plan()
+    g = np.zeros(MAX_NMODES, dtype=np.uint32)
+    t = np.zeros(MAX_NMODES, dtype=np.uint32)
+    lib().adatm_plan_space_efficient(C.byref(plan), nmodes, C.byref(out), _uptr(g), _uptr(t))
+    return out, [int(x) for x in g[:nmodes]], [int(x) for x in t[:nmodes]]
 
 
 def plan_use(plan: Plan, nmodes: int):

@@ -15,7 +15,8 @@
  *   ONEMODE / TWOMODE / ALLMODE  SPLATT's plain CSF sets: the root of a CSF
  *        by a root pass, any other level from the leaves (tag 2), as
  *        mttkrp_csf does (src/mttkrp.c:1885-1936);
- *   ADAPTIVE_SPACE_EFFICIENT is rejected (SPLATT_ERROR_BADINPUT).
+ *   ADAPTIVE_SPACE_EFFICIENT keeps the CSFs of the memoising groups only;
+ *   the other groups' modes are computed from their leaves (tag 2).
  *
  * Differences from the reference, on purpose:
  *   - splatt_mttkrp serves adaptive CSFs (the reference's mttkrp_csf has no
@@ -46,7 +47,8 @@ typedef struct reg_ent {
   uint32_t N;
   uint64_t nnz;
   uint64_t dims[MAXN];
-  adatm_plan plan;
+  adatm_plan plan;  /* the groups that have a CSF */
+  adatm_plan full;  /* the plan as chosen / described by the caller */
   uint32_t use_grp[MAXN], use_tag[MAXN];
   double ttnormsq;
   int device;
@@ -290,9 +292,9 @@ static void free_csf_array(splatt_csf *t, uint32_t n)
 
 /* ---- construction ---------------------------------------------------------- */
 static int build_from_coo(adatm_coo const *t, double const *opts,
-                          adatm_plan const *given, uint32_t const *given_grp,
-                          uint32_t const *given_tag, splatt_csf **tensors,
-                          uint32_t *n_csf_out)
+                          adatm_plan const *given, adatm_plan const *given_full,
+                          uint32_t const *given_grp, uint32_t const *given_tag,
+                          splatt_csf **tensors, uint32_t *n_csf_out)
 {
   uint32_t const N = t->nmodes;
   int const alloc = (int)opts[SPLATT_OPTION_CSF_ALLOC];
@@ -305,6 +307,7 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
   int rc = SPLATT_SUCCESS;
   if (given) {
     r->plan = *given;
+    r->full = given_full ? *given_full : *given;
     memcpy(r->use_grp, given_grp, N * sizeof(uint32_t));
     memcpy(r->use_tag, given_tag, N * sizeof(uint32_t));
   } else if (alloc == SPLATT_ADAPTIVE || alloc == SPLATT_ADAPTIVE_TIME_EFFICIENT) {
@@ -336,13 +339,23 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
     }
     if (rc == SPLATT_SUCCESS) {
       r->plan = cands[ch];
+      r->full = r->plan;
       adatm_plan_use(&r->plan, N, r->use_grp, r->use_tag);
     }
   } else if (alloc == SPLATT_ADAPTIVE_SPACE_EFFICIENT) {
-    adatm_set_error("SPLATT_ADAPTIVE_SPACE_EFFICIENT is not supported");
-    rc = SPLATT_ERROR_BADINPUT;
+    adatm_plan cands[256], full;
+    int nc = 0, ch = 0;
+    rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
+                             opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands, 256, &nc,
+                             &ch);
+    if (rc == SPLATT_SUCCESS) {
+      full = cands[ch];
+      r->full = full;
+      adatm_plan_space_efficient(&full, N, &r->plan, r->use_grp, r->use_tag);
+    }
   } else {
     rc = plain_plan(alloc, N, t->dims, &r->plan, r->use_grp, r->use_tag);
+    r->full = r->plan;
   }
   if (rc != SPLATT_SUCCESS) {
     free(r);
@@ -406,7 +419,7 @@ int splatt_csf_convert(splatt_idx_t const nmodes, splatt_idx_t const nnz,
   for (uint32_t m = 0; m < nmodes; ++m)
     for (uint64_t j = 0; j < nnz; ++j) t.ind[m][j] = (uint32_t)inds[m][j];
   memcpy(t.vals, vals, nnz * sizeof(double));
-  rc = build_from_coo(&t, opts, NULL, NULL, NULL, tensors, NULL);
+  rc = build_from_coo(&t, opts, NULL, NULL, NULL, NULL, tensors, NULL);
   adatm_coo_free(&t);
   free(dflt);
   return rc;
@@ -698,11 +711,6 @@ splatt_csf *csf_alloc_adaptive(sptensor_t *const tt,
                                double const *const opts, splatt_idx_t *n_csf)
 {
   if (!tt || !grp_prop || n_grp < 1 || tt->nmodes > MAXN) return NULL;
-  if (opts && (int)opts[SPLATT_OPTION_CSF_ALLOC] == SPLATT_ADAPTIVE_SPACE_EFFICIENT &&
-      n_grp > 1) {
-    fprintf(stderr, "csf_alloc_adaptive: SPACE_EFFICIENT is not supported\n");
-    return NULL;
-  }
   uint32_t const N = (uint32_t)tt->nmodes;
   adatm_plan p;
   memset(&p, 0, sizeof(p));
@@ -714,13 +722,17 @@ splatt_csf *csf_alloc_adaptive(sptensor_t *const tt,
     for (uint32_t m = 0; m < N; ++m) p.grp[g].o_opt[m] = (uint32_t)grp_prop[g].o_opt[m];
   }
   uint32_t ug[MAXN], ut[MAXN];
-  adatm_plan_use(&p, N, ug, ut);
+  adatm_plan q = p;
+  if (opts && (int)opts[SPLATT_OPTION_CSF_ALLOC] == SPLATT_ADAPTIVE_SPACE_EFFICIENT)
+    adatm_plan_space_efficient(&p, N, &q, ug, ut); /* memoising groups' CSFs only */
+  else
+    adatm_plan_use(&p, N, ug, ut);
   adatm_coo t;
   if (adatm_coo_from_sptensor(tt, &t) != SPLATT_SUCCESS) return NULL;
   double *dflt = opts ? NULL : splatt_default_opts();
   splatt_csf *out = NULL;
   uint32_t nc = 0;
-  int const rc = build_from_coo(&t, opts ? opts : dflt, &p, ug, ut, &out, &nc);
+  int const rc = build_from_coo(&t, opts ? opts : dflt, &q, &p, ug, ut, &out, &nc);
   adatm_coo_free(&t);
   free(dflt);
   if (rc != SPLATT_SUCCESS) return NULL;
@@ -749,7 +761,8 @@ rcsf_seq_adaptive *rcsf_alloc_adaptive(splatt_csf *const ct,
     ret[j].begin_imten = b;
     ret[j].n_imten = nim;
     ret[j].rcsfs = (splatt_csf *)calloc(nim, sizeof(splatt_csf));
-    splatt_csf const *c = ct + g;
+    /* TIME_EFFICIENT: CSF g; SPACE_EFFICIENT: the j-th memoising group's */
+    splatt_csf const *c = ct + ((n_csf == n_grp) ? g : j);
     for (uint32_t k = 0; k < nim; ++k) {
       /* vCSF k describes the memo of level b-1-k (adatm_rcsf.c:227-276);
          structure aliases the CSF, values live on the device */
@@ -933,10 +946,10 @@ int splatt_cpd_als_adaptive(splatt_csf const *const tensors,
     return SPLATT_ERROR_BADINPUT;
   }
   /* the registered plan must be the one described by the arguments */
-  if (n_grp != r->plan.n_grp) return SPLATT_ERROR_BADINPUT;
+  if (n_grp != r->full.n_grp) return SPLATT_ERROR_BADINPUT;
   for (splatt_idx_t g = 0; g < n_grp; ++g)
-    if (grp_prop[g].memo_mode != r->plan.grp[g].memo_mode ||
-        grp_prop[g].n_imten != r->plan.grp[g].n_imten)
+    if (grp_prop[g].memo_mode != r->full.grp[g].memo_mode ||
+        grp_prop[g].n_imten != r->full.grp[g].n_imten)
       return SPLATT_ERROR_BADINPUT;
   for (uint32_t m = 0; m < r->N; ++m)
     if (use_csfs[m] != r->use_grp[m] || use_tags[m] != r->use_tag[m])

@@ -629,12 +629,78 @@ splatt_idx_t decide_parameters_auto(sptensor_t *const tt,
   return n_grp;
 }
 
+/*
+ * SPACE_EFFICIENT allocation (adatm_rcsf.c:528-552, mttkrp.c:1976-2029): only
+ * the groups that memoise keep a CSF; a mode whose group stores nothing is
+ * computed from the leaves ("tag 2") of the CSF of the nearest memoising group
+ * before it.  The reference locates that group with locate_ind_special, which
+ * stops after comparing two entries (D7, adatm_utils.c:17-24), and skips the
+ * last group when there is one (D8); here every group is located.
+ */
+int adatm_plan_space_efficient(adatm_plan const *p, uint32_t N, adatm_plan *out,
+                               uint32_t *use_grp, uint32_t *use_tag)
+{
+  int keep[ADATM_MAX_NMODES];
+  uint32_t nk = 0;
+  for (uint32_t g = 0; g < p->n_grp; ++g) keep[g] = p->grp[g].n_imten > 0;
+  for (uint32_t g = 0; g < p->n_grp; ++g) nk += keep[g] ? 1u : 0u;
+  if (nk == 0 || p->n_grp == 1) { /* one CSF either way (adatm_rcsf.c:513-516) */
+    for (uint32_t g = 0; g < p->n_grp; ++g) keep[g] = (g == 0);
+    nk = 1;
+  }
+  memset(out, 0, sizeof(*out));
+  int32_t idx[ADATM_MAX_NMODES];
+  for (uint32_t g = 0; g < p->n_grp; ++g) {
+    idx[g] = -1;
+    if (keep[g]) {
+      idx[g] = (int32_t)out->n_grp;
+      out->grp[out->n_grp++] = p->grp[g];
+    }
+  }
+  out->pspace = p->pspace;
+  out->pops = p->pops;
+  for (uint32_t g = 0; g < p->n_grp; ++g) {
+    uint32_t const m0 = p->grp[g].memo_mode;
+    uint32_t const m1 = (g + 1 < p->n_grp) ? p->grp[g + 1].memo_mode : N;
+    int32_t c = idx[g];
+    uint32_t tag0 = 0, tagr = 1;
+    if (c < 0) { /* the nearest memoising group before g (else the first) */
+      for (int32_t h = (int32_t)g - 1; h >= 0 && c < 0; --h) c = idx[h];
+      if (c < 0) c = 0;
+      tag0 = tagr = 2;
+    }
+    use_grp[m0] = (uint32_t)c;
+    use_tag[m0] = tag0;
+    for (uint32_t m = m0 + 1; m < m1; ++m) {
+      use_grp[m] = (uint32_t)c;
+      use_tag[m] = tagr;
+    }
+  }
+  return (int)nk;
+}
+
 void decide_use_csfs(splatt_idx_t const nmodes,
                      group_properties *const grp_prop, int const n_grp,
                      splatt_idx_t const n_csf, splatt_idx_t *use_csfs,
                      splatt_idx_t *use_tags)
 {
-  (void)n_csf;
+  if ((int)n_csf != n_grp) { /* SPACE_EFFICIENT: CSFs of memoising groups only */
+    adatm_plan p, q;
+    uint32_t ug[ADATM_MAX_NMODES], ut[ADATM_MAX_NMODES];
+    memset(&p, 0, sizeof(p));
+    p.n_grp = (uint32_t)n_grp;
+    for (int g = 0; g < n_grp && g < ADATM_MAX_NMODES; ++g) {
+      p.grp[g].memo_mode = (uint32_t)grp_prop[g].memo_mode;
+      p.grp[g].begin_imten = (uint32_t)grp_prop[g].begin_imten;
+      p.grp[g].n_imten = (uint32_t)grp_prop[g].n_imten;
+    }
+    adatm_plan_space_efficient(&p, (uint32_t)nmodes, &q, ug, ut);
+    for (splatt_idx_t m = 0; m < nmodes; ++m) {
+      use_csfs[m] = ug[m];
+      use_tags[m] = ut[m];
+    }
+    return;
+  }
   for (int g = 0; g < n_grp; ++g) {
     splatt_idx_t const m0 = grp_prop[g].memo_mode;
     splatt_idx_t const m1 = (g + 1 < n_grp) ? grp_prop[g + 1].memo_mode : nmodes;

@@ -123,6 +123,14 @@ int adatm_plan_analytic(adatm_coo const *t, uint64_t rank, int ref_quirks,
                         double space_limit, int device, adatm_plan *cands,
                         int maxc, int *ncands, int *chosen);
 
+/* SPACE_EFFICIENT form of a plan (adatm_rcsf.c:528-552): the memoising
+ * groups only, and the use maps that serve every other group's modes from the
+ * leaves of the nearest memoising group before it (tag 2; mttkrp.c:1976-2029
+ * with D7/D8 corrected).  Returns the number of CSFs kept. */
+int adatm_plan_space_efficient(adatm_plan const *p, uint32_t nmodes,
+                               adatm_plan *out, uint32_t *use_grp,
+                               uint32_t *use_tag);
+
 /* Map each mode to (group, tag): tag 0 = recomputed root, 1 = memo reuse
  * (mttkrp.c:1958-1974). */
 void adatm_plan_use(adatm_plan const *p, uint32_t nmodes, uint32_t *use_grp,

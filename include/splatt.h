@@ -235,11 +235,11 @@ void splatt_free_opts(double *opts);
  * With opts[SPLATT_OPTION_CSF_ALLOC] = SPLATT_ADAPTIVE or
  * SPLATT_ADAPTIVE_TIME_EFFICIENT the returned array holds one CSF per
  * memoization group, and the plan, the vCSF memo buffers and the device copy
- * live in a side handle keyed by the returned pointer.  ONEMODE/ALLMODE
- * return the plain SPLATT layouts (1 or nmodes CSFs); TWOMODE and
- * ADAPTIVE_SPACE_EFFICIENT are rejected with SPLATT_ERROR_BADINPUT.
- * Duplicate coordinates are summed and empty indices removed before
- * the CSF is built (tensor dims shrink accordingly; see sptensor_t.indmap).
+ * live in a side handle keyed by the returned pointer;
+ * ADAPTIVE_SPACE_EFFICIENT keeps the CSFs of the memoising groups only and
+ * computes the other groups' modes from their leaves.  ONEMODE/TWOMODE/ALLMODE
+ * return the plain SPLATT layouts (1, 2 or nmodes CSFs).  Coordinates are
+ * taken as given (duplicates stay separate nonzeros; dims = max index + 1).
  * ----------------------------------------------------------------------- */
 int splatt_csf_load(char const *const fname, splatt_idx_t *nmodes,
                     splatt_csf **tensors, double const *const options);

@@ -155,3 +155,30 @@ def test_plan_use_map():
     grp, tag = A.plan_use(p, 8)
     assert grp == [0, 0, 0, 1, 1, 1, 2, 2]
     assert tag == [0, 1, 1, 0, 1, 1, 0, 1]
+
+
+def _mkplan(groups, N):
+    p = A.Plan()
+    p.n_grp = len(groups)
+    for g, (mm, b, nim) in enumerate(groups):
+        p.grp[g].memo_mode, p.grp[g].begin_imten, p.grp[g].n_imten = mm, b, nim
+        for l in range(N):
+            p.grp[g].o_opt[l] = (mm + l) % N
+    return p
+
+
+@pytest.mark.parametrize("groups,N,kept,grp,tag", [
+    # single-group branch (Appendix A N=5 c0): the trailing group stores
+    # nothing, so mode 4 comes from the leaves of group 0's CSF
+    ([(0, 4, 3), (4, 1, 0)], 5, [0], [0, 0, 0, 0, 0], [0, 1, 1, 1, 2]),
+    # every group memoises (N=5 c3): the same maps as TIME_EFFICIENT
+    ([(0, 3, 2), (3, 2, 1)], 5, [0, 3], [0, 0, 0, 1, 1], [0, 1, 1, 0, 1]),
+    # a group in the middle that stores nothing uses the CSF before it
+    ([(0, 3, 1), (2, 1, 0), (3, 2, 1)], 5, [0, 3], [0, 0, 0, 1, 1], [0, 1, 2, 0, 1]),
+    # one group: one CSF whatever it stores (adatm_rcsf.c:513-516)
+    ([(0, 1, 0)], 4, [0], [0, 0, 0, 0], [0, 1, 1, 1]),
+])
+def test_space_efficient_plans(lib, groups, N, kept, grp, tag):
+    q, g, t = A.plan_space_efficient(_mkplan(groups, N), N)
+    assert [q.grp[i].memo_mode for i in range(q.n_grp)] == kept
+    assert g == grp and t == tag

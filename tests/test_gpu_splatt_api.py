@@ -148,6 +148,34 @@ def test_splatt_cpd_als(prec, tol):
     L.splatt_free_csf(csf, opts)
 
 
+@pytest.mark.parametrize("dims", [[40, 30, 20, 12], [20, 18, 16, 14, 12], [14, 13, 12, 11, 10, 9]])
+def test_space_efficient_allocation(dims):
+    """SPACE_EFFICIENT keeps the memoising groups' CSFs only; every mode
+    (memo root, reuse, or computed from the leaves) matches the oracle, and
+    CP-ALS runs on it."""
+    print(f"space efficient {dims}", flush=True)
+    L = _lib()
+    opts = L.splatt_default_opts()
+    opts[A.OPT_CSF_ALLOC] = A.ADAPTIVE_SPACE_EFFICIENT
+    inds, vals = _tensor(len(dims) * 3 + 1, dims, 4000)
+    csf = _convert(L, inds, vals, opts)
+    R = 8
+    rng = np.random.default_rng(2)
+    U = [np.ascontiguousarray(rng.random((d, R))) for d in dims]
+    for m in range(len(dims)):
+        ref = O.coo_mttkrp(inds, vals, U, m)
+        assert O.rel_fro(_mttkrp(L, csf, m, U, opts), ref) <= 1e-12, m
+    k = Kruskal()
+    opts[A.OPT_NITER] = 3
+    opts[A.OPT_TOLERANCE] = 0.0
+    opts[A.OPT_VERBOSITY] = 0
+    assert L.splatt_cpd_als(csf, R, opts, C.byref(k)) == 1
+    assert 0.0 < k.fit < 1.0
+    L.splatt_free_kruskal(C.byref(k))
+    L.splatt_free_csf(csf, opts)
+    L.splatt_free_opts(opts)
+
+
 @pytest.mark.parametrize("planner", [A.PLAN_LEARNED, A.PLAN_EXHAUSTIVE])
 def test_planner_options(planner, tmp_path, monkeypatch):
     """The learned (analytic fallback without a model) and exhaustive planners
