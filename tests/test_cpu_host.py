@@ -91,3 +91,30 @@ def test_library_exports_every_declared_symbol():
     assert len(declared) > 40
     missing = [n for n in declared if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_engine_rejects_bad_arguments_before_any_device_call(lib):
+    """Shape, rank, tile and plan checks fail with BADINPUT (2) and a message,
+    without touching a GPU."""
+    import ctypes as C
+    spec = A.make_spec([30, 20, 10, 8], 2000, seed=4)
+    t = A.Tensor.synth(spec)
+    t.canonicalize()
+    plan = A.plan_candidates(4, t.dims)[0]
+    L = A.lib()
+    h = C.c_void_p()
+    for rank, tiles in ((65, (0, 0)), (0, (0, 0)), (8, (4, 0)), (8, (0, 5000))):
+        rc = L.adatm_engine_create_tiled(C.byref(t.c), rank, C.byref(plan), A.FP32, 0,
+                                         tiles[0], tiles[1], C.byref(h))
+        assert rc == 2 and not h.value, (rank, tiles, rc)
+    bad = A.Plan()
+    bad.n_grp = 1
+    bad.grp[0].memo_mode, bad.grp[0].begin_imten, bad.grp[0].n_imten = 0, 3, 5
+    for l in range(4):
+        bad.grp[0].o_opt[l] = l
+    rc = L.adatm_engine_create(C.byref(t.c), 8, C.byref(bad), A.FP32, 0, C.byref(h))
+    assert rc == 2 and b"stores" in L.adatm_last_error()
+    big = A.make_spec([70_000_000, 20, 10, 8], 100, seed=5)
+    tb = A.Tensor.synth(big)
+    rc = L.adatm_engine_create(C.byref(tb.c), 64, C.byref(plan), A.FP32, 0, C.byref(h))
+    assert rc == 2 and b"2^32" in L.adatm_last_error()
