@@ -22,6 +22,27 @@ __device__ __forceinline__ float ldu(bf16raw const *p)
   return __uint_as_float(((uint32_t)p->v) << 16);
 }
 
+/* value of v held by wave lane src (ds_bpermute: the source lane must be
+ * active in the calling branch) */
+__device__ __forceinline__ uint32_t lane_get(uint32_t v, uint32_t src)
+{
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+__device__ __forceinline__ float lane_get(float v, uint32_t src)
+{
+  return __uint_as_float(lane_get(__float_as_uint(v), src));
+}
+__device__ __forceinline__ double lane_get(double v, uint32_t src)
+{
+  uint32_t w[2];
+  __builtin_memcpy(w, &v, 8);
+  w[0] = lane_get(w[0], src);
+  w[1] = lane_get(w[1], src);
+  double r;
+  __builtin_memcpy(&r, w, 8);
+  return r;
+}
+
 /* FP32 -> BF16 with round-to-nearest-even (NaN/Inf keep their class) */
 __device__ __forceinline__ uint16_t to_bf16(float f)
 {

@@ -39,18 +39,15 @@
 
 namespace {
 
-/* Children per batch (RB): all of a batch's loads -- child rows and the rows
- * of the intermediate nodes that close inside it -- are issued together.
- * Larger batches keep more loads in flight, smaller ones need fewer
- * registers; the launch picks RB (adatm_kchain.rb, tuned per plan) from
- * {4, 8, 16} for K <= 2, {4, 8} for K = 3 and 4 for deeper passes. */
-/* single-level passes load the next batch's metadata while the current one
- * computes; multi-level passes keep the registers for occupancy instead */
-template <int K>
-struct pipe_of {
-  static constexpr bool v = (K == 1);
-};
-
+/* The item's children are taken in windows of R: lane j of the group loads
+ * the metadata of window child j (its id, first differing level and value)
+ * in one transaction, the window's close masks come from ballots, and the
+ * node ids of the intermediate levels from one more per-lane load, so a
+ * window costs two dependent round trips instead of two per child.  The
+ * children are then folded in batches of RB (4, 8 or 16; adatm_kchain.rb,
+ * tuned per plan): a batch's row loads -- its children's factor rows and
+ * those of the intermediate nodes they belong to -- are issued together,
+ * with the per-child metadata read from the holding lane (ds_bpermute). */
 template <typename T, typename TU, bool LEAF, int K, int RB>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
@@ -62,7 +59,8 @@ __global__ __launch_bounds__(256) void k_reduce(
   uint32_t const lane = threadIdx.x & 63u;
   uint32_t const grp = lane / R;
   if (grp >= gpw) return;
-  uint32_t const col = lane - grp * R;
+  uint32_t const gb = grp * R; /* the group's first lane */
+  uint32_t const col = lane - gb;
   uint32_t const k =
       (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (k >= nitems) return;
@@ -81,100 +79,102 @@ __global__ __launch_bounds__(256) void k_reduce(
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
 
-  /* batch metadata (child ids, first differing level after each child --
-     the item's last child closes every level -- and the child values or
-     input rows), loaded one batch ahead */
-  uint32_t id[RB], f[RB];
-  T xv[RB];
-  auto load_meta = [&](uint32_t c0, uint32_t *mid, uint32_t *mf, T *mx) {
-    uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
-#pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      uint32_t const c = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
-      mid[i] = idb[c];
-      mf[i] = (c0 + i + 1 >= b) ? 0u : (uint32_t)cfd[c];
-      mx[i] = LEAF ? xin[c] : xin[(size_t)c * R + col];
-    }
+  /* window metadata of child w0 + col (past the item's end: its last
+     child); the item's last child closes every level */
+  auto load_win = [&](uint32_t w0, uint32_t &mid, uint32_t &mf, T &mx) {
+    uint32_t c = w0 + col;
+    c = (c < b) ? c : b - 1;
+    mid = idb[c];
+    uint32_t const fb = cfd[c];
+    mf = (c + 1 >= b) ? 0u : fb;
+    if constexpr (LEAF) mx = xin[c];
   };
-  constexpr bool PIPE = pipe_of<K>::v;
-  if (PIPE) load_meta(a, id, f, xv);
+  uint32_t wid, wf;
+  T wx = (T)0;
+  load_win(a, wid, wf, wx);
 
-  for (uint32_t c0 = a; c0 < b; c0 += RB) {
-    uint32_t const nb = (b - c0 < (uint32_t)RB) ? b - c0 : (uint32_t)RB;
-    if (!PIPE) load_meta(c0, id, f, xv);
-    /* close flags: child i ends the node of relative level j when
-       f[i] <= lt + j (nested: a level closes only if every deeper one does).
-       The node of level j current at child i is node[j] plus the closes of
-       level j before i, so each closing child loads its own node's id and
-       factor row: all of a batch's loads are issued together, indexed by the
-       child, and the folds below are selects. */
-    uint32_t ci[K][RB];
+  for (uint32_t w0 = a; w0 < b; w0 += R) {
+    uint32_t const nw = (b - w0 < R) ? b - w0 : R;
+    uint64_t const valid = (nw >= 64u) ? ~0ull : ((1ull << nw) - 1ull);
+    uint32_t const me = (col < nw) ? col : nw - 1u;
+    uint64_t const below = (1ull << me) - 1ull;
+    /* bit i of cm[j]: window child i ends its node of relative level j
+       (first differing level <= lt + j; nested, deeper levels close too).
+       The lane's child belongs to node node[j] + (closes before it): wn[j]
+       is that node's output row (j = 0) or id (j >= 1). */
+    uint64_t cm[K];
+    uint32_t wn[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      uint32_t cnt = 0;
+      cm[j] = ((uint64_t)__ballot(wf <= lt + (uint32_t)j) >> gb) & valid;
+      uint32_t const n = node[j] + (uint32_t)__popcll(cm[j] & below);
+      if (j == 0)
+        wn[0] = ch.orow ? ch.orow[n] : n;
+      else
+        wn[j] = ch.ids[j][n];
+      node[j] += (uint32_t)__popcll(cm[j]);
+    }
+    uint32_t nid = 0, nf = 0;
+    T nx = (T)0;
+    if (w0 + R < b) load_win(w0 + R, nid, nf, nx);
+
+    for (uint32_t s0 = 0; s0 < nw; s0 += RB) {
+      uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
+      uint32_t src[RB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+        src[i] = gb + s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
+      T xv[RB], uv[RB];
+      T uc[K][RB];
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
-        ci[j][i] = node[j] + cnt;
-        cnt += ((uint32_t)i < nb && f[i] <= lt + j) ? 1u : 0u;
+        uint32_t const id = lane_get(wid, src[i]);
+        uv[i] = (T)ldu(Ub + (id * R + col));
+        if constexpr (LEAF)
+          xv[i] = lane_get(wx, src[i]);
+        else
+          xv[i] = xin[(size_t)(w0 + src[i] - gb) * R + col];
       }
-      node[j] += cnt;
-    }
-    /* loads are unconditional: ci[j][i] of an existing child is the node
-       holding it, a valid node; past the item's end the last child's is
-       reused */
-    uint32_t cid[K][RB];
 #pragma unroll
-    for (int j = 1; j < K; ++j)
+      for (int j = 1; j < K; ++j)
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
-        cid[j][i] = ch.ids[j][(uint32_t)i < nb ? ci[j][i] : ci[j][0]];
-    T uv[RB];
+        for (int i = 0; i < RB; ++i) {
+          uint32_t const id = lane_get(wn[j], src[i]);
+          uc[j][i] = (T)ldu((TU const *)ch.U[j] + (id * R + col));
+        }
+      uint32_t cb[K];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) uv[i] = (T)ldu(Ub + (id[i] * R + col));
-    T uc[K][RB];
+      for (int j = 0; j < K; ++j) cb[j] = (uint32_t)(cm[j] >> s0);
+      auto step = [&](int i) {
+        acc[K - 1] += xv[i] * uv[i];
 #pragma unroll
-    for (int j = 1; j < K; ++j)
+        for (int j = K - 1; j >= 1; --j) {
+          /* fold the level-j node into its parent with its own factor row */
+          bool const c = (cb[j] >> i) & 1u;
+          T const t = acc[j] * uc[j][i];
+          acc[j - 1] = c ? acc[j - 1] + t : acc[j - 1];
+          acc[j] = c ? (T)0 : acc[j];
+        }
+        if ((cb[0] >> i) & 1u) {
+          uint32_t const row = lane_get(wn[0], gb + s0 + (uint32_t)i);
+          T *dst = (slot != NONE32) ? carry + (size_t)slot * R
+                                    : out + (size_t)row * R;
+          dst[col] = acc[0];
+          acc[0] = (T)0;
+        }
+      };
+      if (nb == (uint32_t)RB) {
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
-        uc[j][i] = (T)ldu((TU const *)ch.U[j] + (cid[j][i] * R + col));
-    uint32_t nid[RB], nf[RB];
-    T nxv[RB];
-    uint32_t const c1 = c0 + RB;
-    if (PIPE && c1 < b) load_meta(c1, nid, nf, nxv);
-    auto step = [&](int i) {
-      acc[K - 1] += xv[i] * uv[i];
+        for (int i = 0; i < RB; ++i) step(i);
+      } else {
 #pragma unroll
-      for (int j = K - 1; j >= 1; --j) {
-        /* fold node ci[j][i] into its parent with its own factor row */
-        bool const c = f[i] <= lt + j;
-        T const t = acc[j] * uc[j][i];
-        acc[j - 1] = c ? acc[j - 1] + t : acc[j - 1];
-        acc[j] = c ? (T)0 : acc[j];
-      }
-      if (f[i] <= lt) {
-        T *dst = (slot != NONE32)
-                     ? carry + (size_t)slot * R
-                     : out + (size_t)(ch.orow ? ch.orow[ci[0][i]] : ci[0][i]) * R;
-        dst[col] = acc[0];
-        acc[0] = (T)0;
-      }
-    };
-    if (nb == (uint32_t)RB) {
-#pragma unroll
-      for (int i = 0; i < RB; ++i) step(i);
-    } else {
-#pragma unroll
-      for (int i = 0; i < RB; ++i)
-        if ((uint32_t)i < nb) step(i);
-    }
-    if (PIPE && c1 < b) {
-#pragma unroll
-      for (int i = 0; i < RB; ++i) {
-        id[i] = nid[i];
-        f[i] = nf[i];
-        xv[i] = nxv[i];
+        for (int i = 0; i < RB; ++i)
+          if ((uint32_t)i < nb) step(i);
       }
     }
+    wid = nid;
+    wf = nf;
+    wx = nx;
   }
 }
 
@@ -383,15 +383,17 @@ template <typename T, typename TU, bool LEAF, int K>
 void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                void const *xin, void *out, void *carry, uint32_t R)
 {
+  /* a batch never spans windows: no more children per batch than R */
+  uint32_t const rb = (ch->rb > R) ? (R >= 8 ? 8u : 4u) : ch->rb;
   if constexpr (K <= 2) {
-    if (ch->rb == 16)
+    if (rb == 16)
       go_reduce_rb<T, TU, LEAF, K, 16>(s, w, ch, xin, out, carry, R);
-    else if (ch->rb == 4)
+    else if (rb == 4)
       go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
     else
       go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
   } else if constexpr (K == 3) {
-    if (ch->rb == 4)
+    if (rb == 4)
       go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
     else
       go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
