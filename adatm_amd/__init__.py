@@ -34,6 +34,8 @@ OPT_CSF_ALLOC, OPT_TILE, OPT_TILEDEPTH, OPT_DECOMP, OPT_COMM = 5, 6, 7, 8, 9
 OPT_PRECISION, OPT_PLANNER, OPT_GRAPH, OPT_REFQUIRKS = 10, 11, 12, 13
 OPT_SPACE_LIMIT, OPT_DEVICE, OPT_RANK_HINT, OPT_NOPTIONS = 14, 15, 16, 17
 PLAN_ANALYTIC, PLAN_LEARNED, PLAN_EXHAUSTIVE = 0, 1, 2
+# flag of Engine.set_batch: the next batch's rows load while one folds
+BATCH_PIPELINED = 0x100
 CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
 ADAPTIVE, ADAPTIVE_TIME_EFFICIENT, ADAPTIVE_SPACE_EFFICIENT = 3, 4, 5
 
@@ -553,6 +555,8 @@ class Engine:
             self.set_batch(int(tiles[2]))
 
     def set_batch(self, batch: int) -> None:
+        """children per reduction batch: 0 (default), 4, 8 or 16, optionally
+        | BATCH_PIPELINED"""
         check(lib().adatm_engine_set_batch(self.h, batch), "engine_set_batch")
 
     def close(self) -> None:

@@ -57,6 +57,7 @@ typedef struct {
 typedef struct {
   uint32_t K, lt;
   uint32_t rb;          /* children per batch: 4, 8 or 16 (0: 8, or 4 if K > 3) */
+  uint32_t db;          /* nonzero: load the next batch while one folds (K <= 3) */
   uint32_t const *orow; /* output row of each top node (NULL: the node) */
   uint8_t const *cfd;
   uint32_t const *ids[ADATM_KMAXFUSE + 1];

@@ -48,7 +48,7 @@ namespace {
  * tuned per plan): a batch's row loads -- its children's factor rows and
  * those of the intermediate nodes they belong to -- are issued together,
  * with the per-child metadata read from the holding lane (ds_bpermute). */
-template <typename T, typename TU, bool LEAF, int K, int RB>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -118,30 +118,29 @@ __global__ __launch_bounds__(256) void k_reduce(
     T nx = (T)0;
     if (w0 + R < b) load_win(w0 + R, nid, nf, nx);
 
-    for (uint32_t s0 = 0; s0 < nw; s0 += RB) {
+    /* a batch's row loads: its children's factor rows and input values or
+       rows, and the factor rows of the level-j nodes holding them */
+    auto load_rows = [&](uint32_t s0, T (&xv)[RB], T (&uv)[RB], T (&uc)[K][RB]) {
       uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
-      uint32_t src[RB];
-#pragma unroll
-      for (int i = 0; i < RB; ++i)
-        src[i] = gb + s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
-      T xv[RB], uv[RB];
-      T uc[K][RB];
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
-        uint32_t const id = lane_get(wid, src[i]);
+        uint32_t const src = gb + s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
+        uint32_t const id = lane_get(wid, src);
         uv[i] = (T)ldu(Ub + (id * R + col));
         if constexpr (LEAF)
-          xv[i] = lane_get(wx, src[i]);
+          xv[i] = lane_get(wx, src);
         else
-          xv[i] = xin[(size_t)(w0 + src[i] - gb) * R + col];
-      }
+          xv[i] = xin[(size_t)(w0 + src - gb) * R + col];
 #pragma unroll
-      for (int j = 1; j < K; ++j)
-#pragma unroll
-        for (int i = 0; i < RB; ++i) {
-          uint32_t const id = lane_get(wn[j], src[i]);
-          uc[j][i] = (T)ldu((TU const *)ch.U[j] + (id * R + col));
+        for (int j = 1; j < K; ++j) {
+          uint32_t const nid_j = lane_get(wn[j], src);
+          uc[j][i] = (T)ldu((TU const *)ch.U[j] + (nid_j * R + col));
         }
+      }
+    };
+    auto fold = [&](uint32_t s0, T const (&xv)[RB], T const (&uv)[RB],
+                    T const (&uc)[K][RB]) {
+      uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
       uint32_t cb[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) cb[j] = (uint32_t)(cm[j] >> s0);
@@ -170,6 +169,29 @@ __global__ __launch_bounds__(256) void k_reduce(
 #pragma unroll
         for (int i = 0; i < RB; ++i)
           if ((uint32_t)i < nb) step(i);
+      }
+    };
+    /* DB: the next batch's rows are loaded while the current one folds */
+    T xa[RB], ua[RB], ca[K][RB];
+    if constexpr (DB) load_rows(0, xa, ua, ca);
+    for (uint32_t s0 = 0; s0 < nw; s0 += RB) {
+      if constexpr (DB) {
+        bool const more = s0 + RB < nw;
+        T xb[RB], ub[RB], cb2[K][RB];
+        if (more) load_rows(s0 + RB, xb, ub, cb2);
+        fold(s0, xa, ua, ca);
+        if (more) {
+#pragma unroll
+          for (int i = 0; i < RB; ++i) {
+            xa[i] = xb[i];
+            ua[i] = ub[i];
+#pragma unroll
+            for (int j = 1; j < K; ++j) ca[j][i] = cb2[j][i];
+          }
+        }
+      } else {
+        load_rows(s0, xa, ua, ca);
+        fold(s0, xa, ua, ca);
       }
     }
     wid = nid;
@@ -367,14 +389,14 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
   return hipGetLastError();
 }
 
-template <typename T, typename TU, bool LEAF, int K, int RB>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB>
 void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                   void const *xin, void *out, void *carry, uint32_t R)
 {
   uint32_t const gpw = 64u / R;
   uint32_t const per_block = 4u * gpw;
   dim3 const grid((w->nitems + per_block - 1) / per_block);
-  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB>), grid, dim3(256), 0, s,
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB>), grid, dim3(256), 0, s,
                      w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
                      (T const *)xin, (T *)out, (T *)carry, R, gpw);
 }
@@ -385,20 +407,30 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
 {
   /* a batch never spans windows: no more children per batch than R */
   uint32_t const rb = (ch->rb > R) ? (R >= 8 ? 8u : 4u) : ch->rb;
+  bool const db = ch->db != 0;
+  if constexpr (K <= 3) {
+    if (db) {
+      if (rb == 4)
+        go_reduce_rb<T, TU, LEAF, K, 4, true>(s, w, ch, xin, out, carry, R);
+      else
+        go_reduce_rb<T, TU, LEAF, K, 8, true>(s, w, ch, xin, out, carry, R);
+      return;
+    }
+  }
   if constexpr (K <= 2) {
     if (rb == 16)
-      go_reduce_rb<T, TU, LEAF, K, 16>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 16, false>(s, w, ch, xin, out, carry, R);
     else if (rb == 4)
-      go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
     else
-      go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 8, false>(s, w, ch, xin, out, carry, R);
   } else if constexpr (K == 3) {
     if (rb == 4)
-      go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
     else
-      go_reduce_rb<T, TU, LEAF, K, 8>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 8, false>(s, w, ch, xin, out, carry, R);
   } else {
-    go_reduce_rb<T, TU, LEAF, K, 4>(s, w, ch, xin, out, carry, R);
+    go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
   }
 }
 

@@ -185,7 +185,37 @@ def test_tiles_and_autotuner():
         e.close()
     (rt, st, bt), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
     assert rt in (32, 64, 128, 256, 512) and st in (16, 32, 64, 128, 256)
-    assert bt in (0, 4, 16) and ms > 0
+    assert bt in (0, 4, 16, 4 | A.BATCH_PIPELINED, 8 | A.BATCH_PIPELINED) and ms > 0
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+def test_every_batch_every_plan(prec):
+    """Children per reduction batch (4/8/16, plain or pipelined) change only
+    when rows are loaded, never the summation order: every batch value gives
+    the oracle's MTTKRP on every plan, and all agree bit for bit."""
+    print(f"batches prec={prec}", flush=True)
+    t = _tensor([120, 90, 60, 40, 20, 10], 30000, A.DIST_ZIPF, seed=31)
+    R = 16
+    U = _factors(t, R, seed=9)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
+    P = A.BATCH_PIPELINED
+    for plan in A.plan_candidates(t.nmodes, t.dims):
+        e = A.Engine(t, R, plan, precision=prec, tiles=(64, 0, 0))
+        for m in range(t.nmodes):
+            e.set_factor(m, U[m])
+        first = None
+        for batch in (0, 4, 8, 16, 4 | P, 8 | P):
+            e.set_batch(batch)
+            out = [e.mttkrp(m) for m in range(t.nmodes)]
+            for m in range(t.nmodes):
+                assert O.rel_fro(out[m], ref[m]) <= TOL[prec], (plan.groups(t.nmodes), batch, m)
+            if first is None:
+                first = out
+            else:
+                for m in range(t.nmodes):
+                    assert np.array_equal(out[m], first[m]), (plan.groups(t.nmodes), batch, m)
+        e.close()
 
 
 def test_batched_cpd_matches_each_instance():
