@@ -43,6 +43,27 @@ __device__ __forceinline__ double lane_get(double v, uint32_t src)
   return r;
 }
 
+/* value of v held by lane l of the wave, l wave-uniform (v_readlane: the
+ * result is a scalar) */
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l)
+{
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float lane_read(float v, uint32_t l)
+{
+  return __uint_as_float(lane_read(__float_as_uint(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, uint32_t l)
+{
+  uint32_t w[2];
+  __builtin_memcpy(w, &v, 8);
+  w[0] = lane_read(w[0], l);
+  w[1] = lane_read(w[1], l);
+  double r;
+  __builtin_memcpy(&r, w, 8);
+  return r;
+}
+
 /* FP32 -> BF16 with round-to-nearest-even (NaN/Inf keep their class) */
 __device__ __forceinline__ uint16_t to_bf16(float f)
 {

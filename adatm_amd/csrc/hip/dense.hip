@@ -213,55 +213,90 @@ __global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
   }
 }
 
+/* out[e] = sum over the nblk partial rows of part[b*len + e] in a fixed
+ * order: 8 interleaved chunk sums per element (chunk c takes rows c, c+8,
+ * ...), combined in chunk order.  Block: 32 elements x 8 chunks. */
 __global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ part,
                                                   uint32_t nblk, uint32_t len,
                                                   double *__restrict__ out)
 {
-  for (uint32_t e = threadIdx.x + blockIdx.x * DB; e < len; e += DB * gridDim.x) {
-    double s = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) s += part[(size_t)b * len + e];
-    out[e] = s;
+  __shared__ double red[DB];
+  uint32_t const c = threadIdx.x >> 5, el = threadIdx.x & 31u;
+  uint32_t const e = blockIdx.x * 32u + el;
+  double s = 0.0;
+  if (e < len)
+    for (uint32_t b = c; b < nblk; b += 8) s += part[(size_t)b * len + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (c == 0 && e < len) {
+    double t = red[el];
+#pragma unroll
+    for (uint32_t q = 1; q < 8; ++q) t += red[q * 32u + el];
+    out[e] = t;
   }
 }
 
-/* Vinv = (hadamard_{k != m} G_k)^-1 by in-place Gauss-Jordan; the matrix
- * is symmetric positive definite, so no pivoting is needed.  One block. */
+/* Vinv = (hadamard_{k != m} G_k)^-1 by Gauss-Jordan without pivoting (the
+ * matrix is symmetric positive definite).  The matrix lives in registers:
+ * thread (s, j), s = tid / 64, j = tid % 64, holds column j of rows s, s+4,
+ * s+8, ...; each pivot step broadcasts the pivot row and the pivot column
+ * through LDS, double-buffered so that one barrier per pivot suffices.
+ * R <= 64, one block. */
 __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
                                                   uint32_t N, uint32_t m,
                                                   uint32_t R,
                                                   double *__restrict__ Vinv,
                                                   int *__restrict__ status)
 {
-  HIP_DYNAMIC_SHARED(double, a) /* R x R, then R elimination factors */
-  double *fac = a + (size_t)R * R;
-  __shared__ double pinv;
-  __shared__ int bad;
-  for (uint32_t x = threadIdx.x; x < R * R; x += DB) {
+  constexpr int Q = 16; /* rows per thread */
+  __shared__ double rowk[2][64], colk[2][64];
+  uint32_t const s = threadIdx.x >> 6, j = threadIdx.x & 63u;
+  double a[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint32_t const i = s + 4u * (uint32_t)q;
     double v = 1.0;
-    for (uint32_t k = 0; k < N; ++k)
-      if (k != m) v *= G[(size_t)k * R * R + x];
-    a[x] = v;
+    if (i < R && j < R) {
+      size_t const x = (size_t)i * R + j;
+      for (uint32_t k = 0; k < N; ++k)
+        if (k != m) v *= G[(size_t)k * R * R + x];
+    }
+    a[q] = v;
   }
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
+  int bad = 0;
   for (uint32_t k = 0; k < R; ++k) {
-    if (threadIdx.x == 0) {
-      double const p = a[k * R + k];
-      if (!(p > 0.0)) bad = 1;
-      pinv = 1.0 / p;
+    uint32_t const b = k & 1u;
+    if (j < R) {
+      if ((k & 3u) == s) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if ((uint32_t)q == (k >> 2)) v = a[q];
+        rowk[b][j] = v;
+      }
+      if (j == k) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (s + 4u * (uint32_t)q < R) colk[b][s + 4u * (uint32_t)q] = a[q];
+      }
     }
-    for (uint32_t i = threadIdx.x; i < R; i += DB) fac[i] = a[i * R + k];
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < R; j += DB)
-      a[k * R + j] = ((j == k) ? 1.0 : a[k * R + j]) * pinv;
-    __syncthreads();
-    for (uint32_t x = threadIdx.x; x < R * R; x += DB) {
-      uint32_t const i = x / R, j = x - i * R;
-      if (i != k) a[x] = ((j == k) ? 0.0 : a[x]) - fac[i] * a[k * R + j];
+    double const p = rowk[b][k];
+    if (!(p > 0.0)) bad = 1;
+    double const pinv = 1.0 / p;
+    double const rk = ((j == k) ? 1.0 : rowk[b][j < R ? j : 0]) * pinv;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint32_t const i = s + 4u * (uint32_t)q;
+      double const f = colk[b][i < R ? i : 0];
+      a[q] = (i == k) ? rk : ((j == k) ? 0.0 : a[q]) - f * rk;
     }
-    __syncthreads();
   }
-  for (uint32_t x = threadIdx.x; x < R * R; x += DB) Vinv[x] = a[x];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint32_t const i = s + 4u * (uint32_t)q;
+    if (i < R && j < R) Vinv[(size_t)i * R + j] = a[q];
+  }
   if (threadIdx.x == 0 && status) *status = bad;
 }
 
@@ -310,21 +345,36 @@ __global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
   }
 }
 
-/* lambda from the column statistics: 2-norm (first sweep) or max(1, max) */
-__global__ void k_lambda(double const *__restrict__ part, uint32_t nblk,
-                         uint32_t R, uint32_t const *__restrict__ iter,
-                         double *__restrict__ lambda)
+/* lambda from the column statistics: 2-norm (first sweep) or max(1, max).
+ * Block: R columns x C = DB/R chunks of the partial rows (chunk c takes rows
+ * c, c+C, ...), combined in chunk order (R <= 64). */
+__global__ __launch_bounds__(DB) void k_lambda(double const *__restrict__ part,
+                                               uint32_t nblk, uint32_t R,
+                                               uint32_t const *__restrict__ iter,
+                                               double *__restrict__ lambda)
 {
-  uint32_t const r = threadIdx.x;
-  if (r >= R) return;
-  int const first = (*iter == 0u);
+  __shared__ double rs[DB], rm[DB];
+  uint32_t const C = DB / R;
+  uint32_t const c = threadIdx.x / R, r = threadIdx.x - c * R;
   double s = 0.0, m = -1.0e300;
-  for (uint32_t b = 0; b < nblk; ++b) {
-    s += part[(size_t)b * 2 * R + r];
-    double const v = part[(size_t)b * 2 * R + R + r];
-    m = v > m ? v : m;
+  if (c < C)
+    for (uint32_t b = c; b < nblk; b += C) {
+      s += part[(size_t)b * 2 * R + r];
+      double const v = part[(size_t)b * 2 * R + R + r];
+      m = v > m ? v : m;
+    }
+  rs[threadIdx.x] = s;
+  rm[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    for (uint32_t q = 1; q < C; ++q) {
+      s += rs[q * R + r];
+      double const v = rm[q * R + r];
+      m = v > m ? v : m;
+    }
+    int const first = (*iter == 0u);
+    lambda[r] = first ? sqrt(s) : (m > 1.0 ? m : 1.0);
   }
-  lambda[r] = first ? sqrt(s) : (m > 1.0 ? m : 1.0);
 }
 
 /* per-block partial of sum_i A[i,r] * M[i,r] */
@@ -423,6 +473,14 @@ __global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ par
   }
 }
 
+/* blocks of the row-parallel dense steps: at least `rows` rows each, at most
+ * NBLK_MAX blocks (the partial-sum rows) */
+uint32_t rows_blocks(uint64_t I, uint32_t rows)
+{
+  uint64_t const b = (I + rows - 1) / rows;
+  return (uint32_t)(b < 1 ? 1 : (b > NBLK_MAX ? NBLK_MAX : b));
+}
+
 uint32_t dense_blocks(uint64_t I)
 {
   uint64_t const b = (I + 255) / 256;
@@ -433,7 +491,7 @@ template <typename T>
 hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
                      double const *lambda, double *part, double *G_out)
 {
-  uint32_t const nb = dense_blocks(I);
+  uint32_t const nb = rows_blocks(I, 32);
   if constexpr (sizeof(T) == 4) {
     hipLaunchKernelGGL(k_gram_mfma, dim3(nb), dim3(DB), 0, s, (float *)A, Abf, I, R,
                        lambda, part);
@@ -445,7 +503,7 @@ hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint32_t const len = R * R;
-  hipLaunchKernelGGL(k_sum_parts, dim3((len + DB - 1) / DB), dim3(DB), 0, s,
+  hipLaunchKernelGGL(k_sum_parts, dim3((len + 31) / 32), dim3(DB), 0, s,
                      part, nb, len, G_out);
   return hipGetLastError();
 }
@@ -509,9 +567,8 @@ int adatm_k_sqnorm(hipStream_t s, int prec, void const *x, uint64_t n,
 int adatm_k_solve_inv(hipStream_t s, double const *G, uint32_t N, uint32_t m,
                       uint32_t R, double *Vinv, int *status)
 {
-  size_t const lds = ((size_t)R * R + R) * sizeof(double);
-  hipLaunchKernelGGL(k_solve_inv, dim3(1), dim3(DB), lds, s, G, N, m, R, Vinv,
-                     status);
+  if (R > 64) return herr(hipErrorInvalidValue);
+  hipLaunchKernelGGL(k_solve_inv, dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
   return herr(hipGetLastError());
 }
 
@@ -520,7 +577,7 @@ int adatm_k_update(hipStream_t s, int prec, void const *M, double const *Vinv,
                    uint32_t const *iter, double *lambda, double *part,
                    double *G_out)
 {
-  uint32_t const nb = dense_blocks(I);
+  uint32_t const nb = rows_blocks(I, DB / R);
   size_t const lds = ((size_t)R * R + 2 * DB) * sizeof(double);
   hipError_t e;
   if (prec == KP_FP64)
@@ -531,8 +588,7 @@ int adatm_k_update(hipStream_t s, int prec, void const *M, double const *Vinv,
                        (float const *)M, Vinv, (float *)A, I, R, part);
   e = hipGetLastError();
   if (e != hipSuccess) return herr(e);
-  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(R < 64 ? 64 : ((R + 63) / 64) * 64),
-                     0, s, part, nb, R, iter, lambda);
+  hipLaunchKernelGGL(k_lambda, dim3(1), dim3(DB), 0, s, part, nb, R, iter, lambda);
   e = hipGetLastError();
   if (e != hipSuccess) return herr(e);
   if (prec == KP_FP64)

@@ -47,8 +47,9 @@ namespace {
  * children are then folded in batches of RB (4, 8 or 16; adatm_kchain.rb,
  * tuned per plan): a batch's row loads -- its children's factor rows and
  * those of the intermediate nodes they belong to -- are issued together,
- * with the per-child metadata read from the holding lane (ds_bpermute). */
-template <typename T, typename TU, bool LEAF, int K, int RB, bool DB>
+ * with the per-child metadata read from the holding lane (ds_bpermute; for
+ * R > 32, one group per wave, v_readlane into scalars). */
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -57,13 +58,29 @@ __global__ __launch_bounds__(256) void k_reduce(
     uint32_t R, uint32_t gpw)
 {
   uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / R;
-  if (grp >= gpw) return;
+  /* ONE (R > 32): one group per wave, so the item index, the window and
+     every per-child value read from a lane are wave-uniform (scalar) */
+  uint32_t grp, k;
+  if constexpr (ONE) {
+    if (lane >= R) return;
+    grp = 0;
+    k = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+                                       (threadIdx.x >> 6));
+  } else {
+    grp = lane / R;
+    if (grp >= gpw) return;
+    k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+  }
   uint32_t const gb = grp * R; /* the group's first lane */
   uint32_t const col = lane - gb;
-  uint32_t const k =
-      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (k >= nitems) return;
+  /* value v of the group's lane idx */
+  auto getl = [&](auto v, uint32_t idx) -> decltype(v) {
+    if constexpr (ONE)
+      return lane_read(v, idx);
+    else
+      return lane_get(v, gb + idx);
+  };
 
   uint32_t const a = istart[k], b = istart[k + 1];
   uint32_t const slot = islot[k];
@@ -124,16 +141,16 @@ __global__ __launch_bounds__(256) void k_reduce(
       uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
-        uint32_t const src = gb + s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
-        uint32_t const id = lane_get(wid, src);
+        uint32_t const src = s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
+        uint32_t const id = getl(wid, src);
         uv[i] = (T)ldu(Ub + (id * R + col));
         if constexpr (LEAF)
-          xv[i] = lane_get(wx, src);
+          xv[i] = getl(wx, src);
         else
-          xv[i] = xin[(size_t)(w0 + src - gb) * R + col];
+          xv[i] = xin[(size_t)(w0 + src) * R + col];
 #pragma unroll
         for (int j = 1; j < K; ++j) {
-          uint32_t const nid_j = lane_get(wn[j], src);
+          uint32_t const nid_j = getl(wn[j], src);
           uc[j][i] = (T)ldu((TU const *)ch.U[j] + (nid_j * R + col));
         }
       }
@@ -155,7 +172,7 @@ __global__ __launch_bounds__(256) void k_reduce(
           acc[j] = c ? (T)0 : acc[j];
         }
         if ((cb[0] >> i) & 1u) {
-          uint32_t const row = lane_get(wn[0], gb + s0 + (uint32_t)i);
+          uint32_t const row = getl(wn[0], s0 + (uint32_t)i);
           T *dst = (slot != NONE32) ? carry + (size_t)slot * R
                                     : out + (size_t)row * R;
           dst[col] = acc[0];
@@ -396,9 +413,14 @@ void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
   uint32_t const gpw = 64u / R;
   uint32_t const per_block = 4u * gpw;
   dim3 const grid((w->nitems + per_block - 1) / per_block);
-  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB>), grid, dim3(256), 0, s,
-                     w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
-                     (T const *)xin, (T *)out, (T *)carry, R, gpw);
+  if (gpw == 1)
+    hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, true>), grid, dim3(256), 0,
+                       s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+                       (T const *)xin, (T *)out, (T *)carry, R, gpw);
+  else
+    hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false>), grid, dim3(256), 0,
+                       s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+                       (T const *)xin, (T *)out, (T *)carry, R, gpw);
 }
 
 template <typename T, typename TU, bool LEAF, int K>

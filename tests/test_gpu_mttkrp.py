@@ -188,14 +188,14 @@ def test_tiles_and_autotuner():
     assert bt in (0, 4, 16, 4 | A.BATCH_PIPELINED, 8 | A.BATCH_PIPELINED) and ms > 0
 
 
-@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
-def test_every_batch_every_plan(prec):
+@pytest.mark.parametrize("prec,R", [(A.FP64, 16), (A.FP32, 16), (A.FP32, 48)])
+def test_every_batch_every_plan(prec, R):
     """Children per reduction batch (4/8/16, plain or pipelined) change only
     when rows are loaded, never the summation order: every batch value gives
-    the oracle's MTTKRP on every plan, and all agree bit for bit."""
-    print(f"batches prec={prec}", flush=True)
+    the oracle's MTTKRP on every plan, and all agree bit for bit.  R = 48
+    runs the one-group-per-wave kernels (lane values read as scalars)."""
+    print(f"batches prec={prec} R={R}", flush=True)
     t = _tensor([120, 90, 60, 40, 20, 10], 30000, A.DIST_ZIPF, seed=31)
-    R = 16
     U = _factors(t, R, seed=9)
     inds, vals = t.inds().astype(np.int64), t.vals.copy()
     ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
