@@ -239,17 +239,17 @@ __global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ par
 /* Vinv = (hadamard_{k != m} G_k)^-1 by Gauss-Jordan without pivoting (the
  * matrix is symmetric positive definite).  The matrix lives in registers:
  * thread (s, j), s = tid / 64, j = tid % 64, holds column j of rows s, s+4,
- * s+8, ...; each pivot step broadcasts the pivot row and the pivot column
- * through LDS, double-buffered so that one barrier per pivot suffices.
- * R <= 64, one block. */
+ * ..., s+4(Q-1) (R <= 4Q <= 64); each pivot step broadcasts the pivot row and
+ * the pivot column through LDS, double-buffered so that one barrier per
+ * pivot suffices, and updates branch-free.  One block. */
+template <int Q>
 __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
                                                   uint32_t N, uint32_t m,
                                                   uint32_t R,
                                                   double *__restrict__ Vinv,
                                                   int *__restrict__ status)
 {
-  constexpr int Q = 16; /* rows per thread */
-  __shared__ double rowk[2][64], colk[2][64];
+  __shared__ double rowk[2][64], colk[2][4 * Q];
   uint32_t const s = threadIdx.x >> 6, j = threadIdx.x & 63u;
   double a[Q];
 #pragma unroll
@@ -264,31 +264,28 @@ __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
     a[q] = v;
   }
   int bad = 0;
+  /* entries of rows or columns >= R never flow into rows and columns < R */
   for (uint32_t k = 0; k < R; ++k) {
     uint32_t const b = k & 1u;
-    if (j < R) {
-      if ((k & 3u) == s) {
-        double v = 0.0;
+    if ((k & 3u) == s) {
+      double v = a[0];
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          if ((uint32_t)q == (k >> 2)) v = a[q];
-        rowk[b][j] = v;
-      }
-      if (j == k) {
+      for (int q = 1; q < Q; ++q) v = ((uint32_t)q == (k >> 2)) ? a[q] : v;
+      rowk[b][j] = v;
+    }
+    if (j == k) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          if (s + 4u * (uint32_t)q < R) colk[b][s + 4u * (uint32_t)q] = a[q];
-      }
+      for (int q = 0; q < Q; ++q) colk[b][s + 4u * (uint32_t)q] = a[q];
     }
     __syncthreads();
     double const p = rowk[b][k];
-    if (!(p > 0.0)) bad = 1;
+    bad |= !(p > 0.0);
     double const pinv = 1.0 / p;
-    double const rk = ((j == k) ? 1.0 : rowk[b][j < R ? j : 0]) * pinv;
+    double const rk = ((j == k) ? 1.0 : rowk[b][j]) * pinv;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       uint32_t const i = s + 4u * (uint32_t)q;
-      double const f = colk[b][i < R ? i : 0];
+      double const f = colk[b][i];
       a[q] = (i == k) ? rk : ((j == k) ? 0.0 : a[q]) - f * rk;
     }
   }
@@ -568,7 +565,14 @@ int adatm_k_solve_inv(hipStream_t s, double const *G, uint32_t N, uint32_t m,
                       uint32_t R, double *Vinv, int *status)
 {
   if (R > 64) return herr(hipErrorInvalidValue);
-  hipLaunchKernelGGL(k_solve_inv, dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  if (R <= 8)
+    hipLaunchKernelGGL((k_solve_inv<2>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  else if (R <= 16)
+    hipLaunchKernelGGL((k_solve_inv<4>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  else if (R <= 32)
+    hipLaunchKernelGGL((k_solve_inv<8>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  else
+    hipLaunchKernelGGL((k_solve_inv<16>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
   return herr(hipGetLastError());
 }
 

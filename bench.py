@@ -91,6 +91,10 @@ def main() -> int:
                          "(no autotuning)")
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="batched workload: GPU_MAX_HW_QUEUES for this process (hardware "
+                         "queues the instances' graph branches spread over; 0 = the HIP "
+                         "runtime's default, 4), at most 32")
     ap.add_argument("--bf16", action="store_true",
                     help="also sweep with BF16 factor-row storage (FP32 accumulation)")
     ap.add_argument("--copy-cal", type=int, default=0, metavar="BYTES",
@@ -285,6 +289,9 @@ def batched_c5(args) -> int:
     instance, FP32 with FP64 solves, every instance's sweep in one hipGraph
     (one launch per batched sweep), 50 sweeps; against the same engines run
     one instance after another, and FP64 fits of the same instances."""
+    if args.hw_queues:
+        # read by the HIP runtime when it starts: set before the library loads
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.hw_queues, 1), 32))
     import numpy as np
     import adatm_amd as A
 
@@ -332,6 +339,7 @@ def batched_c5(args) -> int:
                                "log-U[1e4,1e6], uniform or Zipf(1.1), R in {8,16,32}, "
                                "analytic plans, autotuned tiles",
                    "nnz_total": nnz_total,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "tiles": [list(x) for x in tiles]},
         "sequential_ms_per_sweep": round(seq_ms, 4),
         "batching_speedup": round(seq_ms / per, 3),
