@@ -36,6 +36,8 @@ OPT_SPACE_LIMIT, OPT_DEVICE, OPT_RANK_HINT, OPT_NOPTIONS = 14, 15, 16, 17
 PLAN_ANALYTIC, PLAN_LEARNED, PLAN_EXHAUSTIVE = 0, 1, 2
 # flag of Engine.set_batch: the next batch's rows load while one folds
 BATCH_PIPELINED = 0x100
+# flag of Engine.set_batch: 16-byte row pieces per lane (128-bit gathers)
+BATCH_WIDE = 0x200
 CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
 ADAPTIVE, ADAPTIVE_TIME_EFFICIENT, ADAPTIVE_SPACE_EFFICIENT = 3, 4, 5
 
@@ -556,7 +558,7 @@ class Engine:
 
     def set_batch(self, batch: int) -> None:
         """children per reduction batch: 0 (default), 4, 8 or 16, optionally
-        | BATCH_PIPELINED"""
+        | BATCH_PIPELINED or | BATCH_WIDE"""
         check(lib().adatm_engine_set_batch(self.h, batch), "engine_set_batch")
 
     def close(self) -> None:

@@ -58,6 +58,8 @@ typedef struct {
   uint32_t K, lt;
   uint32_t rb;          /* children per batch: 4, 8 or 16 (0: 8, or 4 if K > 3) */
   uint32_t db;          /* nonzero: load the next batch while one folds (K <= 3) */
+  uint32_t wide;        /* nonzero: 16-byte row pieces per lane (R/4 or R/2 lanes
+                           per item; R a multiple, >= 8 lanes) */
   uint32_t const *orow; /* output row of each top node (NULL: the node) */
   uint8_t const *cfd;
   uint32_t const *ids[ADATM_KMAXFUSE + 1];

@@ -64,6 +64,36 @@ __device__ __forceinline__ double lane_read(double v, uint32_t l)
   return r;
 }
 
+/* V consecutive row elements at p, loaded as one access of V*sizeof(TU)
+ * bytes (p aligned to that size), converted to the compute type */
+template <typename T, int V>
+__device__ __forceinline__ void ldrow(T const *p, T (&o)[V])
+{
+  if constexpr (V == 1)
+    o[0] = *p;
+  else
+    __builtin_memcpy(o, __builtin_assume_aligned(p, V * sizeof(T)), V * sizeof(T));
+}
+template <int V>
+__device__ __forceinline__ void ldrow(bf16raw const *p, float (&o)[V])
+{
+  uint16_t r[V];
+  if constexpr (V == 1)
+    r[0] = p->v;
+  else
+    __builtin_memcpy(r, __builtin_assume_aligned(p, V * 2), V * 2);
+#pragma unroll
+  for (int v = 0; v < V; ++v) o[v] = __uint_as_float(((uint32_t)r[v]) << 16);
+}
+template <typename T, int V>
+__device__ __forceinline__ void strow(T *p, T const (&o)[V])
+{
+  if constexpr (V == 1)
+    *p = o[0];
+  else
+    __builtin_memcpy(__builtin_assume_aligned(p, V * sizeof(T)), o, V * sizeof(T));
+}
+
 /* FP32 -> BF16 with round-to-nearest-even (NaN/Inf keep their class) */
 __device__ __forceinline__ uint16_t to_bf16(float f)
 {

@@ -49,7 +49,7 @@ namespace {
  * those of the intermediate nodes they belong to -- are issued together,
  * with the per-child metadata read from the holding lane (ds_bpermute; for
  * R > 32, one group per wave, v_readlane into scalars). */
-template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -57,22 +57,25 @@ __global__ __launch_bounds__(256) void k_reduce(
     T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
+  /* a group of G = R/V lanes per item, lane c holding columns [cV, cV+V) */
+  uint32_t const G = R / V;
   uint32_t const lane = threadIdx.x & 63u;
-  /* ONE (R > 32): one group per wave, so the item index, the window and
+  /* ONE (G > 32): one group per wave, so the item index, the window and
      every per-child value read from a lane are wave-uniform (scalar) */
   uint32_t grp, k;
   if constexpr (ONE) {
-    if (lane >= R) return;
+    if (lane >= G) return;
     grp = 0;
     k = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
                                        (threadIdx.x >> 6));
   } else {
-    grp = lane / R;
+    grp = lane / G;
     if (grp >= gpw) return;
     k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   }
-  uint32_t const gb = grp * R; /* the group's first lane */
+  uint32_t const gb = grp * G; /* the group's first lane */
   uint32_t const col = lane - gb;
+  uint32_t const cv = col * V; /* the lane's first column */
   if (k >= nitems) return;
   /* value v of the group's lane idx */
   auto getl = [&](auto v, uint32_t idx) -> decltype(v) {
@@ -90,9 +93,11 @@ __global__ __launch_bounds__(256) void k_reduce(
   node[0] = iseg[k];
 #pragma unroll
   for (int j = 1; j < K; ++j) node[j] = ianc[(size_t)k * (K - 1) + (j - 1)];
-  T acc[K];
+  T acc[K][V];
 #pragma unroll
-  for (int j = 0; j < K; ++j) acc[j] = (T)0;
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[j][v] = (T)0;
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
 
@@ -110,8 +115,8 @@ __global__ __launch_bounds__(256) void k_reduce(
   T wx = (T)0;
   load_win(a, wid, wf, wx);
 
-  for (uint32_t w0 = a; w0 < b; w0 += R) {
-    uint32_t const nw = (b - w0 < R) ? b - w0 : R;
+  for (uint32_t w0 = a; w0 < b; w0 += G) {
+    uint32_t const nw = (b - w0 < G) ? b - w0 : G;
     uint64_t const valid = (nw >= 64u) ? ~0ull : ((1ull << nw) - 1ull);
     uint32_t const me = (col < nw) ? col : nw - 1u;
     uint64_t const below = (1ull << me) - 1ull;
@@ -133,50 +138,57 @@ __global__ __launch_bounds__(256) void k_reduce(
     }
     uint32_t nid = 0, nf = 0;
     T nx = (T)0;
-    if (w0 + R < b) load_win(w0 + R, nid, nf, nx);
+    if (w0 + G < b) load_win(w0 + G, nid, nf, nx);
 
     /* a batch's row loads: its children's factor rows and input values or
        rows, and the factor rows of the level-j nodes holding them */
-    auto load_rows = [&](uint32_t s0, T (&xv)[RB], T (&uv)[RB], T (&uc)[K][RB]) {
+    auto load_rows = [&](uint32_t s0, T (&xv)[RB][V], T (&uv)[RB][V],
+                         T (&uc)[K][RB][V]) {
       uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
         uint32_t const src = s0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1u);
         uint32_t const id = getl(wid, src);
-        uv[i] = (T)ldu(Ub + (id * R + col));
+        ldrow(Ub + (id * R + cv), uv[i]);
         if constexpr (LEAF)
-          xv[i] = getl(wx, src);
+          xv[i][0] = getl(wx, src);
         else
-          xv[i] = xin[(size_t)(w0 + src) * R + col];
+          ldrow(xin + ((size_t)(w0 + src) * R + cv), xv[i]);
 #pragma unroll
         for (int j = 1; j < K; ++j) {
           uint32_t const nid_j = getl(wn[j], src);
-          uc[j][i] = (T)ldu((TU const *)ch.U[j] + (nid_j * R + col));
+          ldrow((TU const *)ch.U[j] + (nid_j * R + cv), uc[j][i]);
         }
       }
     };
-    auto fold = [&](uint32_t s0, T const (&xv)[RB], T const (&uv)[RB],
-                    T const (&uc)[K][RB]) {
+    auto fold = [&](uint32_t s0, T const (&xv)[RB][V], T const (&uv)[RB][V],
+                    T const (&uc)[K][RB][V]) {
       uint32_t const nb = (nw - s0 < (uint32_t)RB) ? nw - s0 : (uint32_t)RB;
       uint32_t cb[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) cb[j] = (uint32_t)(cm[j] >> s0);
       auto step = [&](int i) {
-        acc[K - 1] += xv[i] * uv[i];
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          acc[K - 1][v] += xv[i][LEAF ? 0 : v] * uv[i][v];
 #pragma unroll
         for (int j = K - 1; j >= 1; --j) {
           /* fold the level-j node into its parent with its own factor row */
           bool const c = (cb[j] >> i) & 1u;
-          T const t = acc[j] * uc[j][i];
-          acc[j - 1] = c ? acc[j - 1] + t : acc[j - 1];
-          acc[j] = c ? (T)0 : acc[j];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            T const t = acc[j][v] * uc[j][i][v];
+            acc[j - 1][v] = c ? acc[j - 1][v] + t : acc[j - 1][v];
+            acc[j][v] = c ? (T)0 : acc[j][v];
+          }
         }
         if ((cb[0] >> i) & 1u) {
           uint32_t const row = getl(wn[0], s0 + (uint32_t)i);
           T *dst = (slot != NONE32) ? carry + (size_t)slot * R
                                     : out + (size_t)row * R;
-          dst[col] = acc[0];
-          acc[0] = (T)0;
+          strow<T, V>(dst + cv, acc[0]);
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[0][v] = (T)0;
         }
       };
       if (nb == (uint32_t)RB) {
@@ -189,22 +201,24 @@ __global__ __launch_bounds__(256) void k_reduce(
       }
     };
     /* DB: the next batch's rows are loaded while the current one folds */
-    T xa[RB], ua[RB], ca[K][RB];
+    T xa[RB][V], ua[RB][V], ca[K][RB][V];
     if constexpr (DB) load_rows(0, xa, ua, ca);
     for (uint32_t s0 = 0; s0 < nw; s0 += RB) {
       if constexpr (DB) {
         bool const more = s0 + RB < nw;
-        T xb[RB], ub[RB], cb2[K][RB];
+        T xb[RB][V], ub[RB][V], cb2[K][RB][V];
         if (more) load_rows(s0 + RB, xb, ub, cb2);
         fold(s0, xa, ua, ca);
         if (more) {
 #pragma unroll
-          for (int i = 0; i < RB; ++i) {
-            xa[i] = xb[i];
-            ua[i] = ub[i];
+          for (int i = 0; i < RB; ++i)
 #pragma unroll
-            for (int j = 1; j < K; ++j) ca[j][i] = cb2[j][i];
-          }
+            for (int v = 0; v < V; ++v) {
+              xa[i][v] = xb[i][v];
+              ua[i][v] = ub[i][v];
+#pragma unroll
+              for (int j = 1; j < K; ++j) ca[j][i][v] = cb2[j][i][v];
+            }
         }
       } else {
         load_rows(s0, xa, ua, ca);
@@ -406,53 +420,76 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
   return hipGetLastError();
 }
 
-template <typename T, typename TU, bool LEAF, int K, int RB, bool DB>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, int V>
 void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                   void const *xin, void *out, void *carry, uint32_t R)
 {
-  uint32_t const gpw = 64u / R;
+  uint32_t const G = R / V;
+  uint32_t const gpw = 64u / G;
   uint32_t const per_block = 4u * gpw;
   dim3 const grid((w->nitems + per_block - 1) / per_block);
-  if (gpw == 1)
-    hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, true>), grid, dim3(256), 0,
-                       s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
-                       (T const *)xin, (T *)out, (T *)carry, R, gpw);
-  else
-    hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false>), grid, dim3(256), 0,
-                       s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
-                       (T const *)xin, (T *)out, (T *)carry, R, gpw);
+  if constexpr (V == 1) {
+    if (gpw == 1) {
+      hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, true, 1>), grid, dim3(256),
+                         0, s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+                         (T const *)xin, (T *)out, (T *)carry, R, gpw);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false, V>), grid, dim3(256), 0,
+                     s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+                     (T const *)xin, (T *)out, (T *)carry, R, gpw);
 }
+
+/* columns per lane of the wide layout: 16-byte rows pieces (8 for BF16
+ * storage, V = 4) when R is a multiple and a group keeps >= 8 lanes */
+template <typename T>
+constexpr int wide_v() { return sizeof(T) == 8 ? 2 : 4; }
 
 template <typename T, typename TU, bool LEAF, int K>
 void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                void const *xin, void *out, void *carry, uint32_t R)
 {
-  /* a batch never spans windows: no more children per batch than R */
-  uint32_t const rb = (ch->rb > R) ? (R >= 8 ? 8u : 4u) : ch->rb;
+  constexpr int VW = wide_v<T>();
+  bool const wide = ch->wide && R % VW == 0 && R / VW >= 8;
+  uint32_t const G = wide ? R / VW : R;
+  /* a batch never spans windows: no more children per batch than G */
+  uint32_t const rb = (ch->rb > G) ? (G >= 8 ? 8u : 4u) : ch->rb;
+  if (wide) {
+    if constexpr (K <= 3) {
+      if (rb == 4)
+        go_reduce_rb<T, TU, LEAF, K, 4, false, VW>(s, w, ch, xin, out, carry, R);
+      else
+        go_reduce_rb<T, TU, LEAF, K, 8, false, VW>(s, w, ch, xin, out, carry, R);
+    } else {
+      go_reduce_rb<T, TU, LEAF, K, 4, false, VW>(s, w, ch, xin, out, carry, R);
+    }
+    return;
+  }
   bool const db = ch->db != 0;
   if constexpr (K <= 3) {
     if (db) {
       if (rb == 4)
-        go_reduce_rb<T, TU, LEAF, K, 4, true>(s, w, ch, xin, out, carry, R);
+        go_reduce_rb<T, TU, LEAF, K, 4, true, 1>(s, w, ch, xin, out, carry, R);
       else
-        go_reduce_rb<T, TU, LEAF, K, 8, true>(s, w, ch, xin, out, carry, R);
+        go_reduce_rb<T, TU, LEAF, K, 8, true, 1>(s, w, ch, xin, out, carry, R);
       return;
     }
   }
   if constexpr (K <= 2) {
     if (rb == 16)
-      go_reduce_rb<T, TU, LEAF, K, 16, false>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 16, false, 1>(s, w, ch, xin, out, carry, R);
     else if (rb == 4)
-      go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 4, false, 1>(s, w, ch, xin, out, carry, R);
     else
-      go_reduce_rb<T, TU, LEAF, K, 8, false>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 8, false, 1>(s, w, ch, xin, out, carry, R);
   } else if constexpr (K == 3) {
     if (rb == 4)
-      go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 4, false, 1>(s, w, ch, xin, out, carry, R);
     else
-      go_reduce_rb<T, TU, LEAF, K, 8, false>(s, w, ch, xin, out, carry, R);
+      go_reduce_rb<T, TU, LEAF, K, 8, false, 1>(s, w, ch, xin, out, carry, R);
   } else {
-    go_reduce_rb<T, TU, LEAF, K, 4, false>(s, w, ch, xin, out, carry, R);
+    go_reduce_rb<T, TU, LEAF, K, 4, false, 1>(s, w, ch, xin, out, carry, R);
   }
 }
 

@@ -297,6 +297,7 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
   ch.lt = lt;
   ch.rb = e->batch & 0xffu;
   ch.db = (e->batch & ADATM_BATCH_PIPELINED) ? 1u : 0u;
+  ch.wide = (e->batch & ADATM_BATCH_WIDE) ? 1u : 0u;
   ch.orow = (lt == 0) ? g->orow : NULL;
   ch.cfd = g->cfd[lb];
   for (uint32_t j = 1; j <= ch.K; ++j) {
@@ -1854,7 +1855,7 @@ uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
 
 int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
 {
-  uint32_t const rb = batch & ~ADATM_BATCH_PIPELINED;
+  uint32_t const rb = batch & ~(ADATM_BATCH_PIPELINED | ADATM_BATCH_WIDE);
   if (!e || (rb != 0 && rb != 4 && rb != 8 && rb != 16))
     return SPLATT_ERROR_BADINPUT;
   if (batch != e->batch && e->gvalid) { /* the captured sweep holds the old one */
@@ -1870,8 +1871,8 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
  * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
  * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
  * reduction tile, each a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
- * median); then, on the best engine, the children per batch over {4, 8, 16}
- * and the pipelined 4 and 8. */
+ * median); then, on the best engine, the children per batch over {4, 8, 16},
+ * the pipelined 4 and 8, and the wide 4 and 8. */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, uint32_t *batch, double *best_ms)
@@ -1903,9 +1904,10 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
     int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, br, bs, &e);
     if (rc != SPLATT_SUCCESS) return rc;
     rc = adatm_engine_init_factors(e, 1);
-    static uint32_t const BT[4] = {4, 16, 4 | ADATM_BATCH_PIPELINED,
-                                   8 | ADATM_BATCH_PIPELINED};
-    for (int i = 0; rc == SPLATT_SUCCESS && i < 4; ++i) {
+    static uint32_t const BT[6] = {4, 16, 4 | ADATM_BATCH_PIPELINED,
+                                   8 | ADATM_BATCH_PIPELINED, 4 | ADATM_BATCH_WIDE,
+                                   8 | ADATM_BATCH_WIDE};
+    for (int i = 0; rc == SPLATT_SUCCESS && i < 6; ++i) {
       double ms = INFINITY;
       rc = adatm_engine_set_batch(e, BT[i]);
       if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);

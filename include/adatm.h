@@ -219,6 +219,11 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
  * batch's rows are loaded while one is folded (passes of <= 3 levels, batch
  * 4 or 8).  A launch-time choice: no rebuild. */
 #define ADATM_BATCH_PIPELINED 0x100u
+/* ORed into the batch: each lane holds 16 bytes of a row (4 FP32/BF16 or 2
+ * FP64 columns, so R/4 or R/2 lanes per work item) and gathers rows with
+ * 128-bit loads; applies when R is a multiple and >= 8 lanes remain, with
+ * batch 4 or 8, not pipelined. */
+#define ADATM_BATCH_WIDE 0x200u
 int adatm_engine_set_batch(adatm_engine *e, uint32_t batch);
 /* Autotuner: the tiles and batch of one plan chosen by timing CP-ALS sweeps
  * (nrep, median) of engines built with each candidate; batch (nullable) gets

@@ -185,27 +185,29 @@ def test_tiles_and_autotuner():
         e.close()
     (rt, st, bt), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
     assert rt in (32, 64, 128, 256, 512) and st in (16, 32, 64, 128, 256)
-    assert bt in (0, 4, 16, 4 | A.BATCH_PIPELINED, 8 | A.BATCH_PIPELINED) and ms > 0
+    assert bt & ~(A.BATCH_PIPELINED | A.BATCH_WIDE) in (0, 4, 8, 16) and ms > 0
 
 
-@pytest.mark.parametrize("prec,R", [(A.FP64, 16), (A.FP32, 16), (A.FP32, 48)])
+@pytest.mark.parametrize("prec,R", [(A.FP64, 16), (A.FP32, 16), (A.FP32, 32), (A.FP32, 48)])
 def test_every_batch_every_plan(prec, R):
     """Children per reduction batch (4/8/16, plain or pipelined) change only
     when rows are loaded, never the summation order: every batch value gives
     the oracle's MTTKRP on every plan, and all agree bit for bit.  R = 48
-    runs the one-group-per-wave kernels (lane values read as scalars)."""
+    runs the one-group-per-wave kernels (lane values read as scalars); the
+    wide batches put 16-byte row pieces in each lane (FP64 R = 16: 8 lanes of
+    2 columns; FP32 R = 32 / 48: 8 / 12 lanes of 4 columns)."""
     print(f"batches prec={prec} R={R}", flush=True)
     t = _tensor([120, 90, 60, 40, 20, 10], 30000, A.DIST_ZIPF, seed=31)
     U = _factors(t, R, seed=9)
     inds, vals = t.inds().astype(np.int64), t.vals.copy()
     ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
-    P = A.BATCH_PIPELINED
+    P, W = A.BATCH_PIPELINED, A.BATCH_WIDE
     for plan in A.plan_candidates(t.nmodes, t.dims):
         e = A.Engine(t, R, plan, precision=prec, tiles=(64, 0, 0))
         for m in range(t.nmodes):
             e.set_factor(m, U[m])
         first = None
-        for batch in (0, 4, 8, 16, 4 | P, 8 | P):
+        for batch in (0, 4, 8, 16, 4 | P, 8 | P, 4 | W, 8 | W):
             e.set_batch(batch)
             out = [e.mttkrp(m) for m in range(t.nmodes)]
             for m in range(t.nmodes):
