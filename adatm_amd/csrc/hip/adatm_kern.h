@@ -79,6 +79,7 @@ int adatm_k_reduce(hipStream_t s, int prec, int leaf, adatm_kwork const *w,
  * one value per node (the leaf level: Z = the nonzero values). */
 typedef struct {
   uint32_t d;
+  uint32_t wide; /* nonzero: 16-byte row pieces per lane, as adatm_kchain.wide */
   uint32_t const *node;
   uint32_t const *rid[ADATM_KMAXSCAT];
   void const *U[ADATM_KMAXSCAT];

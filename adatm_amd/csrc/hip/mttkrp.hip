@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_reduce(
  * batch in which no output row ends is summed without per-node checks. */
 #define SB 8
 
-template <typename T, typename TU, int D, bool ZS>
+template <typename T, typename TU, int D, bool ZS, int V>
 __global__ __launch_bounds__(256) void k_scatter(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -245,13 +245,16 @@ __global__ __launch_bounds__(256) void k_scatter(
     T const *__restrict__ Z, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
+  /* a group of G = R/V lanes per item, lane c holding columns [cV, cV+V) */
+  uint32_t const G = R / V;
   uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / R;
+  uint32_t const grp = lane / G;
   if (grp >= gpw) return;
-  uint32_t const col = lane - grp * R;
+  uint32_t const cv = (lane - grp * G) * V;
   uint32_t const k =
       (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (k >= nitems) return;
+  T const zero[V] = {};
   uint32_t const a = istart[k], b = istart[k + 1];
   uint32_t const slot = islot[k];
   bool const heavy = slot != NONE32;
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void k_scatter(
   if (!heavy) {
     e = rowptr[p + 1];
     while (e == a) { /* leading empty rows */
-      out[(size_t)p * R + col] = (T)0;
+      strow<T, V>(out + ((size_t)p * R + cv), zero);
       if (++p == pe) return;
       e = rowptr[p + 1];
     }
@@ -279,31 +282,50 @@ __global__ __launch_bounds__(256) void k_scatter(
     }
   };
   load_idx(a, nd, rr);
-  T acc = (T)0;
+  T acc[V];
+#pragma unroll
+  for (int c = 0; c < V; ++c) acc[c] = (T)0;
   for (uint32_t c0 = a; c0 < b; c0 += SB) {
     uint32_t const nb = (b - c0 < (uint32_t)SB) ? b - c0 : (uint32_t)SB;
-    T v[SB];
+    T v[SB][V];
 #pragma unroll
     for (int i = 0; i < SB; ++i) {
-      T y = (T)ldu((TU const *)sc.U[0] + (rr[0][i] * R + col));
+      T y[V], u[V], z[V];
+      ldrow((TU const *)sc.U[0] + (rr[0][i] * R + cv), y);
 #pragma unroll
-      for (int l = 1; l < D; ++l)
-        y = y * (T)ldu((TU const *)sc.U[l] + (rr[l][i] * R + col));
-      v[i] = (ZS ? Z[nd[i]] : Z[(size_t)nd[i] * R + col]) * y;
+      for (int l = 1; l < D; ++l) {
+        ldrow((TU const *)sc.U[l] + (rr[l][i] * R + cv), u);
+#pragma unroll
+        for (int c = 0; c < V; ++c) y[c] = y[c] * u[c];
+      }
+      if constexpr (ZS) {
+        T const zs = Z[nd[i]];
+#pragma unroll
+        for (int c = 0; c < V; ++c) v[i][c] = zs * y[c];
+      } else {
+        ldrow(Z + ((size_t)nd[i] * R + cv), z);
+#pragma unroll
+        for (int c = 0; c < V; ++c) v[i][c] = z[c] * y[c];
+      }
     }
     uint32_t const c1 = c0 + SB;
     if (c1 < b) load_idx(c1, nd, rr);
     if (nb == (uint32_t)SB && (heavy || e >= c0 + SB)) {
       /* no row ends before the batch's last node (a piece has no row end) */
-      T s01 = v[0] + v[1], s23 = v[2] + v[3], s45 = v[4] + v[5], s67 = v[6] + v[7];
-      acc += (s01 + s23) + (s45 + s67);
+#pragma unroll
+      for (int c = 0; c < V; ++c) {
+        T const s01 = v[0][c] + v[1][c], s23 = v[2][c] + v[3][c];
+        T const s45 = v[4][c] + v[5][c], s67 = v[6][c] + v[7][c];
+        acc[c] += (s01 + s23) + (s45 + s67);
+      }
       if (!heavy && e == c0 + SB) { /* the row ends with the batch */
-        out[(size_t)p * R + col] = acc;
-        acc = (T)0;
+        strow<T, V>(out + ((size_t)p * R + cv), acc);
+#pragma unroll
+        for (int c = 0; c < V; ++c) acc[c] = (T)0;
         while (++p < pe) {
           e = rowptr[p + 1];
           if (e != c0 + SB) break;
-          out[(size_t)p * R + col] = (T)0;
+          strow<T, V>(out + ((size_t)p * R + cv), zero);
         }
       }
     } else {
@@ -311,22 +333,24 @@ __global__ __launch_bounds__(256) void k_scatter(
       for (int i = 0; i < SB; ++i) {
         uint32_t const c = c0 + i;
         if ((uint32_t)i < nb) {
-          acc += v[i];
+#pragma unroll
+          for (int q = 0; q < V; ++q) acc[q] += v[i][q];
           if (!heavy && c + 1 == e) {
-            out[(size_t)p * R + col] = acc;
-            acc = (T)0;
+            strow<T, V>(out + ((size_t)p * R + cv), acc);
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[q] = (T)0;
             /* next non-empty row; empty rows in between get zeros */
             while (++p < pe) {
               e = rowptr[p + 1];
               if (e != c + 1) break;
-              out[(size_t)p * R + col] = (T)0;
+              strow<T, V>(out + ((size_t)p * R + cv), zero);
             }
           }
         }
       }
     }
   }
-  if (heavy) carry[(size_t)slot * R + col] = acc;
+  if (heavy) strow<T, V>(carry + ((size_t)slot * R + cv), acc);
 }
 
 /* Chunk c of split parent h = cseg[c]: the sum of its <= ADATM_FIX_CHUNK
@@ -513,22 +537,34 @@ hipError_t reduce_k(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
   return hipSuccess;
 }
 
+template <typename T, typename TU, int D, int V>
+void go_scatter_v(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
+                  adatm_kscat const *sc, void const *Z, int zs, void *out,
+                  void *carry, uint32_t R)
+{
+  uint32_t const gpw = 64u / (R / V);
+  uint32_t const per_block = 4u * gpw;
+  dim3 const grid((w->nitems + per_block - 1) / per_block);
+  if (zs)
+    hipLaunchKernelGGL((k_scatter<T, TU, D, true, V>), grid, dim3(256), 0, s,
+                       w->nitems, w->start, w->seg, w->slot, rowptr, *sc,
+                       (T const *)Z, (T *)out, (T *)carry, R, gpw);
+  else
+    hipLaunchKernelGGL((k_scatter<T, TU, D, false, V>), grid, dim3(256), 0, s,
+                       w->nitems, w->start, w->seg, w->slot, rowptr, *sc,
+                       (T const *)Z, (T *)out, (T *)carry, R, gpw);
+}
+
 template <typename T, typename TU, int D>
 void go_scatter(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr,
                 adatm_kscat const *sc, void const *Z, int zs, void *out,
                 void *carry, uint32_t R)
 {
-  uint32_t const gpw = 64u / R;
-  uint32_t const per_block = 4u * gpw;
-  dim3 const grid((w->nitems + per_block - 1) / per_block);
-  if (zs)
-    hipLaunchKernelGGL((k_scatter<T, TU, D, true>), grid, dim3(256), 0, s,
-                       w->nitems, w->start, w->seg, w->slot, rowptr, *sc,
-                       (T const *)Z, (T *)out, (T *)carry, R, gpw);
+  constexpr int VW = wide_v<T>();
+  if (sc->wide && R % VW == 0 && R / VW >= 8)
+    go_scatter_v<T, TU, D, VW>(s, w, rowptr, sc, Z, zs, out, carry, R);
   else
-    hipLaunchKernelGGL((k_scatter<T, TU, D, false>), grid, dim3(256), 0, s, w->nitems,
-                     w->start, w->seg, w->slot, rowptr, *sc, (T const *)Z,
-                     (T *)out, (T *)carry, R, gpw);
+    go_scatter_v<T, TU, D, 1>(s, w, rowptr, sc, Z, zs, out, carry, R);
 }
 
 template <typename T, typename TU>

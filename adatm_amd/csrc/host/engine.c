@@ -384,6 +384,7 @@ static int mttkrp_level(adatm_engine *e, dgroup *g, uint32_t d, int tag)
   adatm_kscat sc;
   memset(&sc, 0, sizeof(sc));
   sc.d = d;
+  sc.wide = (e->batch & ADATM_BATCH_WIDE) ? 1u : 0u;
   sc.node = S->node;
   for (uint32_t l = 0; l < d; ++l) {
     sc.rid[l] = S->rid[l];
