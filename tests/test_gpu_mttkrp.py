@@ -290,3 +290,47 @@ def test_planner_device_equals_host():
     a, ca = A.plan_analytic(t, 32, device=0)
     b, cb = A.plan_analytic(t, 32, device=-1)
     assert ca == cb and [(p.pspace, p.pops) for p in a] == [(p.pspace, p.pops) for p in b]
+
+
+def _degenerate_cases():
+    rng = np.random.default_rng(77)
+    out = []
+    # one nonzero (every mode compacts to one index)
+    out.append(("one nonzero", np.array([[3], [2], [1], [0]]), np.array([1.5]), [5, 4, 3, 2], 8))
+    # one fiber: a single (i0, i1, i2) path with 3000 leaves (pieces split it)
+    n = 3000
+    out.append(("one fiber", np.stack([np.full(n, 1), np.full(n, 2), np.full(n, 0), np.arange(n)]),
+                rng.standard_normal(n), [3, 4, 2, n], 16))
+    # one root slice holding every nonzero
+    n = 6000
+    out.append(("one root slice", np.stack([np.zeros(n, dtype=np.int64), rng.integers(0, 40, n),
+                                            rng.integers(0, 30, n), rng.integers(0, 20, n)]),
+                rng.standard_normal(n), [1, 40, 30, 20], 32))
+    # rank one, duplicates merged by canonicalisation
+    n = 4000
+    idx = np.stack([rng.integers(0, 15, n), rng.integers(0, 12, n), rng.integers(0, 10, n)])
+    out.append(("rank one, duplicates", idx, rng.standard_normal(n), [15, 12, 10], 1))
+    return out
+
+
+@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+def test_degenerate_tensors(case, prec):
+    """Edge shapes: a single nonzero, a single fiber split across pieces, a
+    single root slice, rank one with duplicate coordinates: every plan and
+    mode against the oracle's COO MTTKRP of the canonical tensor."""
+    name, inds, vals, dims, R = _degenerate_cases()[case]
+    print(f"degenerate {name} prec={prec}", flush=True)
+    t = A.Tensor.from_arrays(inds.astype(np.uint32), vals.astype(np.float64), dims)
+    t.canonicalize()
+    U = _factors(t, R, seed=3)
+    ci, cv = t.inds().astype(np.int64), t.vals.copy()
+    ref = [O.coo_mttkrp(ci, cv, U, m) for m in range(t.nmodes)]
+    for plan in A.plan_candidates(t.nmodes, t.dims):
+        e = A.Engine(t, R, plan, precision=prec, tiles=(32, 16, 0))
+        for m in range(t.nmodes):
+            e.set_factor(m, U[m])
+        for m in range(t.nmodes):
+            err = O.rel_fro(e.mttkrp(m), ref[m])
+            assert err <= TOL[prec], (name, plan.groups(t.nmodes), m, err)
+        e.close()
