@@ -1,10 +1,12 @@
 /*
  * Memoized-MTTKRP kernels for MI355X (gfx950, wave64).
  *
- * A lane group of G = R lanes (R <= 64; 64/R groups per wavefront) owns one
- * work item and lane j holds column j of every R-wide row, so each factor-row
+ * A lane group of G lanes owns one work item; lane j holds V consecutive
+ * columns of every R-wide row (G = R/V: V = 1, or in the wide layout 16
+ * bytes per lane -- 4 FP32/BF16 or 2 FP64 columns), so each factor-row
  * gather and each memo-row access of a group is one contiguous R*sizeof(T)
- * transaction.  Work items are balanced by the number of children they walk;
+ * transaction, made of 128-bit lane accesses in the wide layout.  R <= 64;
+ * 64/G groups per wavefront.  Work items are balanced by the number of children they walk;
  * a parent longer than one item is cut into pieces whose partial rows go to
  * carry slots, summed in a fixed order by k_fix_chunk / k_fix_multi, so results are
  * deterministic and no output needs zero-filling or atomics.
