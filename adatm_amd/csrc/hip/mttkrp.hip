@@ -477,7 +477,10 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                void const *xin, void *out, void *carry, uint32_t R)
 {
   constexpr int VW = wide_v<T>();
-  bool const wide = ch->wide && R % VW == 0 && R / VW >= 8;
+  /* the default batch (rb 0) runs leaf passes in the wide layout: their
+     factor-row gathers are what the wide loads speed up */
+  bool const wide = (ch->wide || (ch->rb == 0 && !ch->db && LEAF)) && R % VW == 0 &&
+                    R / VW >= 8;
   uint32_t const G = wide ? R / VW : R;
   /* a batch never spans windows: no more children per batch than G */
   uint32_t const rb = (ch->rb > G) ? (G >= 8 ? 8u : 4u) : ch->rb;

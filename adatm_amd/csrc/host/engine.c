@@ -1872,8 +1872,9 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
  * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
  * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
  * reduction tile, each a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
- * median); then, on the best engine, the children per batch over {4, 8, 16},
- * the pipelined 4 and 8, and the wide 4 and 8. */
+ * median); then, on the best engine, the default batch against the
+ * children per batch over {4, 8, 16}, the pipelined 4 and 8 and the wide 4
+ * and 8. */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, uint32_t *batch, double *best_ms)
@@ -1905,10 +1906,10 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
     int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, br, bs, &e);
     if (rc != SPLATT_SUCCESS) return rc;
     rc = adatm_engine_init_factors(e, 1);
-    static uint32_t const BT[6] = {4, 16, 4 | ADATM_BATCH_PIPELINED,
+    static uint32_t const BT[7] = {4, 8, 16, 4 | ADATM_BATCH_PIPELINED,
                                    8 | ADATM_BATCH_PIPELINED, 4 | ADATM_BATCH_WIDE,
                                    8 | ADATM_BATCH_WIDE};
-    for (int i = 0; rc == SPLATT_SUCCESS && i < 6; ++i) {
+    for (int i = 0; rc == SPLATT_SUCCESS && i < 7; ++i) {
       double ms = INFINITY;
       rc = adatm_engine_set_batch(e, BT[i]);
       if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);

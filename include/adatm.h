@@ -214,8 +214,10 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
                               uint32_t red_tile, uint32_t scat_tile,
                               adatm_engine **out);
 /* Children per batch of the fused reductions (4, 8, 16; 0 = default 8, or 4
- * for passes fusing more than 3 levels; 16 applies to passes of <= 2 levels,
- * 8 at most to 3), optionally ORed with ADATM_BATCH_PIPELINED: the next
+ * for passes fusing more than 3 levels, with leaf passes in the wide layout
+ * below where it applies; 16 applies to passes of <= 2 levels, 8 at most to
+ * 3; explicit values keep every pass in the narrow layout unless ORed with
+ * ADATM_BATCH_WIDE), optionally ORed with ADATM_BATCH_PIPELINED: the next
  * batch's rows are loaded while one is folded (passes of <= 3 levels, batch
  * 4 or 8).  A launch-time choice: no rebuild. */
 #define ADATM_BATCH_PIPELINED 0x100u
