@@ -1,0 +1,38 @@
+"""The bench contract pieces that need no GPU: the multi-GPU refusal, the
+traffic summary the roofline reads, and the kernel names it is keyed by."""
+import json
+import os
+import subprocess
+import sys
+
+import bench
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_template_names_match_the_profiled_instantiations():
+    assert bench.template_name(0, 2, 5) == "k_reduce<float, float, true, 2>"
+    assert bench.template_name(1, 1, 5) == "k_reduce<float, float, false, 1>"
+    assert bench.template_name(2, 2, 5) == "k_scatter<float, float, 2, false>"
+    assert bench.template_name(2, 4, 5) == "k_scatter<float, float, 4, true>"
+
+
+def test_committed_traffic_covers_the_c2_passes():
+    line = [l for l in open(os.path.join(ROOT, "profiles", "c2_hbm_traffic.md"))
+            if l.startswith("TRAFFIC_JSON ")]
+    assert len(line) == 1
+    t = json.loads(line[0][len("TRAFFIC_JSON "):])
+    for k in ("k_reduce<float, float, true, 2>", "k_reduce<float, float, true, 3>",
+              "k_reduce<float, float, false, 1>", "k_scatter<float, float, 2, false>",
+              "k_scatter<float, float, 1, false>"):
+        assert k in t and t[k]["fetch_bytes"] > 0 and t[k]["write_bytes"] > 0
+        assert bench.measured_traffic(2, k) == t[k]["fetch_bytes"] + t[k]["write_bytes"]
+    # the memo level written by the K=2 leaf pass (56M rows x 128 B) shows up
+    assert t["k_reduce<float, float, true, 2>"]["write_bytes"] > 7.0e9
