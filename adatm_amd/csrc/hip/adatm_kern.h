@@ -122,6 +122,32 @@ int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
                 uint64_t I_last, double const *lambda, double const *G,
                 uint32_t N, uint32_t R, double ttnormsq, double *part,
                 double *fits, uint32_t *iter);
+/* The dense update of many CP-ALS instances at once (one per descriptor,
+ * each at its own mode m): solve, A = M Vinv with column statistics, lambda,
+ * normalisation and Gram, with every instance's arithmetic, block split and
+ * summation order those of adatm_k_solve_inv + adatm_k_update on it alone.
+ * Descriptors and block maps live in device memory; bm_* map every block of
+ * a grouped grid to its descriptor (adatm_k_dense_blocks gives the counts). */
+typedef struct {
+  uint32_t N, m, R, pad;
+  uint64_t I;
+  double const *G;       /* N grams R x R */
+  double *Vinv;
+  int *status;
+  void const *M;
+  void *A, *Abf;
+  uint32_t const *iter;
+  double *lambda, *part, *G_out;
+  uint32_t mm0, nmm;     /* first block and block count of the product */
+  uint32_t gr0, ngr;     /* ... of the Gram */
+  uint32_t sp0, nsp;     /* ... of the Gram's partial sums */
+} adatm_kdense;
+void adatm_k_dense_blocks(uint64_t I, uint32_t R, uint32_t *nmm, uint32_t *ngr,
+                          uint32_t *nsp);
+int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uint32_t n,
+                          uint32_t Rmax, uint32_t const *bm_mm, uint32_t nb_mm,
+                          uint32_t const *bm_gr, uint32_t nb_gr, uint32_t const *bm_sp,
+                          uint32_t nb_sp);
 /* FP64 host matrix <-> device factor in the path's precision */
 int adatm_k_convert_in(hipStream_t s, int prec, double const *src64,
                        void *dst, void *dst_bf, uint64_t n);

@@ -57,18 +57,18 @@ __global__ void k_convert_out(T const *__restrict__ src, double *__restrict__ ds
 /* partial Gram of rows [lo, hi); optional column scaling 1/lambda written
  * back to A (and its bf16 copy) before accumulation */
 template <typename T>
-__global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
+__device__ __forceinline__ void gram_part_body(uint32_t bid, uint32_t nblk, T *__restrict__ A,
                                                   bf16raw *__restrict__ Abf,
                                                   uint64_t I, uint32_t R,
                                                   double const *__restrict__ lambda,
                                                   double *__restrict__ part)
 {
   HIP_DYNAMIC_SHARED(double, sh) /* GROWS * R */
-  uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
-  uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+  uint64_t const chunk = (I + nblk - 1) / nblk;
+  uint64_t const lo = (uint64_t)bid * chunk;
   uint64_t const hi = lo + chunk < I ? lo + chunk : I;
   uint32_t const RR = R * R;
-  double *myp = part + (size_t)blockIdx.x * RR;
+  double *myp = part + (size_t)bid * RR;
   for (uint32_t e0 = 0; e0 < RR; e0 += DB * 16) {
     double acc[16];
 #pragma unroll
@@ -115,6 +115,16 @@ __global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
+                                                  bf16raw *__restrict__ Abf,
+                                                  uint64_t I, uint32_t R,
+                                                  double const *__restrict__ lambda,
+                                                  double *__restrict__ part)
+{
+  gram_part_body<T>(blockIdx.x, gridDim.x, A, Abf, I, R, lambda, part);
+}
+
 /* Gram of FP32 rows on the matrix cores: v_mfma_f32_32x32x2f32 takes two
  * rows of A per instruction.  With a = b = A[r0 + lane/32][c0 + lane%32],
  * the instruction's A operand is A^T (32 x 2, element [m][k] from lane
@@ -128,7 +138,7 @@ __global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
  * back (and to the bf16 copy), as k_gram_part does. */
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-__global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
+__device__ __forceinline__ void gram_mfma_body(uint32_t bid, uint32_t nblk, float *__restrict__ A,
                                                   bf16raw *__restrict__ Abf,
                                                   uint64_t I, uint32_t R,
                                                   double const *__restrict__ lambda,
@@ -139,8 +149,8 @@ __global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
   uint32_t const w = threadIdx.x >> 6;
   uint32_t const c0 = lane & 31u, c1 = 32u + (lane & 31u);
   bool const two = R > 32;
-  uint64_t const chunk = ((I + gridDim.x - 1) / gridDim.x + 7) & ~(uint64_t)7;
-  uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+  uint64_t const chunk = ((I + nblk - 1) / nblk + 7) & ~(uint64_t)7;
+  uint64_t const lo = (uint64_t)bid * chunk;
   uint64_t const hi = lo + chunk < I ? lo + chunk : I;
   double const l0 = (lambda && c0 < R) ? lambda[c0] : 1.0;
   double const l1 = (lambda && two && c1 < R) ? lambda[c1] : 1.0;
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
       a01[i] += red[ww][1][i * 64 + lane];
       a11[i] += red[ww][2][i * 64 + lane];
     }
-  double *myp = part + (size_t)blockIdx.x * R * R;
+  double *myp = part + (size_t)bid * R * R;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     uint32_t const row = (uint32_t)(i % 4) + 4u * (lane >> 5) + 8u * (uint32_t)(i / 4);
@@ -213,16 +223,25 @@ __global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
   }
 }
 
+__global__ __launch_bounds__(DB) void k_gram_mfma(float *__restrict__ A,
+                                                  bf16raw *__restrict__ Abf,
+                                                  uint64_t I, uint32_t R,
+                                                  double const *__restrict__ lambda,
+                                                  double *__restrict__ part)
+{
+  gram_mfma_body(blockIdx.x, gridDim.x, A, Abf, I, R, lambda, part);
+}
+
 /* out[e] = sum over the nblk partial rows of part[b*len + e] in a fixed
  * order: 8 interleaved chunk sums per element (chunk c takes rows c, c+8,
  * ...), combined in chunk order.  Block: 32 elements x 8 chunks. */
-__global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ part,
+__device__ __forceinline__ void sum_parts_body(uint32_t bid, double const *__restrict__ part,
                                                   uint32_t nblk, uint32_t len,
                                                   double *__restrict__ out)
 {
   __shared__ double red[DB];
   uint32_t const c = threadIdx.x >> 5, el = threadIdx.x & 31u;
-  uint32_t const e = blockIdx.x * 32u + el;
+  uint32_t const e = bid * 32u + el;
   double s = 0.0;
   if (e < len)
     for (uint32_t b = c; b < nblk; b += 8) s += part[(size_t)b * len + e];
@@ -236,6 +255,13 @@ __global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ par
   }
 }
 
+__global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ part,
+                                                  uint32_t nblk, uint32_t len,
+                                                  double *__restrict__ out)
+{
+  sum_parts_body(blockIdx.x, part, nblk, len, out);
+}
+
 /* Vinv = (hadamard_{k != m} G_k)^-1 by Gauss-Jordan without pivoting (the
  * matrix is symmetric positive definite).  The matrix lives in registers:
  * thread (s, j), s = tid / 64, j = tid % 64, holds column j of rows s, s+4,
@@ -243,7 +269,7 @@ __global__ __launch_bounds__(DB) void k_sum_parts(double const *__restrict__ par
  * the pivot column through LDS, double-buffered so that one barrier per
  * pivot suffices, and updates branch-free.  One block. */
 template <int Q>
-__global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
+__device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
                                                   uint32_t N, uint32_t m,
                                                   uint32_t R,
                                                   double *__restrict__ Vinv,
@@ -297,9 +323,19 @@ __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
   if (threadIdx.x == 0 && status) *status = bad;
 }
 
+template <int Q>
+__global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
+                                                  uint32_t N, uint32_t m,
+                                                  uint32_t R,
+                                                  double *__restrict__ Vinv,
+                                                  int *__restrict__ status)
+{
+  solve_inv_body<Q>(G, N, m, R, Vinv, status);
+}
+
 /* A = M * Vinv (FP64 accumulation), per-block column sum of squares / max */
 template <typename T>
-__global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
+__device__ __forceinline__ void matmul_stats_body(uint32_t bid, uint32_t nblk, T const *__restrict__ M,
                                                      double const *__restrict__ Vinv,
                                                      T *__restrict__ A, uint64_t I,
                                                      uint32_t R,
@@ -313,8 +349,8 @@ __global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
   uint32_t const rr = threadIdx.x / R, cc = threadIdx.x - rr * R;
   double ssq = 0.0, mx = -1.0e300;
   if (rr < rpi) {
-    uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
-    uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+    uint64_t const chunk = (I + nblk - 1) / nblk;
+    uint64_t const lo = (uint64_t)bid * chunk;
     uint64_t const hi = lo + chunk < I ? lo + chunk : I;
     for (uint64_t i = lo + rr; i < hi; i += rpi) {
       T const *mrow = M + (size_t)i * R;
@@ -337,15 +373,25 @@ __global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
       double const v = red[DB + t * R + threadIdx.x];
       m = v > m ? v : m;
     }
-    part[(size_t)blockIdx.x * 2 * R + threadIdx.x] = s;
-    part[(size_t)blockIdx.x * 2 * R + R + threadIdx.x] = m;
+    part[(size_t)bid * 2 * R + threadIdx.x] = s;
+    part[(size_t)bid * 2 * R + R + threadIdx.x] = m;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void k_matmul_stats(T const *__restrict__ M,
+                                                     double const *__restrict__ Vinv,
+                                                     T *__restrict__ A, uint64_t I,
+                                                     uint32_t R,
+                                                     double *__restrict__ part)
+{
+  matmul_stats_body<T>(blockIdx.x, gridDim.x, M, Vinv, A, I, R, part);
 }
 
 /* lambda from the column statistics: 2-norm (first sweep) or max(1, max).
  * Block: R columns x C = DB/R chunks of the partial rows (chunk c takes rows
  * c, c+C, ...), combined in chunk order (R <= 64). */
-__global__ __launch_bounds__(DB) void k_lambda(double const *__restrict__ part,
+__device__ __forceinline__ void lambda_body(double const *__restrict__ part,
                                                uint32_t nblk, uint32_t R,
                                                uint32_t const *__restrict__ iter,
                                                double *__restrict__ lambda)
@@ -372,6 +418,14 @@ __global__ __launch_bounds__(DB) void k_lambda(double const *__restrict__ part,
     int const first = (*iter == 0u);
     lambda[r] = first ? sqrt(s) : (m > 1.0 ? m : 1.0);
   }
+}
+
+__global__ __launch_bounds__(DB) void k_lambda(double const *__restrict__ part,
+                                               uint32_t nblk, uint32_t R,
+                                               uint32_t const *__restrict__ iter,
+                                               double *__restrict__ lambda)
+{
+  lambda_body(part, nblk, R, iter, lambda);
 }
 
 /* per-block partial of sum_i A[i,r] * M[i,r] */
@@ -505,6 +559,53 @@ hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
   return hipGetLastError();
 }
 
+/* ---- grouped dense updates (one descriptor per instance) ----------------- */
+template <int Q>
+__global__ __launch_bounds__(DB) void g_solve(adatm_kdense const *__restrict__ d)
+{
+  adatm_kdense const &x = d[blockIdx.x];
+  solve_inv_body<Q>(x.G, x.N, x.m, x.R, x.Vinv, x.status);
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void g_matmul(adatm_kdense const *__restrict__ d,
+                                               uint32_t const *__restrict__ bm)
+{
+  adatm_kdense const &x = d[bm[blockIdx.x]];
+  matmul_stats_body<T>(blockIdx.x - x.mm0, x.nmm, (T const *)x.M, x.Vinv, (T *)x.A, x.I,
+                       x.R, x.part);
+}
+
+__global__ __launch_bounds__(DB) void g_lambda(adatm_kdense const *__restrict__ d)
+{
+  adatm_kdense const &x = d[blockIdx.x];
+  lambda_body(x.part, x.nmm, x.R, x.iter, x.lambda);
+}
+
+__global__ __launch_bounds__(DB) void g_gram_mfma(adatm_kdense const *__restrict__ d,
+                                                  uint32_t const *__restrict__ bm)
+{
+  adatm_kdense const &x = d[bm[blockIdx.x]];
+  gram_mfma_body(blockIdx.x - x.gr0, x.ngr, (float *)x.A, (bf16raw *)x.Abf, x.I, x.R,
+                 x.lambda, x.part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void g_gram_part(adatm_kdense const *__restrict__ d,
+                                                  uint32_t const *__restrict__ bm)
+{
+  adatm_kdense const &x = d[bm[blockIdx.x]];
+  gram_part_body<T>(blockIdx.x - x.gr0, x.ngr, (T *)x.A, nullptr, x.I, x.R, x.lambda,
+                    x.part);
+}
+
+__global__ __launch_bounds__(DB) void g_sum(adatm_kdense const *__restrict__ d,
+                                            uint32_t const *__restrict__ bm)
+{
+  adatm_kdense const &x = d[bm[blockIdx.x]];
+  sum_parts_body(blockIdx.x - x.sp0, x.part, x.ngr, x.R * x.R, x.G_out);
+}
+
 } // namespace
 
 /* ======================================================================== */
@@ -573,6 +674,50 @@ int adatm_k_solve_inv(hipStream_t s, double const *G, uint32_t N, uint32_t m,
     hipLaunchKernelGGL((k_solve_inv<8>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
   else
     hipLaunchKernelGGL((k_solve_inv<16>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  return herr(hipGetLastError());
+}
+
+void adatm_k_dense_blocks(uint64_t I, uint32_t R, uint32_t *nmm, uint32_t *ngr,
+                          uint32_t *nsp)
+{
+  *nmm = rows_blocks(I, DB / R);
+  *ngr = rows_blocks(I, 32);
+  *nsp = (R * R + 31) / 32;
+}
+
+int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uint32_t n,
+                          uint32_t Rmax, uint32_t const *bm_mm, uint32_t nb_mm,
+                          uint32_t const *bm_gr, uint32_t nb_gr, uint32_t const *bm_sp,
+                          uint32_t nb_sp)
+{
+  if (n == 0) return 0;
+  if (Rmax > 64) return herr(hipErrorInvalidValue);
+  if (Rmax <= 8)
+    hipLaunchKernelGGL((g_solve<2>), dim3(n), dim3(DB), 0, s, desc);
+  else if (Rmax <= 16)
+    hipLaunchKernelGGL((g_solve<4>), dim3(n), dim3(DB), 0, s, desc);
+  else if (Rmax <= 32)
+    hipLaunchKernelGGL((g_solve<8>), dim3(n), dim3(DB), 0, s, desc);
+  else
+    hipLaunchKernelGGL((g_solve<16>), dim3(n), dim3(DB), 0, s, desc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  size_t const lds = ((size_t)Rmax * Rmax + 2 * DB) * sizeof(double);
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((g_matmul<double>), dim3(nb_mm), dim3(DB), lds, s, desc, bm_mm);
+  else
+    hipLaunchKernelGGL((g_matmul<float>), dim3(nb_mm), dim3(DB), lds, s, desc, bm_mm);
+  if ((e = hipGetLastError()) != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(g_lambda, dim3(n), dim3(DB), 0, s, desc);
+  if ((e = hipGetLastError()) != hipSuccess) return herr(e);
+  if (prec == KP_FP64) {
+    size_t const rows = (size_t)GROWS * Rmax * sizeof(double);
+    hipLaunchKernelGGL((g_gram_part<double>), dim3(nb_gr), dim3(DB), rows, s, desc, bm_gr);
+  } else {
+    hipLaunchKernelGGL(g_gram_mfma, dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(g_sum, dim3(nb_sp), dim3(DB), 0, s, desc, bm_sp);
   return herr(hipGetLastError());
 }
 

@@ -1934,6 +1934,14 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
  * forked from and joined to the batch stream inside the capture, so the
  * graph's branches are independent and small instances run side by side.
  * ======================================================================== */
+/* one round of the grouped sweep: every instance with more than r modes runs
+ * its mode-r MTTKRP on its own stream, then one grouped dense update */
+typedef struct {
+  uint32_t n, desc0, Rmax;            /* instances, first descriptor, largest R */
+  uint32_t nb_mm, nb_gr, nb_sp;       /* grouped grids */
+  uint32_t mm_off, gr_off, sp_off;    /* their block maps in dmap */
+} batch_round;
+
 struct adatm_batch {
   int n, device;
   adatm_engine **e;
@@ -1942,6 +1950,12 @@ struct adatm_batch {
   hipGraph_t graph;
   hipGraphExec_t gexec;
   int gvalid;
+  /* grouped dense updates (all engines of one precision) */
+  int grouped;
+  uint32_t nround;
+  batch_round *round;
+  adatm_kdense *ddesc; /* device descriptors of every round */
+  uint32_t *dmap;      /* device block maps of every round */
 };
 
 void adatm_batch_destroy(adatm_batch *b)
@@ -1956,9 +1970,130 @@ void adatm_batch_destroy(adatm_batch *b)
       if (b->join[i]) hipEventDestroy(b->join[i]);
   if (b->fork) hipEventDestroy(b->fork);
   if (b->s) hipStreamDestroy(b->s);
+  if (b->ddesc) hipFree(b->ddesc);
+  if (b->dmap) hipFree(b->dmap);
+  free(b->round);
   free(b->join);
   free(b->e);
   free(b);
+}
+
+/* Descriptors and block maps of the grouped sweep, uploaded once: round r
+ * holds every instance with more than r modes at mode r, with the block
+ * counts of its own single-instance launches. */
+static int batch_rounds(adatm_batch *b)
+{
+  uint32_t nround = 0, ndesc = 0;
+  for (int i = 0; i < b->n; ++i)
+    if (b->e[i]->N > nround) nround = b->e[i]->N;
+  for (int i = 0; i < b->n; ++i) ndesc += b->e[i]->N;
+  b->round = (batch_round *)calloc(nround, sizeof(batch_round));
+  adatm_kdense *hd = (adatm_kdense *)calloc(ndesc, sizeof(adatm_kdense));
+  uint64_t nmap = 0;
+  for (int i = 0; i < b->n; ++i)
+    for (uint32_t m = 0; m < b->e[i]->N; ++m) {
+      uint32_t a, g, p;
+      adatm_k_dense_blocks(b->e[i]->dims[m], b->e[i]->R, &a, &g, &p);
+      nmap += (uint64_t)a + g + p;
+    }
+  uint32_t *hm = (uint32_t *)malloc((size_t)nmap * sizeof(uint32_t));
+  if (!b->round || !hd || !hm) {
+    free(hd);
+    free(hm);
+    return SPLATT_ERROR_NOMEMORY;
+  }
+  uint32_t nd = 0, nm = 0;
+  for (uint32_t r = 0; r < nround; ++r) {
+    batch_round *B = b->round + r;
+    B->desc0 = nd;
+    for (int i = 0; i < b->n; ++i) {
+      adatm_engine *e = b->e[i];
+      if (e->N <= r) continue;
+      adatm_kdense *x = hd + nd;
+      x->N = e->N;
+      x->m = r;
+      x->R = e->R;
+      x->I = e->dims[r];
+      x->G = e->G;
+      x->Vinv = e->Vinv;
+      x->status = e->status;
+      x->M = e->M;
+      x->A = e->A[r];
+      x->Abf = e->Abf[r];
+      x->iter = e->iter;
+      x->lambda = e->lambda;
+      x->part = e->part;
+      x->G_out = e->G + (size_t)r * e->R * e->R;
+      adatm_k_dense_blocks(x->I, x->R, &x->nmm, &x->ngr, &x->nsp);
+      x->mm0 = B->nb_mm;
+      x->gr0 = B->nb_gr;
+      x->sp0 = B->nb_sp;
+      B->nb_mm += x->nmm;
+      B->nb_gr += x->ngr;
+      B->nb_sp += x->nsp;
+      if (e->R > B->Rmax) B->Rmax = e->R;
+      ++B->n;
+      ++nd;
+    }
+    uint32_t *off[3] = {&B->mm_off, &B->gr_off, &B->sp_off};
+    for (int k = 0; k < 3; ++k) {
+      *off[k] = nm;
+      for (uint32_t q = 0; q < B->n; ++q) {
+        adatm_kdense const *x = hd + B->desc0 + q;
+        uint32_t const c = k == 0 ? x->nmm : (k == 1 ? x->ngr : x->nsp);
+        for (uint32_t t = 0; t < c; ++t) hm[nm++] = q;
+      }
+    }
+  }
+  b->nround = nround;
+  int rc = SPLATT_SUCCESS;
+  if (hipMalloc((void **)&b->ddesc, (size_t)nd * sizeof(adatm_kdense)) != hipSuccess ||
+      hipMalloc((void **)&b->dmap, (size_t)(nm ? nm : 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMemcpy(b->ddesc, hd, (size_t)nd * sizeof(adatm_kdense), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->dmap, hm, (size_t)nm * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    rc = hip_fail(hipGetLastError(), "batch descriptors");
+  free(hd);
+  free(hm);
+  return rc;
+}
+
+/* the grouped sweep: per round, the instances' MTTKRPs forked on their own
+ * streams, then one grouped dense update on the batch stream; the fits last */
+static int batch_enqueue_grouped(adatm_batch *b)
+{
+  for (uint32_t r = 0; r < b->nround; ++r) {
+    batch_round const *B = b->round + r;
+    if (hipEventRecord(b->fork, b->s) != hipSuccess) return SPLATT_ERROR_DEVICE;
+    for (int i = 0; i < b->n; ++i) {
+      adatm_engine *e = b->e[i];
+      if (e->N <= r) continue;
+      if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) return SPLATT_ERROR_DEVICE;
+      int const rc = mttkrp_dev(e, r);
+      if (rc != SPLATT_SUCCESS) return rc;
+      if (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
+          hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess)
+        return SPLATT_ERROR_DEVICE;
+    }
+    KCHK(adatm_k_dense_grouped(b->s, b->e[0]->prec, b->ddesc + B->desc0, B->n, B->Rmax,
+                                b->dmap + B->mm_off, B->nb_mm, b->dmap + B->gr_off,
+                                B->nb_gr, b->dmap + B->sp_off, B->nb_sp),
+          "grouped update");
+  }
+  if (hipEventRecord(b->fork, b->s) != hipSuccess) return SPLATT_ERROR_DEVICE;
+  for (int i = 0; i < b->n; ++i) {
+    adatm_engine *e = b->e[i];
+    uint32_t const N = e->N;
+    if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) return SPLATT_ERROR_DEVICE;
+    KCHK(adatm_k_fit(e->stream, e->prec, e->A[N - 1], e->M, e->dims[N - 1], e->lambda,
+                      e->G, N, e->R, e->ttnormsq, e->part, e->fits, e->iter),
+          "fit");
+    if (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
+        hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess)
+      return SPLATT_ERROR_DEVICE;
+  }
+  return SPLATT_SUCCESS;
 }
 
 int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
@@ -1981,6 +2116,10 @@ int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
     return SPLATT_ERROR_NOMEMORY;
   }
   memcpy(b->e, engines, (size_t)n * sizeof(adatm_engine *));
+  /* grouped dense updates need one precision across the instances */
+  b->grouped = 1;
+  for (int i = 1; i < n; ++i)
+    if (engines[i]->prec != engines[0]->prec) b->grouped = 0;
   int rc = SPLATT_SUCCESS;
   if (hipSetDevice(b->device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
@@ -2000,10 +2139,17 @@ int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
 static int batch_graph(adatm_batch *b)
 {
   if (b->gvalid) return SPLATT_SUCCESS;
+  if (b->grouped && !b->ddesc) {
+    int const rc = batch_rounds(b);
+    if (rc != SPLATT_SUCCESS) return rc;
+  }
   HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal), "begin capture");
   int rc = SPLATT_SUCCESS;
-  if (hipEventRecord(b->fork, b->s) != hipSuccess) rc = SPLATT_ERROR_DEVICE;
-  for (int i = 0; rc == SPLATT_SUCCESS && i < b->n; ++i) {
+  if (b->grouped)
+    rc = batch_enqueue_grouped(b);
+  else if (hipEventRecord(b->fork, b->s) != hipSuccess)
+    rc = SPLATT_ERROR_DEVICE;
+  for (int i = 0; !b->grouped && rc == SPLATT_SUCCESS && i < b->n; ++i) {
     adatm_engine *e = b->e[i];
     if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) {
       rc = SPLATT_ERROR_DEVICE;

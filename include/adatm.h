@@ -307,8 +307,12 @@ int adatm_engine_get_csf(adatm_engine *e, uint32_t grp, uint32_t **fptr,
 
 /* ----------------------------------------------------------------------- *
  * Batched CP-ALS of many independent engines on one device (no counterpart
- * in the reference): one hipGraph holds every instance's ALS sweep, each
- * instance on its own forked stream, so one graph launch runs a batched sweep.
+ * in the reference): one hipGraph holds every instance's ALS sweep, so one
+ * graph launch runs a batched sweep.  When the engines share one precision
+ * the sweep runs in rounds: round r forks every instance with more than r
+ * modes onto its own stream for its mode-r MTTKRP, then one grouped launch
+ * per dense step updates all of them (results equal to each engine's own
+ * sweep, bit for bit); otherwise each instance's whole sweep is a branch.
  * The batch borrows the engines (destroy them after the batch).
  * ----------------------------------------------------------------------- */
 typedef struct adatm_batch adatm_batch;
