@@ -136,11 +136,14 @@ typedef struct {
   int *status;
   void const *M;
   void *A, *Abf;
-  uint32_t const *iter;
+  uint32_t *iter;
   double *lambda, *part, *G_out;
   uint32_t mm0, nmm;     /* first block and block count of the product */
   uint32_t gr0, ngr;     /* ... of the Gram */
   uint32_t sp0, nsp;     /* ... of the Gram's partial sums */
+  uint32_t in0, nin;     /* fit: first block and block count of the inner product */
+  double ttnormsq;       /* fit: squared norm of the tensor */
+  double *fits;          /* fit: fits[*iter] is written, then ++*iter */
 } adatm_kdense;
 void adatm_k_dense_blocks(uint64_t I, uint32_t R, uint32_t *nmm, uint32_t *ngr,
                           uint32_t *nsp);
@@ -148,6 +151,12 @@ int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uin
                           uint32_t Rmax, uint32_t const *bm_mm, uint32_t nb_mm,
                           uint32_t const *bm_gr, uint32_t nb_gr, uint32_t const *bm_sp,
                           uint32_t nb_sp);
+/* the fits of many instances (descriptors with m = N-1: A the last factor,
+ * M its MTTKRP, I its rows; in0/nin, ttnormsq, fits, iter), as adatm_k_fit
+ * on each; adatm_k_fit_blocks(I) = the inner product's block count */
+uint32_t adatm_k_fit_blocks(uint64_t I);
+int adatm_k_fit_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uint32_t n,
+                        uint32_t const *bm_in, uint32_t nb_in);
 /* FP64 host matrix <-> device factor in the path's precision */
 int adatm_k_convert_in(hipStream_t s, int prec, double const *src64,
                        void *dst, void *dst_bf, uint64_t n);

@@ -430,7 +430,7 @@ __global__ __launch_bounds__(DB) void k_lambda(double const *__restrict__ part,
 
 /* per-block partial of sum_i A[i,r] * M[i,r] */
 template <typename T>
-__global__ __launch_bounds__(DB) void k_inner_part(T const *__restrict__ A,
+__device__ __forceinline__ void inner_part_body(uint32_t bid, uint32_t nblk, T const *__restrict__ A,
                                                    T const *__restrict__ M,
                                                    uint64_t I, uint32_t R,
                                                    double *__restrict__ part)
@@ -440,8 +440,8 @@ __global__ __launch_bounds__(DB) void k_inner_part(T const *__restrict__ A,
   uint32_t const rr = threadIdx.x / R, cc = threadIdx.x - rr * R;
   double s = 0.0;
   if (rr < rpi) {
-    uint64_t const chunk = (I + gridDim.x - 1) / gridDim.x;
-    uint64_t const lo = (uint64_t)blockIdx.x * chunk;
+    uint64_t const chunk = (I + nblk - 1) / nblk;
+    uint64_t const lo = (uint64_t)bid * chunk;
     uint64_t const hi = lo + chunk < I ? lo + chunk : I;
     for (uint64_t i = lo + rr; i < hi; i += rpi)
       s += (double)A[(size_t)i * R + cc] * (double)M[(size_t)i * R + cc];
@@ -451,8 +451,17 @@ __global__ __launch_bounds__(DB) void k_inner_part(T const *__restrict__ A,
   if (threadIdx.x < R) {
     double t = 0.0;
     for (uint32_t q = 0; q < rpi; ++q) t += red[q * R + threadIdx.x];
-    part[(size_t)blockIdx.x * R + threadIdx.x] = t;
+    part[(size_t)bid * R + threadIdx.x] = t;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void k_inner_part(T const *__restrict__ A,
+                                                   T const *__restrict__ M,
+                                                   uint64_t I, uint32_t R,
+                                                   double *__restrict__ part)
+{
+  inner_part_body<T>(blockIdx.x, gridDim.x, A, M, I, R, part);
 }
 
 /* per-block partial sum of squares */
@@ -481,7 +490,7 @@ __global__ __launch_bounds__(DB) void k_sqnorm_part(T const *__restrict__ x,
 
 /* fit = 1 - sqrt(|X|^2 + lambda^T (hada G) lambda - 2 <X,Z>) / |X|
  * (adatm_cpd.c:59-90, 109-160, 179-200); one block */
-__global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ part,
+__device__ __forceinline__ void fit_final_body(double const *__restrict__ part,
                                                   uint32_t nblk,
                                                   double const *__restrict__ lambda,
                                                   double const *__restrict__ G,
@@ -522,6 +531,18 @@ __global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ par
     fits[it] = 1.0 - residual / sqrt(ttnormsq);
     *iter = it + 1u;
   }
+}
+
+__global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ part,
+                                                  uint32_t nblk,
+                                                  double const *__restrict__ lambda,
+                                                  double const *__restrict__ G,
+                                                  uint32_t N, uint32_t R,
+                                                  double ttnormsq,
+                                                  double *__restrict__ fits,
+                                                  uint32_t *__restrict__ iter)
+{
+  fit_final_body(part, nblk, lambda, G, N, R, ttnormsq, fits, iter);
 }
 
 /* blocks of the row-parallel dense steps: at least `rows` rows each, at most
@@ -604,6 +625,21 @@ __global__ __launch_bounds__(DB) void g_sum(adatm_kdense const *__restrict__ d,
 {
   adatm_kdense const &x = d[bm[blockIdx.x]];
   sum_parts_body(blockIdx.x - x.sp0, x.part, x.ngr, x.R * x.R, x.G_out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void g_inner(adatm_kdense const *__restrict__ d,
+                                              uint32_t const *__restrict__ bm)
+{
+  adatm_kdense const &x = d[bm[blockIdx.x]];
+  inner_part_body<T>(blockIdx.x - x.in0, x.nin, (T const *)x.A, (T const *)x.M, x.I, x.R,
+                     x.part);
+}
+
+__global__ __launch_bounds__(DB) void g_fit(adatm_kdense const *__restrict__ d)
+{
+  adatm_kdense const &x = d[blockIdx.x];
+  fit_final_body(x.part, x.nin, x.lambda, x.G, x.N, x.R, x.ttnormsq, x.fits, x.iter);
 }
 
 } // namespace
@@ -718,6 +754,22 @@ int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uin
   }
   if ((e = hipGetLastError()) != hipSuccess) return herr(e);
   hipLaunchKernelGGL(g_sum, dim3(nb_sp), dim3(DB), 0, s, desc, bm_sp);
+  return herr(hipGetLastError());
+}
+
+uint32_t adatm_k_fit_blocks(uint64_t I) { return dense_blocks(I); }
+
+int adatm_k_fit_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uint32_t n,
+                        uint32_t const *bm_in, uint32_t nb_in)
+{
+  if (n == 0) return 0;
+  if (prec == KP_FP64)
+    hipLaunchKernelGGL((g_inner<double>), dim3(nb_in), dim3(DB), 0, s, desc, bm_in);
+  else
+    hipLaunchKernelGGL((g_inner<float>), dim3(nb_in), dim3(DB), 0, s, desc, bm_in);
+  hipError_t const e = hipGetLastError();
+  if (e != hipSuccess) return herr(e);
+  hipLaunchKernelGGL(g_fit, dim3(n), dim3(DB), 0, s, desc);
   return herr(hipGetLastError());
 }
 

@@ -1954,8 +1954,9 @@ struct adatm_batch {
   int grouped;
   uint32_t nround;
   batch_round *round;
-  adatm_kdense *ddesc; /* device descriptors of every round */
-  uint32_t *dmap;      /* device block maps of every round */
+  uint32_t fit0, nb_in, in_off; /* the fits: descriptors, grid, block map */
+  adatm_kdense *ddesc; /* device descriptors of every round, then the fits */
+  uint32_t *dmap;      /* device block maps of every round, then the fits' */
 };
 
 void adatm_batch_destroy(adatm_batch *b)
@@ -1986,7 +1987,7 @@ static int batch_rounds(adatm_batch *b)
   uint32_t nround = 0, ndesc = 0;
   for (int i = 0; i < b->n; ++i)
     if (b->e[i]->N > nround) nround = b->e[i]->N;
-  for (int i = 0; i < b->n; ++i) ndesc += b->e[i]->N;
+  for (int i = 0; i < b->n; ++i) ndesc += b->e[i]->N + 1; /* + its fit */
   b->round = (batch_round *)calloc(nround, sizeof(batch_round));
   adatm_kdense *hd = (adatm_kdense *)calloc(ndesc, sizeof(adatm_kdense));
   uint64_t nmap = 0;
@@ -1996,6 +1997,7 @@ static int batch_rounds(adatm_batch *b)
       adatm_k_dense_blocks(b->e[i]->dims[m], b->e[i]->R, &a, &g, &p);
       nmap += (uint64_t)a + g + p;
     }
+  for (int i = 0; i < b->n; ++i) nmap += adatm_k_fit_blocks(b->e[i]->dims[b->e[i]->N - 1]);
   uint32_t *hm = (uint32_t *)malloc((size_t)nmap * sizeof(uint32_t));
   if (!b->round || !hd || !hm) {
     free(hd);
@@ -2045,6 +2047,31 @@ static int batch_rounds(adatm_batch *b)
       }
     }
   }
+  /* the fits: every instance at its last mode */
+  b->fit0 = nd;
+  b->nb_in = 0;
+  for (int i = 0; i < b->n; ++i) {
+    adatm_engine *e = b->e[i];
+    adatm_kdense *x = hd + nd++;
+    x->N = e->N;
+    x->m = e->N - 1;
+    x->R = e->R;
+    x->I = e->dims[e->N - 1];
+    x->G = e->G;
+    x->A = e->A[e->N - 1];
+    x->M = e->M;
+    x->lambda = e->lambda;
+    x->part = e->part;
+    x->iter = e->iter;
+    x->ttnormsq = e->ttnormsq;
+    x->fits = e->fits;
+    x->nin = adatm_k_fit_blocks(x->I);
+    x->in0 = b->nb_in;
+    b->nb_in += x->nin;
+  }
+  b->in_off = nm;
+  for (int i = 0; i < b->n; ++i)
+    for (uint32_t t = 0; t < hd[b->fit0 + i].nin; ++t) hm[nm++] = (uint32_t)i;
   b->nround = nround;
   int rc = SPLATT_SUCCESS;
   if (hipMalloc((void **)&b->ddesc, (size_t)nd * sizeof(adatm_kdense)) != hipSuccess ||
@@ -2060,7 +2087,8 @@ static int batch_rounds(adatm_batch *b)
 }
 
 /* the grouped sweep: per round, the instances' MTTKRPs forked on their own
- * streams, then one grouped dense update on the batch stream; the fits last */
+ * streams, then one grouped dense update on the batch stream; one grouped
+ * fit last */
 static int batch_enqueue_grouped(adatm_batch *b)
 {
   for (uint32_t r = 0; r < b->nround; ++r) {
@@ -2081,18 +2109,9 @@ static int batch_enqueue_grouped(adatm_batch *b)
                                 B->nb_gr, b->dmap + B->sp_off, B->nb_sp),
           "grouped update");
   }
-  if (hipEventRecord(b->fork, b->s) != hipSuccess) return SPLATT_ERROR_DEVICE;
-  for (int i = 0; i < b->n; ++i) {
-    adatm_engine *e = b->e[i];
-    uint32_t const N = e->N;
-    if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) return SPLATT_ERROR_DEVICE;
-    KCHK(adatm_k_fit(e->stream, e->prec, e->A[N - 1], e->M, e->dims[N - 1], e->lambda,
-                      e->G, N, e->R, e->ttnormsq, e->part, e->fits, e->iter),
-          "fit");
-    if (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
-        hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess)
-      return SPLATT_ERROR_DEVICE;
-  }
+  KCHK(adatm_k_fit_grouped(b->s, b->e[0]->prec, b->ddesc + b->fit0, (uint32_t)b->n,
+                           b->dmap + b->in_off, b->nb_in),
+       "grouped fit");
   return SPLATT_SUCCESS;
 }
 
