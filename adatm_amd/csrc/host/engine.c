@@ -1988,6 +1988,7 @@ static int batch_rounds(adatm_batch *b)
   for (int i = 0; i < b->n; ++i)
     if (b->e[i]->N > nround) nround = b->e[i]->N;
   for (int i = 0; i < b->n; ++i) ndesc += b->e[i]->N + 1; /* + its fit */
+  free(b->round); /* a failed earlier attempt */
   b->round = (batch_round *)calloc(nround, sizeof(batch_round));
   adatm_kdense *hd = (adatm_kdense *)calloc(ndesc, sizeof(adatm_kdense));
   uint64_t nmap = 0;
@@ -2081,6 +2082,12 @@ static int batch_rounds(adatm_batch *b)
       hipMemcpy(b->dmap, hm, (size_t)nm * sizeof(uint32_t), hipMemcpyHostToDevice) !=
           hipSuccess)
     rc = hip_fail(hipGetLastError(), "batch descriptors");
+  if (rc != SPLATT_SUCCESS) {
+    hipFree(b->ddesc);
+    hipFree(b->dmap);
+    b->ddesc = NULL;
+    b->dmap = NULL;
+  }
   free(hd);
   free(hm);
   return rc;
