@@ -272,85 +272,104 @@ __global__ __launch_bounds__(256) void k_scatter(
       e = rowptr[p + 1];
     }
   }
-  uint32_t nd[SB], rr[D][SB];
-  auto load_idx = [&](uint32_t c0, uint32_t *mn, uint32_t (*mr)[SB]) {
-    uint32_t const nb = (b - c0 < (uint32_t)SB) ? b - c0 : (uint32_t)SB;
+  /* positions are taken in windows of W = a multiple of SB <= G (G if
+     G < SB): lane j < W
+     loads the node and ancestor row ids of window position j (past the
+     item's end: its last position), and each batch of SB reads them from
+     the holding lanes, so a window costs one index round trip */
+  uint32_t const gb = grp * G;
+  uint32_t const col = lane - gb;
+  uint32_t const W = (G >= SB) ? (G / SB) * SB : G;
+  auto load_win = [&](uint32_t w0, uint32_t &mn, uint32_t (&mr)[D]) {
+    uint32_t q = w0 + (col < W ? col : 0u);
+    q = (q < b) ? q : b - 1;
+    mn = sc.node[q];
 #pragma unroll
-    for (int i = 0; i < SB; ++i) {
-      uint32_t const q = c0 + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
-      mn[i] = sc.node[q];
-#pragma unroll
-      for (int l = 0; l < D; ++l) mr[l][i] = sc.rid[l][q];
-    }
+    for (int l = 0; l < D; ++l) mr[l] = sc.rid[l][q];
   };
-  load_idx(a, nd, rr);
+  uint32_t wnd, wrr[D];
+  load_win(a, wnd, wrr);
   T acc[V];
 #pragma unroll
   for (int c = 0; c < V; ++c) acc[c] = (T)0;
-  for (uint32_t c0 = a; c0 < b; c0 += SB) {
-    uint32_t const nb = (b - c0 < (uint32_t)SB) ? b - c0 : (uint32_t)SB;
-    T v[SB][V];
-#pragma unroll
-    for (int i = 0; i < SB; ++i) {
-      T y[V], u[V], z[V];
-      ldrow((TU const *)sc.U[0] + (rr[0][i] * R + cv), y);
-#pragma unroll
-      for (int l = 1; l < D; ++l) {
-        ldrow((TU const *)sc.U[l] + (rr[l][i] * R + cv), u);
-#pragma unroll
-        for (int c = 0; c < V; ++c) y[c] = y[c] * u[c];
-      }
-      if constexpr (ZS) {
-        T const zs = Z[nd[i]];
-#pragma unroll
-        for (int c = 0; c < V; ++c) v[i][c] = zs * y[c];
-      } else {
-        ldrow(Z + ((size_t)nd[i] * R + cv), z);
-#pragma unroll
-        for (int c = 0; c < V; ++c) v[i][c] = z[c] * y[c];
-      }
-    }
-    uint32_t const c1 = c0 + SB;
-    if (c1 < b) load_idx(c1, nd, rr);
-    if (nb == (uint32_t)SB && (heavy || e >= c0 + SB)) {
-      /* no row ends before the batch's last node (a piece has no row end) */
-#pragma unroll
-      for (int c = 0; c < V; ++c) {
-        T const s01 = v[0][c] + v[1][c], s23 = v[2][c] + v[3][c];
-        T const s45 = v[4][c] + v[5][c], s67 = v[6][c] + v[7][c];
-        acc[c] += (s01 + s23) + (s45 + s67);
-      }
-      if (!heavy && e == c0 + SB) { /* the row ends with the batch */
-        strow<T, V>(out + ((size_t)p * R + cv), acc);
-#pragma unroll
-        for (int c = 0; c < V; ++c) acc[c] = (T)0;
-        while (++p < pe) {
-          e = rowptr[p + 1];
-          if (e != c0 + SB) break;
-          strow<T, V>(out + ((size_t)p * R + cv), zero);
-        }
-      }
-    } else {
+  for (uint32_t w0 = a; w0 < b; w0 += W) {
+    uint32_t const we = (b - w0 < W) ? b : w0 + W;
+    uint32_t nwnd = 0, nwrr[D];
+    if (we < b) load_win(we, nwnd, nwrr);
+    for (uint32_t c0 = w0; c0 < we; c0 += SB) {
+      uint32_t const nb = (we - c0 < (uint32_t)SB) ? we - c0 : (uint32_t)SB;
+      uint32_t nd[SB], rr[D][SB];
 #pragma unroll
       for (int i = 0; i < SB; ++i) {
-        uint32_t const c = c0 + i;
-        if ((uint32_t)i < nb) {
+        uint32_t const src = gb + (c0 - w0) + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
+        nd[i] = lane_get(wnd, src);
 #pragma unroll
-          for (int q = 0; q < V; ++q) acc[q] += v[i][q];
-          if (!heavy && c + 1 == e) {
-            strow<T, V>(out + ((size_t)p * R + cv), acc);
+        for (int l = 0; l < D; ++l) rr[l][i] = lane_get(wrr[l], src);
+      }
+      T v[SB][V];
 #pragma unroll
-            for (int q = 0; q < V; ++q) acc[q] = (T)0;
-            /* next non-empty row; empty rows in between get zeros */
-            while (++p < pe) {
-              e = rowptr[p + 1];
-              if (e != c + 1) break;
-              strow<T, V>(out + ((size_t)p * R + cv), zero);
+      for (int i = 0; i < SB; ++i) {
+        T y[V], u[V], z[V];
+        ldrow((TU const *)sc.U[0] + (rr[0][i] * R + cv), y);
+#pragma unroll
+        for (int l = 1; l < D; ++l) {
+          ldrow((TU const *)sc.U[l] + (rr[l][i] * R + cv), u);
+#pragma unroll
+          for (int c = 0; c < V; ++c) y[c] = y[c] * u[c];
+        }
+        if constexpr (ZS) {
+          T const zs = Z[nd[i]];
+#pragma unroll
+          for (int c = 0; c < V; ++c) v[i][c] = zs * y[c];
+        } else {
+          ldrow(Z + ((size_t)nd[i] * R + cv), z);
+#pragma unroll
+          for (int c = 0; c < V; ++c) v[i][c] = z[c] * y[c];
+        }
+      }
+      if (nb == (uint32_t)SB && (heavy || e >= c0 + SB)) {
+        /* no row ends before the batch's last node (a piece has no row end) */
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+          T const s01 = v[0][c] + v[1][c], s23 = v[2][c] + v[3][c];
+          T const s45 = v[4][c] + v[5][c], s67 = v[6][c] + v[7][c];
+          acc[c] += (s01 + s23) + (s45 + s67);
+        }
+        if (!heavy && e == c0 + SB) { /* the row ends with the batch */
+          strow<T, V>(out + ((size_t)p * R + cv), acc);
+#pragma unroll
+          for (int c = 0; c < V; ++c) acc[c] = (T)0;
+          while (++p < pe) {
+            e = rowptr[p + 1];
+            if (e != c0 + SB) break;
+            strow<T, V>(out + ((size_t)p * R + cv), zero);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < SB; ++i) {
+          uint32_t const c = c0 + i;
+          if ((uint32_t)i < nb) {
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[q] += v[i][q];
+            if (!heavy && c + 1 == e) {
+              strow<T, V>(out + ((size_t)p * R + cv), acc);
+#pragma unroll
+              for (int q = 0; q < V; ++q) acc[q] = (T)0;
+              /* next non-empty row; empty rows in between get zeros */
+              while (++p < pe) {
+                e = rowptr[p + 1];
+                if (e != c + 1) break;
+                strow<T, V>(out + ((size_t)p * R + cv), zero);
+              }
             }
           }
         }
       }
     }
+    wnd = nwnd;
+#pragma unroll
+    for (int l = 0; l < D; ++l) wrr[l] = nwrr[l];
   }
   if (heavy) strow<T, V>(carry + ((size_t)slot * R + cv), acc);
 }
