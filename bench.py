@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -66,6 +67,18 @@ def measured_traffic(config: int, kernel: str):
     except OSError:
         return None
     return None
+
+
+def finite(x):
+    """the JSON line holds no NaN/Inf (strict parsers reject them): a
+    non-finite number becomes null"""
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    if isinstance(x, dict):
+        return {k: finite(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [finite(v) for v in x]
+    return x
 
 
 def log(msg: str) -> None:
@@ -258,7 +271,7 @@ def main() -> int:
         "launches_ms": [[launch_name(c, i), round(ms, 4), round(b / 1e9, 4)]
                         for c, i, ms, b in r32["launches"] if c != 3],
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(finite(out), allow_nan=False), flush=True)
     return 0
 
 
@@ -346,7 +359,7 @@ def batched_c5(args) -> int:
         "aggregate_mttkrp_GBps": round(seq_bytes / (per * 1e-3) / 1e9, 1),
         "max_abs_fit_diff_fp32_vs_fp64": dfit,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(finite(out), allow_nan=False), flush=True)
     b.close()
     return 0
 

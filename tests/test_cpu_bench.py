@@ -36,3 +36,9 @@ def test_committed_traffic_covers_the_c2_passes():
         assert bench.measured_traffic(2, k) == t[k]["fetch_bytes"] + t[k]["write_bytes"]
     # the memo level written by the K=2 leaf pass (56M rows x 128 B) shows up
     assert t["k_reduce<float, float, true, 2>"]["write_bytes"] > 7.0e9
+
+
+def test_json_line_holds_no_nan():
+    out = bench.finite({"fit": float("nan"), "errs": [1e-7, float("inf")], "cfg": {"n": 3}})
+    assert out == {"fit": None, "errs": [1e-7, None], "cfg": {"n": 3}}
+    json.dumps(out, allow_nan=False)
