@@ -278,49 +278,77 @@ __device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
   __shared__ double rowk[2][64], colk[2][4 * Q];
   uint32_t const s = threadIdx.x >> 6, j = threadIdx.x & 63u;
   double a[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    uint32_t const i = s + 4u * (uint32_t)q;
-    double v = 1.0;
-    if (i < R && j < R) {
-      size_t const x = (size_t)i * R + j;
-      for (uint32_t k = 0; k < N; ++k)
-        if (k != m) v *= G[(size_t)k * R * R + x];
-    }
-    a[q] = v;
-  }
-  int bad = 0;
-  /* entries of rows or columns >= R never flow into rows and columns < R */
-  for (uint32_t k = 0; k < R; ++k) {
-    uint32_t const b = k & 1u;
-    if ((k & 3u) == s) {
-      double v = a[0];
-#pragma unroll
-      for (int q = 1; q < Q; ++q) v = ((uint32_t)q == (k >> 2)) ? a[q] : v;
-      rowk[b][j] = v;
-    }
-    if (j == k) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) colk[b][s + 4u * (uint32_t)q] = a[q];
-    }
-    __syncthreads();
-    double const p = rowk[b][k];
-    bad |= !(p > 0.0);
-    double const pinv = 1.0 / p;
-    double const rk = ((j == k) ? 1.0 : rowk[b][j]) * pinv;
+  /* the matrix (plus ridge on the diagonal) into registers */
+  auto build = [&](double ridge) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       uint32_t const i = s + 4u * (uint32_t)q;
-      double const f = colk[b][i];
-      a[q] = (i == k) ? rk : ((j == k) ? 0.0 : a[q]) - f * rk;
+      double v = 1.0;
+      if (i < R && j < R) {
+        size_t const x = (size_t)i * R + j;
+        for (uint32_t k = 0; k < N; ++k)
+          if (k != m) v *= G[(size_t)k * R * R + x];
+        if (i == j) v += ridge;
+      }
+      a[q] = v;
     }
+  };
+  build(0.0);
+  int bad = 0, reg = 0;
+  /* a pivot that is not positive (the product is not numerically SPD):
+     solve again with a ridge of 1e-10, then 1e-6, of the largest diagonal */
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (attempt > 0) {
+      /* the diagonal through LDS, its maximum read by every thread */
+      build(0.0);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        uint32_t const i = s + 4u * (uint32_t)q;
+        if (i == j) colk[0][i] = a[q];
+      }
+      __syncthreads();
+      double dm = 0.0;
+      for (uint32_t i = 0; i < R; ++i) dm = colk[0][i] > dm ? colk[0][i] : dm;
+      __syncthreads();
+      build(dm * (attempt == 1 ? 1e-10 : 1e-6));
+    }
+    bad = 0;
+    /* entries of rows or columns >= R never flow into rows and columns < R */
+    for (uint32_t k = 0; k < R; ++k) {
+      uint32_t const b = k & 1u;
+      if ((k & 3u) == s) {
+        double v = a[0];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) v = ((uint32_t)q == (k >> 2)) ? a[q] : v;
+        rowk[b][j] = v;
+      }
+      if (j == k) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) colk[b][s + 4u * (uint32_t)q] = a[q];
+      }
+      __syncthreads();
+      double const p = rowk[b][k];
+      bad |= !(p > 0.0);
+      double const pinv = 1.0 / p;
+      double const rk = ((j == k) ? 1.0 : rowk[b][j]) * pinv;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        uint32_t const i = s + 4u * (uint32_t)q;
+        double const f = colk[b][i];
+        a[q] = (i == k) ? rk : ((j == k) ? 0.0 : a[q]) - f * rk;
+      }
+    }
+    if (!bad) break;
+    reg = 1;
+    __syncthreads(); /* the LDS broadcasts are reused by the next attempt */
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     uint32_t const i = s + 4u * (uint32_t)q;
     if (i < R && j < R) Vinv[(size_t)i * R + j] = a[q];
   }
-  if (threadIdx.x == 0 && status) *status = bad;
+  /* 0: clean; 1: solved with a ridge; 2: no positive pivots even then */
+  if (threadIdx.x == 0 && status) *status = bad ? 2 : reg;
 }
 
 template <int Q>
