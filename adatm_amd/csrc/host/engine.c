@@ -1874,12 +1874,18 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
  * reduction tile, each a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
  * median); then, on the best engine, the default batch against the
  * children per batch over {4, 8, 16}, the pipelined 4 and 8 and the wide 4
- * and 8. */
+ * and 8.  The defaults are timed first and a candidate displaces the best
+ * so far only when it is more than TUNE_MARGIN faster: near-equal choices
+ * differ by run-to-run noise, and without the margin the choice (and with it
+ * the kernel that dominates the sweep) flips between runs. */
+#define TUNE_MARGIN 0.99
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, uint32_t *batch, double *best_ms)
 {
-  static uint32_t const RT[5] = {32, 64, 128, 256, 512}, ST[5] = {16, 32, 64, 128, 256};
+  /* defaults first */
+  static uint32_t const RT[5] = {RED_TILE, 32, 64, 256, 512},
+                        ST[5] = {SCAT_TILE, 16, 32, 128, 256};
   uint32_t br = RED_TILE, bs = SCAT_TILE;
   double bt = INFINITY;
   for (int pass = 0; pass < 2; ++pass)
@@ -1894,7 +1900,7 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
       if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
       adatm_engine_destroy(e);
       if (rc != SPLATT_SUCCESS) return rc;
-      if (ms < bt) {
+      if (ms < bt * TUNE_MARGIN) {
         bt = ms;
         br = r;
         bs = sc;
@@ -1913,7 +1919,7 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
       double ms = INFINITY;
       rc = adatm_engine_set_batch(e, BT[i]);
       if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
-      if (rc == SPLATT_SUCCESS && ms < bt) {
+      if (rc == SPLATT_SUCCESS && ms < bt * TUNE_MARGIN) {
         bt = ms;
         bb = BT[i];
       }

@@ -228,8 +228,9 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
 #define ADATM_BATCH_WIDE 0x200u
 int adatm_engine_set_batch(adatm_engine *e, uint32_t batch);
 /* Autotuner: the tiles and batch of one plan chosen by timing CP-ALS sweeps
- * (nrep, median) of engines built with each candidate; batch (nullable) gets
- * the value for adatm_engine_set_batch (0 = default). */
+ * (nrep, median) of engines built with each candidate; a candidate replaces
+ * the defaults (timed first) only when it is more than 1% faster.  batch
+ * (nullable) gets the value for adatm_engine_set_batch (0 = default). */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, uint32_t *batch, double *best_ms);
