@@ -1871,10 +1871,11 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
 /* Work-item tiles and reduction batch of one plan chosen by timing: the
  * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
  * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
- * reduction tile, each a fresh engine timed on nrep CP-ALS sweeps (hipGraph,
- * median); then, on the best engine, the default batch against the
- * children per batch over {4, 8, 16}, the pipelined 4 and 8 and the wide 4
- * and 8.  The defaults are timed first and a candidate displaces the best
+ * reduction tile, then (when the scatter tile moved) the reduction tile
+ * again with the new scatter tile, each a fresh engine timed on nrep CP-ALS
+ * sweeps (hipGraph, median); then, on the best engine, the default batch
+ * against the children per batch over {4, 8, 16}, the pipelined 4 and 8 and
+ * the wide 4 and 8.  The defaults are timed first and a candidate displaces the best
  * so far only when it is more than TUNE_MARGIN faster: near-equal choices
  * differ by run-to-run noise, and without the margin the choice (and with it
  * the kernel that dominates the sweep) flips between runs. */
@@ -1888,10 +1889,17 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
                         ST[5] = {SCAT_TILE, 16, 32, 128, 256};
   uint32_t br = RED_TILE, bs = SCAT_TILE;
   double bt = INFINITY;
-  for (int pass = 0; pass < 2; ++pass)
+  uint32_t r1 = 0; /* the reduction tile after the second pass */
+  for (int pass = 0; pass < 3; ++pass) {
+    if (pass == 2) {
+      if (bs == SCAT_TILE) break; /* every reduction tile was timed with it */
+      r1 = br;
+    }
     for (int i = 0; i < 5; ++i) {
-      uint32_t const r = pass == 0 ? RT[i] : br, sc = pass == 0 ? SCAT_TILE : ST[i];
+      uint32_t const r = pass == 1 ? br : RT[i];
+      uint32_t const sc = pass == 0 ? SCAT_TILE : (pass == 1 ? ST[i] : bs);
       if (pass == 1 && sc == SCAT_TILE) continue; /* timed in the first pass */
+      if (pass == 2 && r == r1) continue;         /* timed in the second pass */
       adatm_engine *e = NULL;
       int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, r, sc, &e);
       if (rc != SPLATT_SUCCESS) return rc;
@@ -1906,6 +1914,7 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
         bs = sc;
       }
     }
+  }
   uint32_t bb = 0;
   {
     adatm_engine *e = NULL;
