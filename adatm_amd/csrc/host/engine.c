@@ -1610,10 +1610,14 @@ int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
     if (rc != SPLATT_SUCCESS) return rc;
   }
   HIPCHK(hipStreamSynchronize(e->stream), "warmup sync");
-  double *t = (double *)malloc((size_t)nrep * sizeof(double));
+  double t[1000]; /* nrep <= 1000, checked above */
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a), "event");
-  HIPCHK(hipEventCreate(&b), "event");
+  hipError_t const eb = hipEventCreate(&b);
+  if (eb != hipSuccess) {
+    hipEventDestroy(a);
+    return hip_fail(eb, "event");
+  }
   for (int i = 0; i < nrep; ++i) {
     if (what == 1 && (uint32_t)(nwarm + i + 1) >= e->fits_cap) {
       rc = cpd_prepare(e);
@@ -1634,13 +1638,9 @@ int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
-  if (rc != SPLATT_SUCCESS) {
-    free(t);
-    return rc;
-  }
+  if (rc != SPLATT_SUCCESS) return rc;
   qsort(t, (size_t)nrep, sizeof(double), cmp_double);
   *ms_median = (nrep % 2) ? t[nrep / 2] : 0.5 * (t[nrep / 2 - 1] + t[nrep / 2]);
-  free(t);
 
   if (per_kernel_ms) {
     /* one instrumented, non-graph pass: per-class device time and bytes */
@@ -1790,7 +1790,11 @@ int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
   HIPCHK(hipStreamSynchronize(e->stream), "pre sync");
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a), "event");
-  HIPCHK(hipEventCreate(&b), "event");
+  hipError_t const eb = hipEventCreate(&b);
+  if (eb != hipSuccess) {
+    hipEventDestroy(a);
+    return hip_fail(eb, "event");
+  }
   hipEventRecord(a, e->stream);
   rc = SPLATT_SUCCESS;
   for (uint32_t i = 0; i < nsteps && rc == SPLATT_SUCCESS; ++i) {
@@ -2238,7 +2242,11 @@ int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *f
   if (rc != SPLATT_SUCCESS) return rc;
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0), "event");
-  HIPCHK(hipEventCreate(&t1), "event");
+  hipError_t const et1 = hipEventCreate(&t1);
+  if (et1 != hipSuccess) {
+    hipEventDestroy(t0);
+    return hip_fail(et1, "event");
+  }
   hipEventRecord(t0, b->s);
   for (uint32_t it = 0; it < niters; ++it)
     if (hipGraphLaunch(b->gexec, b->s) != hipSuccess) {
