@@ -1,5 +1,6 @@
 """The bench contract pieces that need no GPU: the multi-GPU refusal, the
-traffic summary the roofline reads, and the kernel names it is keyed by."""
+traffic summary the roofline reads, the kernel names it is keyed by, and the
+CPU baseline's sampling."""
 import json
 import os
 import subprocess
@@ -42,3 +43,17 @@ def test_json_line_holds_no_nan():
     out = bench.finite({"fit": float("nan"), "errs": [1e-7, float("inf")], "cfg": {"n": 3}})
     assert out == {"fit": None, "errs": [1e-7, None], "cfg": {"n": 3}}
     json.dumps(out, allow_nan=False)
+
+
+def test_cpu_baseline_times_a_small_workload_whole():
+    """A workload no larger than the sample cap is timed whole, unscaled; a
+    larger one is timed on a prefix of its draws and scaled by the ratio."""
+    import types
+    import adatm_amd as A
+    whole = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=6_000_000,
+                                                     ndraws=40_000), 8, A)
+    assert whole["unit"] == "ms/sweep" and whole["sample"].startswith("the whole workload")
+    part = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=10_000,
+                                                    ndraws=40_000), 8, A)
+    assert part["unit"] == "ms/sweep (scaled)" and "scaled x4.0" in part["sample"]
+    assert whole["value"] > 0 and part["value"] > 0
