@@ -213,6 +213,7 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     mttkrp_ms = sum(times[c] for c in MTTKRP_CLASSES)
     mttkrp_bytes = sum(pk[16 + c] for c in MTTKRP_CLASSES)
+    traffic = measured_traffic(args.config, template_name(dcls, dinfo, t.nmodes))
 
     cpu = None
     if not args.no_cpu:
@@ -246,8 +247,9 @@ def main() -> int:
             "bound": "hbm", "kernel": launch_name(dcls, dinfo),
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.config, template_name(dcls, dinfo, t.nmodes)),
-            "traffic_source": f"profiles/c{args.config}_hbm_traffic.md (rocprofv3 --pmc, copy-calibrated)",
+            "traffic": traffic,
+            "traffic_source": (f"profiles/c{args.config}_hbm_traffic.md (rocprofv3 --pmc, "
+                               f"copy-calibrated)" if traffic is not None else None),
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
