@@ -21,6 +21,7 @@ SPLATT_SUCCESS = 1
 SPLATT_ERROR_BADINPUT = 2
 SPLATT_ERROR_NOMEMORY = 3
 SPLATT_ERROR_DEVICE = 4
+SPLATT_ERROR_NUMERIC = 5
 
 FP64, FP32, BF16ROWS = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16rows": BF16ROWS}
@@ -95,6 +96,11 @@ class Plan(C.Structure):
 
 
 _lib = None
+
+
+# timer(ctx, reduction tile, scatter tile, batch, &ms) of adatm_autotune_search
+TILE_TIMER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                         C.POINTER(C.c_double))
 
 
 def lib() -> C.CDLL:
@@ -189,6 +195,13 @@ def lib() -> C.CDLL:
                                             C.c_int, P(Plan), C.c_int, P(C.c_int),
                                             P(C.c_int), P(C.c_double)]),
         "adatm_costmodel_path": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "adatm_engine_cpd_begin": (C.c_int, [C.c_void_p]),
+        "adatm_engine_cpd_sweep": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int,
+                                             P(C.c_double)]),
+        "adatm_engine_solve_status": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "adatm_autotune_search": (C.c_int, [TILE_TIMER, C.c_void_p, P(C.c_uint32),
+                                            P(C.c_uint32), P(C.c_uint32), P(C.c_double)]),
+        "adatm_k_bf16_bits": (C.c_uint16, [C.c_float]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -533,6 +546,24 @@ def autotune_tiles(t: Tensor, rank: int, plan: Plan, precision: int = FP32,
     return (r.value, s.value, b.value), ms.value
 
 
+def autotune_search(timer):
+    """The autotuner's search driven by timer(red_tile, scat_tile, batch) ->
+    ms; returns ((red_tile, scat_tile, batch), best ms)."""
+    def cb(_ctx, r, s_, b, ms):
+        ms[0] = float(timer(int(r), int(s_), int(b)))
+        return SPLATT_SUCCESS
+    fn = TILE_TIMER(cb)
+    r, s_, b, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+    check(lib().adatm_autotune_search(fn, None, C.byref(r), C.byref(s_), C.byref(b),
+                                      C.byref(ms)), "autotune_search")
+    return (r.value, s_.value, b.value), ms.value
+
+
+def bf16_bits(x: float) -> int:
+    """BF16 bits of an FP32 value as the BF16 factor-row copies round it"""
+    return int(lib().adatm_k_bf16_bits(C.c_float(x)))
+
+
 def costmodel_path():
     buf = C.create_string_buffer(4096)
     return buf.value.decode() if lib().adatm_costmodel_path(buf, 4096) else None
@@ -604,6 +635,22 @@ class Engine:
         check(lib().adatm_engine_cpd(self.h, niters, tol, int(graph), _dptr(fits),
                                      C.byref(done)), "cpd")
         return fits[: done.value]
+
+    def cpd_begin(self) -> None:
+        check(lib().adatm_engine_cpd_begin(self.h), "cpd_begin")
+
+    def cpd_sweep(self, it: int, graph: bool = False) -> float:
+        fit = C.c_double()
+        check(lib().adatm_engine_cpd_sweep(self.h, it, int(graph), C.byref(fit)),
+              "cpd_sweep")
+        return fit.value
+
+    def solve_status(self) -> int:
+        """worst (hadamard of Grams)^-1 status since the last CP-ALS start:
+        0 clean, 1 solved with a ridge, 2 failed"""
+        st = C.c_int()
+        check(lib().adatm_engine_solve_status(self.h, C.byref(st)), "solve_status")
+        return st.value
 
     def time(self, what: int, nwarm: int, nrep: int, graph: bool = False,
              per_kernel: bool = True):

@@ -116,12 +116,12 @@ int adatm_k_update(hipStream_t s, int prec, void const *M, double const *Vinv,
                    void *A, void *Abf, uint64_t I, uint32_t R,
                    uint32_t const *iter, double *lambda, double *part,
                    double *G_out);
-/* fit of the current model into fits[*iter], then ++*iter (needs M of the
- * last mode). */
+/* fit of the current model into fits[*iter] (skipped when *iter >= fcap,
+ * the entries of fits), then ++*iter (needs M of the last mode). */
 int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
                 uint64_t I_last, double const *lambda, double const *G,
                 uint32_t N, uint32_t R, double ttnormsq, double *part,
-                double *fits, uint32_t *iter);
+                double *fits, uint32_t fcap, uint32_t *iter);
 /* The dense update of many CP-ALS instances at once (one per descriptor,
  * each at its own mode m): solve, A = M Vinv with column statistics, lambda,
  * normalisation and Gram, with every instance's arithmetic, block split and
@@ -129,7 +129,8 @@ int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
  * Descriptors and block maps live in device memory; bm_* map every block of
  * a grouped grid to its descriptor (adatm_k_dense_blocks gives the counts). */
 typedef struct {
-  uint32_t N, m, R, pad;
+  uint32_t N, m, R;
+  uint32_t fcap;         /* fit: entries of fits */
   uint64_t I;
   double const *G;       /* N grams R x R */
   double *Vinv;
@@ -157,6 +158,9 @@ int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uin
 uint32_t adatm_k_fit_blocks(uint64_t I);
 int adatm_k_fit_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uint32_t n,
                         uint32_t const *bm_in, uint32_t nb_in);
+/* bits of the BF16 rounding (nearest even; NaN stays a quiet NaN) that
+ * the BF16 factor-row copies use, evaluated on the host */
+uint16_t adatm_k_bf16_bits(float f);
 /* FP64 host matrix <-> device factor in the path's precision */
 int adatm_k_convert_in(hipStream_t s, int prec, double const *src64,
                        void *dst, void *dst_bf, uint64_t n);

@@ -94,10 +94,15 @@ __device__ __forceinline__ void strow(T *p, T const (&o)[V])
     __builtin_memcpy(__builtin_assume_aligned(p, V * sizeof(T)), o, V * sizeof(T));
 }
 
-/* FP32 -> BF16 with round-to-nearest-even (NaN/Inf keep their class) */
-__device__ __forceinline__ uint16_t to_bf16(float f)
+/* FP32 -> BF16 with round-to-nearest-even.  Infinities keep their sign;
+ * every NaN becomes a quiet NaN of its sign (its payload may sit in the low
+ * 16 mantissa bits only, which truncation alone would turn into an
+ * infinity). */
+__host__ __device__ __forceinline__ uint16_t to_bf16(float f)
 {
-  uint32_t u = __float_as_uint(f);
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
   if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);

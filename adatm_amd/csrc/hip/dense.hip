@@ -347,8 +347,13 @@ __device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
     uint32_t const i = s + 4u * (uint32_t)q;
     if (i < R && j < R) Vinv[(size_t)i * R + j] = a[q];
   }
-  /* 0: clean; 1: solved with a ridge; 2: no positive pivots even then */
-  if (threadIdx.x == 0 && status) *status = bad ? 2 : reg;
+  /* 0: clean; 1: solved with a ridge; 2: no positive pivots even then.
+     The status is sticky (the worst since it was last cleared): solves run
+     in stream order, one writer each, so a plain read-compare-write holds. */
+  if (threadIdx.x == 0 && status) {
+    int const v = bad ? 2 : reg;
+    if (v > *status) *status = v;
+  }
 }
 
 template <int Q>
@@ -525,6 +530,7 @@ __device__ __forceinline__ void fit_final_body(double const *__restrict__ part,
                                                   uint32_t N, uint32_t R,
                                                   double ttnormsq,
                                                   double *__restrict__ fits,
+                                                  uint32_t fcap,
                                                   uint32_t *__restrict__ iter)
 {
   __shared__ double red[DB];
@@ -556,7 +562,7 @@ __device__ __forceinline__ void fit_final_body(double const *__restrict__ part,
     double const norm_mats = fabs(red2[0]);
     double const residual = sqrt(ttnormsq + norm_mats - 2.0 * red[0]);
     uint32_t const it = *iter;
-    fits[it] = 1.0 - residual / sqrt(ttnormsq);
+    if (it < fcap) fits[it] = 1.0 - residual / sqrt(ttnormsq); /* never past the buffer */
     *iter = it + 1u;
   }
 }
@@ -568,9 +574,10 @@ __global__ __launch_bounds__(DB) void k_fit_final(double const *__restrict__ par
                                                   uint32_t N, uint32_t R,
                                                   double ttnormsq,
                                                   double *__restrict__ fits,
+                                                  uint32_t fcap,
                                                   uint32_t *__restrict__ iter)
 {
-  fit_final_body(part, nblk, lambda, G, N, R, ttnormsq, fits, iter);
+  fit_final_body(part, nblk, lambda, G, N, R, ttnormsq, fits, fcap, iter);
 }
 
 /* blocks of the row-parallel dense steps: at least `rows` rows each, at most
@@ -667,7 +674,7 @@ __global__ __launch_bounds__(DB) void g_inner(adatm_kdense const *__restrict__ d
 __global__ __launch_bounds__(DB) void g_fit(adatm_kdense const *__restrict__ d)
 {
   adatm_kdense const &x = d[blockIdx.x];
-  fit_final_body(x.part, x.nin, x.lambda, x.G, x.N, x.R, x.ttnormsq, x.fits, x.iter);
+  fit_final_body(x.part, x.nin, x.lambda, x.G, x.N, x.R, x.ttnormsq, x.fits, x.fcap, x.iter);
 }
 
 } // namespace
@@ -831,7 +838,7 @@ int adatm_k_update(hipStream_t s, int prec, void const *M, double const *Vinv,
 int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
                 uint64_t I_last, double const *lambda, double const *G,
                 uint32_t N, uint32_t R, double ttnormsq, double *part,
-                double *fits, uint32_t *iter)
+                double *fits, uint32_t fcap, uint32_t *iter)
 {
   uint32_t const nb = dense_blocks(I_last);
   if (prec == KP_FP64)
@@ -845,9 +852,12 @@ int adatm_k_fit(hipStream_t s, int prec, void const *A_last, void const *M,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return herr(e);
   hipLaunchKernelGGL(k_fit_final, dim3(1), dim3(DB), 0, s, part, nb, lambda, G,
-                     N, R, ttnormsq, fits, iter);
+                     N, R, ttnormsq, fits, fcap, iter);
   return herr(hipGetLastError());
 }
+
+/* the FP32 -> BF16 rounding of the BF16 factor-row copies, on the host */
+uint16_t adatm_k_bf16_bits(float f) { return to_bf16(f); }
 
 int adatm_k_convert_in(hipStream_t s, int prec, double const *src64, void *dst,
                        void *dst_bf, uint64_t n)

@@ -896,6 +896,11 @@ static int cpd_registered(reg_ent *r, splatt_idx_t nfactors,
     oldfit = fit;
   }
   if (verb > SPLATT_VERBOSITY_NONE) printf("CPD = %0.3fs\n", adatm_wtime() - t_all);
+  int st = 0;
+  if (adatm_engine_solve_status(r->eng, &st) == SPLATT_SUCCESS && st == 1 &&
+      verb > SPLATT_VERBOSITY_NONE)
+    fprintf(stderr, "warning: the CP-ALS normal equations were not numerically positive "
+                    "definite; solved with a ridge\n");
   memset(out, 0, sizeof(*out));
   out->rank = nfactors;
   out->nmodes = r->N;

@@ -104,6 +104,7 @@ struct adatm_engine {
   double ttnormsq;
   size_t dev_bytes;
   uint32_t fits_cap;
+  uint32_t nsweep; /* sweeps since cpd_prepare: the device counter `iter` */
   hipGraphExec_t gexec;
   hipGraph_t graph;
   int gvalid;
@@ -1502,13 +1503,14 @@ static int sweep_enqueue(adatm_engine *e)
   ev_begin(e, KC_DENSE, 0, 0.0);
   KCHK(adatm_k_fit(e->stream, e->prec, e->A[N - 1], e->M, e->dims[N - 1],
                    e->lambda, e->G, N, R, e->ttnormsq, e->part, e->fits,
-                   e->iter),
+                   e->fits_cap, e->iter),
        "fit");
   ev_end(e);
   return SPLATT_SUCCESS;
 }
 
-/* Grams of the current factors and the sweep counter reset */
+/* Grams of the current factors, the sweep counter and the solve status
+ * reset */
 static int cpd_prepare(adatm_engine *e)
 {
   uint32_t const R = e->R;
@@ -1517,6 +1519,29 @@ static int cpd_prepare(adatm_engine *e)
                       e->G + (size_t)m * R * R),
          "gram");
   KCHK(adatm_k_zero(e->stream, e->iter, 4 * sizeof(uint32_t)), "zero iter");
+  KCHK(adatm_k_zero(e->stream, e->status, 4 * sizeof(int)), "zero status");
+  e->nsweep = 0;
+  return SPLATT_SUCCESS;
+}
+
+/* worst solve status since cpd_prepare (the stream is synchronised here) */
+static int read_status(adatm_engine *e, int *st)
+{
+  HIPCHK(hipMemcpyAsync(st, e->status, sizeof(int), hipMemcpyDeviceToHost, e->stream),
+         "status download");
+  HIPCHK(hipStreamSynchronize(e->stream), "status sync");
+  return SPLATT_SUCCESS;
+}
+
+/* SPLATT_ERROR_NUMERIC when a solve found no positive pivot even with the
+ * largest ridge (the factors and fits are then not finite) */
+static int status_rc(int st)
+{
+  if (st >= 2) {
+    adatm_set_error("CP-ALS: the normal equations (hadamard of Grams) had no positive "
+                    "pivot even with a ridge of 1e-6 of their largest diagonal");
+    return SPLATT_ERROR_NUMERIC;
+  }
   return SPLATT_SUCCESS;
 }
 
@@ -1543,15 +1568,25 @@ static int graph_build(adatm_engine *e)
   return SPLATT_SUCCESS;
 }
 
+/* one ALS sweep; its fit goes to fits[nsweep], so no sweep is launched once
+ * the buffer is full (adatm_engine_cpd_begin starts over) */
 static int sweep_launch(adatm_engine *e, int use_graph)
 {
+  if (e->nsweep >= e->fits_cap) {
+    adatm_set_error("engine: %u sweeps since the last CP-ALS start fill the fits "
+                    "buffer; start again (adatm_engine_cpd_begin)", e->nsweep);
+    return SPLATT_ERROR_BADINPUT;
+  }
   if (use_graph) {
     int const rc = graph_build(e);
     if (rc != SPLATT_SUCCESS) return rc;
     HIPCHK(hipGraphLaunch(e->gexec, e->stream), "graph launch");
-    return SPLATT_SUCCESS;
+  } else {
+    int const rc = sweep_enqueue(e);
+    if (rc != SPLATT_SUCCESS) return rc;
   }
-  return sweep_enqueue(e);
+  e->nsweep++;
+  return SPLATT_SUCCESS;
 }
 
 int adatm_engine_cpd(adatm_engine *e, uint32_t niters, double tol,
@@ -1583,7 +1618,9 @@ int adatm_engine_cpd(adatm_engine *e, uint32_t niters, double tol,
                      hipMemcpyDeviceToHost),
            "fits download");
   if (iters_done) *iters_done = done;
-  return SPLATT_SUCCESS;
+  int st = 0;
+  rc = read_status(e, &st);
+  return rc != SPLATT_SUCCESS ? rc : status_rc(st);
 }
 
 /* ---- timing ----------------------------------------------------------------- */
@@ -1646,11 +1683,16 @@ int adatm_engine_time(adatm_engine *e, int what, int nwarm, int nrep,
     /* one instrumented, non-graph pass: per-class device time and bytes */
     double tsum[KC_NCLASS] = {0}, bsum[KC_NCLASS] = {0};
     int cnt[KC_NCLASS] = {0};
+    if (what == 1 && e->nsweep >= e->fits_cap) {
+      rc = cpd_prepare(e); /* the pass writes fits[nsweep] */
+      if (rc != SPLATT_SUCCESS) return rc;
+    }
     e->instrument = 1;
     e->nev = 0;
     rc = (what == 0) ? seq_enqueue(e) : sweep_enqueue(e);
     e->instrument = 0;
     if (rc != SPLATT_SUCCESS) return rc;
+    if (what == 1) e->nsweep++;
     HIPCHK(hipStreamSynchronize(e->stream), "instrument sync");
     for (int i = 0; i < e->nev; ++i) {
       float ms = 0.f;
@@ -1780,6 +1822,10 @@ int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
   HIPCHK(hipSetDevice(e->device), "hipSetDevice");
   int rc;
   if (what == 1) {
+    if (nsteps >= e->fits_cap) {
+      adatm_set_error("engine: at most %u timed sweeps per call", e->fits_cap - 1);
+      return SPLATT_ERROR_BADINPUT;
+    }
     rc = cpd_prepare(e);
     if (rc != SPLATT_SUCCESS) return rc;
     if (use_graph) {
@@ -1797,10 +1843,8 @@ int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
   }
   hipEventRecord(a, e->stream);
   rc = SPLATT_SUCCESS;
-  for (uint32_t i = 0; i < nsteps && rc == SPLATT_SUCCESS; ++i) {
-    if (what == 1 && i + 1 >= e->fits_cap) break;
+  for (uint32_t i = 0; i < nsteps && rc == SPLATT_SUCCESS; ++i)
     rc = (what == 0) ? seq_enqueue(e) : sweep_launch(e, use_graph);
-  }
   hipEventRecord(b, e->stream);
   hipError_t const s = hipEventSynchronize(b);
   float ms = 0.f;
@@ -1837,18 +1881,33 @@ int adatm_engine_cpd_begin(adatm_engine *e)
   return SPLATT_SUCCESS;
 }
 
+/* `it` must be the number of sweeps since adatm_engine_cpd_begin: the fit
+ * returned is the one this sweep writes (fits[it]) */
 int adatm_engine_cpd_sweep(adatm_engine *e, uint32_t it, int use_graph,
                            double *fit)
 {
-  if (!e || it >= e->fits_cap) return SPLATT_ERROR_BADINPUT;
+  if (!e || !fit) return SPLATT_ERROR_BADINPUT;
+  if (it != e->nsweep) {
+    adatm_set_error("engine: sweep %u requested, %u sweeps run since the last "
+                    "CP-ALS start", it, e->nsweep);
+    return SPLATT_ERROR_BADINPUT;
+  }
   HIPCHK(hipSetDevice(e->device), "hipSetDevice");
-  int const rc = sweep_launch(e, use_graph);
+  int rc = sweep_launch(e, use_graph);
   if (rc != SPLATT_SUCCESS) return rc;
-  HIPCHK(hipMemcpyAsync(fit, e->fits + it, sizeof(double),
+  HIPCHK(hipMemcpyAsync(fit, e->fits + (e->nsweep - 1), sizeof(double),
                         hipMemcpyDeviceToHost, e->stream),
          "fit download");
-  HIPCHK(hipStreamSynchronize(e->stream), "sweep sync");
-  return SPLATT_SUCCESS;
+  int st = 0;
+  rc = read_status(e, &st); /* synchronises the stream */
+  return rc != SPLATT_SUCCESS ? rc : status_rc(st);
+}
+
+int adatm_engine_solve_status(adatm_engine *e, int *status)
+{
+  if (!e || !status) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(e->device), "hipSetDevice");
+  return read_status(e, status);
 }
 
 uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,
@@ -1876,18 +1935,19 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
  * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
  * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
  * reduction tile, then (when the scatter tile moved) the reduction tile
- * again with the new scatter tile, each a fresh engine timed on nrep CP-ALS
- * sweeps (hipGraph, median); then, on the best engine, the default batch
- * against the children per batch over {4, 8, 16}, the pipelined 4 and 8 and
- * the wide 4 and 8.  The defaults are timed first and a candidate displaces the best
- * so far only when it is more than TUNE_MARGIN faster: near-equal choices
- * differ by run-to-run noise, and without the margin the choice (and with it
- * the kernel that dominates the sweep) flips between runs. */
+ * again with the new scatter tile; then, with the best tiles, the default
+ * batch against the children per batch over {4, 8, 16}, the pipelined 4 and
+ * 8 and the wide 4 and 8.  The defaults are timed first and a candidate
+ * displaces the best so far only when it is more than TUNE_MARGIN faster:
+ * near-equal choices differ by run-to-run noise, and without the margin the
+ * choice (and with it the kernel that dominates the sweep) flips between
+ * runs.  The order of the calls to `timer` is part of the contract (the
+ * unit tests drive it with a table of timings). */
 #define TUNE_MARGIN 0.99
-int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
-                         int precision, int device, int nrep, uint32_t *red_tile,
-                         uint32_t *scat_tile, uint32_t *batch, double *best_ms)
+int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
+                          uint32_t *scat_tile, uint32_t *batch, double *best_ms)
 {
+  if (!timer || !red_tile || !scat_tile) return SPLATT_ERROR_BADINPUT;
   /* defaults first */
   static uint32_t const RT[5] = {RED_TILE, 32, 64, 256, 512},
                         ST[5] = {SCAT_TILE, 16, 32, 128, 256};
@@ -1904,13 +1964,8 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
       uint32_t const sc = pass == 0 ? SCAT_TILE : (pass == 1 ? ST[i] : bs);
       if (pass == 1 && sc == SCAT_TILE) continue; /* timed in the first pass */
       if (pass == 2 && r == r1) continue;         /* timed in the second pass */
-      adatm_engine *e = NULL;
-      int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, r, sc, &e);
-      if (rc != SPLATT_SUCCESS) return rc;
       double ms = INFINITY;
-      rc = adatm_engine_init_factors(e, 1);
-      if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
-      adatm_engine_destroy(e);
+      int const rc = timer(ctx, r, sc, 0, &ms);
       if (rc != SPLATT_SUCCESS) return rc;
       if (ms < bt * TUNE_MARGIN) {
         bt = ms;
@@ -1919,26 +1974,18 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
       }
     }
   }
+  static uint32_t const BT[7] = {4, 8, 16, 4 | ADATM_BATCH_PIPELINED,
+                                 8 | ADATM_BATCH_PIPELINED, 4 | ADATM_BATCH_WIDE,
+                                 8 | ADATM_BATCH_WIDE};
   uint32_t bb = 0;
-  {
-    adatm_engine *e = NULL;
-    int rc = adatm_engine_create_tiled(t, rank, plan, precision, device, br, bs, &e);
+  for (int i = 0; i < 7; ++i) {
+    double ms = INFINITY;
+    int const rc = timer(ctx, br, bs, BT[i], &ms);
     if (rc != SPLATT_SUCCESS) return rc;
-    rc = adatm_engine_init_factors(e, 1);
-    static uint32_t const BT[7] = {4, 8, 16, 4 | ADATM_BATCH_PIPELINED,
-                                   8 | ADATM_BATCH_PIPELINED, 4 | ADATM_BATCH_WIDE,
-                                   8 | ADATM_BATCH_WIDE};
-    for (int i = 0; rc == SPLATT_SUCCESS && i < 7; ++i) {
-      double ms = INFINITY;
-      rc = adatm_engine_set_batch(e, BT[i]);
-      if (rc == SPLATT_SUCCESS) rc = adatm_engine_time(e, 1, 1, nrep, 1, &ms, NULL);
-      if (rc == SPLATT_SUCCESS && ms < bt * TUNE_MARGIN) {
-        bt = ms;
-        bb = BT[i];
-      }
+    if (ms < bt * TUNE_MARGIN) {
+      bt = ms;
+      bb = BT[i];
     }
-    adatm_engine_destroy(e);
-    if (rc != SPLATT_SUCCESS) return rc;
   }
   *red_tile = br;
   *scat_tile = bs;
@@ -1947,11 +1994,59 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
   return SPLATT_SUCCESS;
 }
 
+/* the device timer of adatm_autotune_tiles: CP-ALS sweeps of an engine
+ * built with the tiles (kept while only the batch changes) */
+typedef struct {
+  adatm_coo const *t;
+  uint64_t rank;
+  adatm_plan const *plan;
+  int precision, device, nrep;
+  adatm_engine *e;
+  uint32_t red, scat;
+} tune_ctx;
+
+static int tune_time(void *vctx, uint32_t red, uint32_t scat, uint32_t batch, double *ms)
+{
+  tune_ctx *c = (tune_ctx *)vctx;
+  if (!c->e || c->red != red || c->scat != scat) {
+    adatm_engine_destroy(c->e);
+    c->e = NULL;
+    int rc = adatm_engine_create_tiled(c->t, c->rank, c->plan, c->precision, c->device,
+                                       red, scat, &c->e);
+    if (rc == SPLATT_SUCCESS) rc = adatm_engine_init_factors(c->e, 1);
+    if (rc != SPLATT_SUCCESS) return rc;
+    c->red = red;
+    c->scat = scat;
+  }
+  int const rc = adatm_engine_set_batch(c->e, batch);
+  if (rc != SPLATT_SUCCESS) return rc;
+  return adatm_engine_time(c->e, 1, 1, c->nrep, 1, ms, NULL);
+}
+
+int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                         int precision, int device, int nrep, uint32_t *red_tile,
+                         uint32_t *scat_tile, uint32_t *batch, double *best_ms)
+{
+  tune_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.t = t;
+  c.rank = rank;
+  c.plan = plan;
+  c.precision = precision;
+  c.device = device;
+  c.nrep = nrep;
+  int const rc = adatm_autotune_search(tune_time, &c, red_tile, scat_tile, batch, best_ms);
+  adatm_engine_destroy(c.e);
+  return rc;
+}
+
 /* ======================================================================== *
  * Batched CP-ALS: many independent engines on one device, every instance's
- * ALS sweep captured in one hipGraph.  Each instance runs on its own stream,
- * forked from and joined to the batch stream inside the capture, so the
- * graph's branches are independent and small instances run side by side.
+ * ALS sweep captured in one hipGraph.  Each instance runs on a branch stream
+ * of the batch (not the engine's own stream, which never joins the batch's
+ * capture), forked from and joined to the batch stream inside the capture,
+ * so the graph's branches are independent and small instances run side by
+ * side.
  * ======================================================================== */
 /* one round of the grouped sweep: every instance with more than r modes runs
  * its mode-r MTTKRP on its own stream, then one grouped dense update */
@@ -1965,6 +2060,7 @@ struct adatm_batch {
   int n, device;
   adatm_engine **e;
   hipStream_t s;
+  hipStream_t *bs; /* branch stream of each instance */
   hipEvent_t fork, *join;
   hipGraph_t graph;
   hipGraphExec_t gexec;
@@ -1989,6 +2085,10 @@ void adatm_batch_destroy(adatm_batch *b)
     for (int i = 0; i < b->n; ++i)
       if (b->join[i]) hipEventDestroy(b->join[i]);
   if (b->fork) hipEventDestroy(b->fork);
+  if (b->bs)
+    for (int i = 0; i < b->n; ++i)
+      if (b->bs[i]) hipStreamDestroy(b->bs[i]);
+  free(b->bs);
   if (b->s) hipStreamDestroy(b->s);
   if (b->ddesc) hipFree(b->ddesc);
   if (b->dmap) hipFree(b->dmap);
@@ -2085,6 +2185,7 @@ static int batch_rounds(adatm_batch *b)
     x->iter = e->iter;
     x->ttnormsq = e->ttnormsq;
     x->fits = e->fits;
+    x->fcap = e->fits_cap;
     x->nin = adatm_k_fit_blocks(x->I);
     x->in0 = b->nb_in;
     b->nb_in += x->nin;
@@ -2112,23 +2213,36 @@ static int batch_rounds(adatm_batch *b)
   return rc;
 }
 
-/* the grouped sweep: per round, the instances' MTTKRPs forked on their own
- * streams, then one grouped dense update on the batch stream; one grouped
- * fit last */
+/* instance i's work (its MTTKRP of mode `mode`, or with mode < 0 its whole
+ * sweep) enqueued on its branch stream, forked from and joined to the batch
+ * stream; the engine's launches go to the branch stream while enqueued */
+static int batch_branch(adatm_batch *b, int i, int mode)
+{
+  adatm_engine *e = b->e[i];
+  if (hipStreamWaitEvent(b->bs[i], b->fork, 0) != hipSuccess) return SPLATT_ERROR_DEVICE;
+  hipStream_t const own = e->stream;
+  e->stream = b->bs[i];
+  int const rc = (mode < 0) ? sweep_enqueue(e) : mttkrp_dev(e, (uint32_t)mode);
+  e->stream = own;
+  if (rc != SPLATT_SUCCESS) return rc;
+  if (hipEventRecord(b->join[i], b->bs[i]) != hipSuccess ||
+      hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess)
+    return SPLATT_ERROR_DEVICE;
+  return SPLATT_SUCCESS;
+}
+
+/* the grouped sweep: per round, the instances' MTTKRPs forked on their
+ * branch streams, then one grouped dense update on the batch stream; one
+ * grouped fit last */
 static int batch_enqueue_grouped(adatm_batch *b)
 {
   for (uint32_t r = 0; r < b->nround; ++r) {
     batch_round const *B = b->round + r;
     if (hipEventRecord(b->fork, b->s) != hipSuccess) return SPLATT_ERROR_DEVICE;
     for (int i = 0; i < b->n; ++i) {
-      adatm_engine *e = b->e[i];
-      if (e->N <= r) continue;
-      if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) return SPLATT_ERROR_DEVICE;
-      int const rc = mttkrp_dev(e, r);
+      if (b->e[i]->N <= r) continue;
+      int const rc = batch_branch(b, i, (int)r);
       if (rc != SPLATT_SUCCESS) return rc;
-      if (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
-          hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess)
-        return SPLATT_ERROR_DEVICE;
     }
     KCHK(adatm_k_dense_grouped(b->s, b->e[0]->prec, b->ddesc + B->desc0, B->n, B->Rmax,
                                 b->dmap + B->mm_off, B->nb_mm, b->dmap + B->gr_off,
@@ -2156,7 +2270,8 @@ int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
   b->device = engines[0]->device;
   b->e = (adatm_engine **)malloc((size_t)n * sizeof(adatm_engine *));
   b->join = (hipEvent_t *)calloc((size_t)n, sizeof(hipEvent_t));
-  if (!b->e || !b->join) {
+  b->bs = (hipStream_t *)calloc((size_t)n, sizeof(hipStream_t));
+  if (!b->e || !b->join || !b->bs) {
     adatm_batch_destroy(b);
     return SPLATT_ERROR_NOMEMORY;
   }
@@ -2171,8 +2286,9 @@ int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
       hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
     rc = hip_fail(hipGetLastError(), "batch stream");
   for (int i = 0; rc == SPLATT_SUCCESS && i < n; ++i)
-    if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess)
-      rc = hip_fail(hipGetLastError(), "batch event");
+    if (hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->bs[i], hipStreamNonBlocking) != hipSuccess)
+      rc = hip_fail(hipGetLastError(), "batch event / branch stream");
   if (rc != SPLATT_SUCCESS) {
     adatm_batch_destroy(b);
     return rc;
@@ -2194,18 +2310,8 @@ static int batch_graph(adatm_batch *b)
     rc = batch_enqueue_grouped(b);
   else if (hipEventRecord(b->fork, b->s) != hipSuccess)
     rc = SPLATT_ERROR_DEVICE;
-  for (int i = 0; !b->grouped && rc == SPLATT_SUCCESS && i < b->n; ++i) {
-    adatm_engine *e = b->e[i];
-    if (hipStreamWaitEvent(e->stream, b->fork, 0) != hipSuccess) {
-      rc = SPLATT_ERROR_DEVICE;
-      break;
-    }
-    rc = sweep_enqueue(e);
-    if (rc == SPLATT_SUCCESS &&
-        (hipEventRecord(b->join[i], e->stream) != hipSuccess ||
-         hipStreamWaitEvent(b->s, b->join[i], 0) != hipSuccess))
-      rc = SPLATT_ERROR_DEVICE;
-  }
+  for (int i = 0; !b->grouped && rc == SPLATT_SUCCESS && i < b->n; ++i)
+    rc = batch_branch(b, i, -1);
   hipGraph_t g = NULL;
   hipError_t const ec = hipStreamEndCapture(b->s, &g);
   if (rc != SPLATT_SUCCESS || ec != hipSuccess) {
@@ -2261,13 +2367,21 @@ int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *f
   hipEventDestroy(t1);
   if (rc != SPLATT_SUCCESS) return rc;
   HIPCHK(es, "batch sync");
+  for (int i = 0; i < b->n; ++i) b->e[i]->nsweep = niters; /* the device counters */
   if (ms_total) *ms_total = ms;
   if (fits)
     for (int i = 0; i < b->n; ++i)
       HIPCHK(hipMemcpy(fits + (size_t)i * niters, b->e[i]->fits, niters * sizeof(double),
                        hipMemcpyDeviceToHost),
              "batch fits");
-  return SPLATT_SUCCESS;
+  int worst = 0;
+  for (int i = 0; i < b->n; ++i) {
+    int st = 0;
+    rc = read_status(b->e[i], &st);
+    if (rc != SPLATT_SUCCESS) return rc;
+    if (st > worst) worst = st;
+  }
+  return status_rc(worst);
 }
 
 /* The unbatched comparison: each instance's niters sweeps (its own graph)

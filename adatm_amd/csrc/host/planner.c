@@ -6,8 +6,8 @@
  * decide_candidate_configs, predict_candidates_adaptive,
  * limit_candidates_by_space, choose_candidates_by_strategy,
  * decide_parameters_auto), src/adatm_rcsf.c:329-406 (predictors),
- * src/adatm_sort.c:10-75 (pair_sort), src/adatm_utils.c:108-173 (argmin /
- * array_min_range) and src/mttkrp.c:1940-1974 (decide_use_csfs, the
+ * src/adatm_sort.c:10-75 (pair_sort), src/adatm_utils.c:33-51 and 80-98
+ * (argmin_elem_range / array_min_range) and src/mttkrp.c:1940-1974 (decide_use_csfs, the
  * TIME_EFFICIENT branch).  The steps are kept identical because the chosen
  * plan, including tie orders of the quicksort, is part of the parity
  * contract.  Deliberate differences:
@@ -65,7 +65,7 @@ void pair_sort(pair *const pair_list, splatt_idx_t const size)
   if (size > 1) pair_qsort(pair_list, 0, (size_t)size);
 }
 
-/* first index holding the smallest value > con (adatm_utils.c:108-126) */
+/* first index holding the smallest value > con (adatm_utils.c:33-51) */
 splatt_idx_t argmin_elem_range(splatt_idx_t const *const arr,
                                splatt_idx_t const N, splatt_idx_t const con)
 {
@@ -75,7 +75,8 @@ splatt_idx_t argmin_elem_range(splatt_idx_t const *const arr,
   return best;
 }
 
-/* smallest value > con, 0 when there is none (adatm_utils.c:155-173) */
+/* smallest value > con, 0 when there is none (adatm_utils.c:80-98; the
+ * reference leaves the result uninitialised then, D3) */
 splatt_idx_t array_min_range(splatt_idx_t const *const arr,
                              splatt_idx_t const N, splatt_idx_t const con)
 {

@@ -234,6 +234,18 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch);
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
                          uint32_t *scat_tile, uint32_t *batch, double *best_ms);
+/* The autotuner's search over any timer: timer(ctx, red_tile, scat_tile,
+ * batch, &ms) times one candidate (batch as adatm_engine_set_batch) and
+ * returns SPLATT_SUCCESS; the reduction tiles {128, 32, 64, 256, 512} (scatter
+ * tile 64), the scatter tiles {16, 32, 128, 256} (best reduction tile), the
+ * reduction tiles again when the scatter tile moved, then the batches {4, 8,
+ * 16, 4|PIPELINED, 8|PIPELINED, 4|WIDE, 8|WIDE} on the best tiles, in that
+ * order; a candidate wins only when more than 1% faster than the best so far.
+ * adatm_autotune_tiles is this search over device CP-ALS sweeps. */
+typedef int (*adatm_tile_timer)(void *ctx, uint32_t red_tile, uint32_t scat_tile,
+                                uint32_t batch, double *ms);
+int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
+                          uint32_t *scat_tile, uint32_t *batch, double *best_ms);
 void adatm_engine_destroy(adatm_engine *e);
 
 /* Factor matrices are exchanged in FP64 row-major (dims[m] x rank). */
@@ -281,11 +293,23 @@ int adatm_engine_launches(adatm_engine *e, int max, int *cls, int *info,
 int adatm_engine_run_timed(adatm_engine *e, int what, uint32_t nsteps,
                            int use_graph, double *ms_total);
 
-/* Stepwise CP-ALS: begin (Grams of the current factors, counter reset),
- * then one synchronous sweep at a time returning its fit. */
+/* Stepwise CP-ALS: begin (Grams of the current factors, sweep counter and
+ * solve status reset), then one synchronous sweep at a time returning its
+ * fit.  `it` must equal the number of sweeps run since the begin (0, 1, ...);
+ * at most 4096 sweeps per begin.  Returns SPLATT_ERROR_NUMERIC when a solve
+ * found no positive pivot (see adatm_engine_solve_status). */
 int adatm_engine_cpd_begin(adatm_engine *e);
 int adatm_engine_cpd_sweep(adatm_engine *e, uint32_t it, int use_graph,
                            double *fit);
+
+/* Worst status of the (hadamard of Grams)^-1 solves since the last CP-ALS
+ * start: 0 every pivot positive (the reference's unregularised solve);
+ * 1 some solve needed a ridge (1e-10, then 1e-6, of the largest diagonal:
+ * the normal equations were not numerically positive definite, and the
+ * result departs from an unregularised solve); 2 not even then (factors and
+ * fit are not finite; adatm_engine_cpd and the sweeps return
+ * SPLATT_ERROR_NUMERIC). */
+int adatm_engine_solve_status(adatm_engine *e, int *status);
 
 /* Device bytes of a group's memo at a CSF level (0 when not memoised). */
 uint64_t adatm_engine_memo_bytes(adatm_engine const *e, uint32_t grp,

@@ -83,7 +83,8 @@ typedef enum {
   SPLATT_SUCCESS = 1,
   SPLATT_ERROR_BADINPUT,
   SPLATT_ERROR_NOMEMORY,
-  SPLATT_ERROR_DEVICE /* HIP runtime error on the GPU path */
+  SPLATT_ERROR_DEVICE, /* HIP runtime error on the GPU path */
+  SPLATT_ERROR_NUMERIC /* CP-ALS normal equations not solvable (no positive pivot) */
 } splatt_error_type;
 
 typedef enum {

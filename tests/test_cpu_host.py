@@ -77,6 +77,7 @@ def _declared_functions():
     for h in (ROOT / "include").glob("*.h"):
         txt = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         txt = re.sub(r"//.*", "", txt)
+        txt = re.sub(r"typedef[^;]*;", "", txt)  # function-pointer and struct typedefs
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
             name = m.group(1)
             if name in ("if", "while", "for", "return", "sizeof", "defined"):
@@ -118,3 +119,91 @@ def test_engine_rejects_bad_arguments_before_any_device_call(lib):
     tb = A.Tensor.synth(big)
     rc = L.adatm_engine_create(C.byref(tb.c), 64, C.byref(plan), A.FP32, 0, C.byref(h))
     assert rc == 2 and b"2^32" in L.adatm_last_error()
+
+
+def _bf16_rne(u: int) -> int:
+    """Nearest-even BF16 of the FP32 with bits u, decided by exact rational
+    comparison of the two neighbouring BF16 values (finite inputs)."""
+    from fractions import Fraction
+    import struct
+
+    def val(bits32):
+        return Fraction(struct.unpack("<f", struct.pack("<I", bits32))[0])
+
+    lo = u & 0xFFFF0000
+    hi = (lo + 0x10000) & 0xFFFFFFFF
+    x = val(u)
+    if (hi & 0x7F800000) == 0x7F800000:  # rounding up overflows to infinity
+        top = val(lo) + (val(lo) - val(lo - 0x10000))
+        d_lo, d_hi = abs(x - val(lo)), abs(top - x)
+    else:
+        d_lo, d_hi = abs(x - val(lo)), abs(val(hi) - x)
+    if d_lo < d_hi or (d_lo == d_hi and ((lo >> 16) & 1) == 0):
+        return lo >> 16
+    return (hi >> 16) & 0xFFFF
+
+
+def test_bf16_rounding_bits(lib):
+    """The BF16 factor-row rounding: nearest-even on random finite values
+    (exact rational reference), infinities kept, every NaN (also one whose
+    payload sits in the low 16 mantissa bits only) a quiet NaN of its sign."""
+    import struct
+
+    def bits(u):
+        return A.bf16_bits(struct.unpack("<f", struct.pack("<I", u))[0])
+
+    rng = np.random.default_rng(5)
+    us = list(rng.integers(0, 0x7F7FFFFF, 3000)) + [0x3F808000, 0x3F818000, 0x3F80FFFF,
+                                                   0x00000001, 0x7F7FFFFF, 0]
+    for u in us:
+        for sign in (0, 0x80000000):
+            w = int(u) | sign
+            assert bits(w) == (_bf16_rne(int(u)) | (sign >> 16)), hex(w)
+    assert bits(0x7F800000) == 0x7F80 and bits(0xFF800000) == 0xFF80
+    for nan in (0x7F800001, 0x7F80FFFF, 0x7FC00000, 0x7FFFFFFF):
+        assert bits(nan) & 0x7FFF > 0x7F80 and bits(nan) & 0x0040 and not bits(nan) & 0x8000
+        assert bits(nan | 0x80000000) & 0x8000 and bits(nan | 0x80000000) & 0x7FFF > 0x7F80
+
+
+def test_autotune_search_order_and_margin(lib):
+    """The autotuner's search on a table of timings: the default tiles are
+    timed first; a candidate wins only when > 1% faster; when the scatter
+    tile moves, the reduction tiles are timed again with it (skipping the one
+    already timed); the batches are timed last on the best tiles."""
+    P, W = A.BATCH_PIPELINED, A.BATCH_WIDE
+    table = {(128, 64, 0): 10.0, (32, 64, 0): 9.95,  # within 1%: not taken
+             (64, 64, 0): 9.0, (256, 64, 0): 11.0, (512, 64, 0): 12.0,
+             (64, 16, 0): 9.5, (64, 32, 0): 8.0, (64, 128, 0): 8.5, (64, 256, 0): 9.0,
+             (128, 32, 0): 8.05, (32, 32, 0): 7.0, (256, 32, 0): 7.0, (512, 32, 0): 9.0}
+    calls = []
+
+    def timer(r, s, b):
+        calls.append((r, s, b))
+        return table.get((r, s, b), 7.0 if b == 8 | W else 6.97)
+
+    (r, s, b), ms = A.autotune_search(timer)
+    assert calls[:5] == [(128, 64, 0), (32, 64, 0), (64, 64, 0), (256, 64, 0), (512, 64, 0)]
+    assert calls[5:9] == [(64, 16, 0), (64, 32, 0), (64, 128, 0), (64, 256, 0)]
+    # the scatter tile moved to 32: every reduction tile but 64 again with it
+    assert calls[9:13] == [(128, 32, 0), (32, 32, 0), (256, 32, 0), (512, 32, 0)]
+    batches = [4, 8, 16, 4 | P, 8 | P, 4 | W, 8 | W]
+    assert calls[13:] == [(32, 32, bt) for bt in batches]
+    # 256/32 ties 32/32 (not 1% faster); every batch within 1% of 7.0 loses
+    assert (r, s, b) == (32, 32, 0) and ms == 7.0
+    # the scatter tile stays at its default: no third pass; a batch wins
+    calls.clear()
+    (r, s, b), ms = A.autotune_search(lambda r, s, b: calls.append((r, s, b)) or
+                                      (5.0 if (r, s, b) == (128, 64, 8) else 10.0))
+    assert len(calls) == 5 + 4 + 7
+    assert (r, s, b) == (128, 64, 8) and ms == 5.0
+
+
+def test_stepwise_cpd_refuses_bad_arguments(lib):
+    """cpd_sweep and solve_status check their arguments on the host."""
+    import ctypes as C
+    L = A.lib()
+    fit = C.c_double()
+    assert L.adatm_engine_cpd_sweep(None, 0, 0, C.byref(fit)) == A.SPLATT_ERROR_BADINPUT
+    assert L.adatm_engine_cpd_begin(None) == A.SPLATT_ERROR_BADINPUT
+    st = C.c_int()
+    assert L.adatm_engine_solve_status(None, C.byref(st)) == A.SPLATT_ERROR_BADINPUT

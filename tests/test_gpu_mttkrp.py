@@ -220,10 +220,14 @@ def test_every_batch_every_plan(prec, R):
         e.close()
 
 
-def test_batched_cpd_matches_each_instance():
+@pytest.mark.parametrize("precs", [(A.FP32,) * 4, (A.FP64,) * 4,
+                                   (A.FP32, A.FP64, A.FP32, A.FP64)])
+def test_batched_cpd_matches_each_instance(precs):
     """Uneven instances (orders 4-6, ranks 8/16/32) swept together in one
-    hipGraph: every replay's fits equal each engine's own eager sweeps."""
-    print("batched cpd", flush=True)
+    hipGraph: every replay's fits equal each engine's own eager sweeps.  One
+    precision runs the grouped dense updates (FP32: MFMA Grams; FP64: the
+    FP64 Gram, product and fit), mixed precisions one branch per instance."""
+    print(f"batched cpd precisions={precs}", flush=True)
     specs = [([60, 50, 40, 30], 8000, A.DIST_UNIFORM, 8),
              ([80, 70, 60, 50, 40], 12000, A.DIST_ZIPF, 16),
              ([40, 35, 30, 25, 20, 15], 6000, A.DIST_UNIFORM, 32),
@@ -233,7 +237,7 @@ def test_batched_cpd_matches_each_instance():
         t = _tensor(dims, nd, dist, seed=60 + k)
         cands, ch = A.plan_analytic(t, R)
         for lst in (eng, solo):
-            e = A.Engine(t, R, cands[ch], precision=A.FP32)
+            e = A.Engine(t, R, cands[ch], precision=precs[k])
             e.init_factors(7 + k)
             lst.append(e)
     b = A.Batch(eng)
@@ -334,3 +338,65 @@ def test_degenerate_tensors(case, prec):
             err = O.rel_fro(e.mttkrp(m), ref[m])
             assert err <= TOL[prec], (name, plan.groups(t.nmodes), m, err)
         e.close()
+
+
+def test_solve_status_reports_a_ridge():
+    """Factors whose columns are all equal make every hadamard of Grams
+    exactly singular (powers of two keep the elimination exact), so the first
+    solve finds a zero pivot and solves with a ridge: the host reads status 1
+    and the fits stay finite.  A well-posed run reads status 0."""
+    print("solve status", flush=True)
+    I, J, K, R = 8, 4, 16, 4
+    g = np.stack(np.meshgrid(np.arange(I), np.arange(J), np.arange(K), indexing="ij"))
+    inds = g.reshape(3, -1)
+    vals = np.random.default_rng(1).random(inds.shape[1]) + 0.5
+    t = A.Tensor.from_arrays(inds, vals, [I, J, K])
+    t.canonicalize()
+    assert t.dims == [I, J, K]
+    plan = A.plan_candidates(3, t.dims)[0]
+    e = A.Engine(t, R, plan, precision=A.FP64)
+    for m, d in enumerate(t.dims):
+        e.set_factor(m, np.ones((d, R)))
+    e.cpd_begin()
+    fits = [e.cpd_sweep(it) for it in range(3)]
+    assert e.solve_status() == 1
+    assert np.all(np.isfinite(fits)), fits
+    e.close()
+    t2 = _tensor([50, 40, 30, 20], 20000, A.DIST_UNIFORM, seed=21)
+    cands, ch = A.plan_analytic(t2, 8)
+    e2 = A.Engine(t2, 8, cands[ch], precision=A.FP32)
+    e2.init_factors(3)
+    e2.cpd(4, graph=True)
+    assert e2.solve_status() == 0
+
+
+def test_stepwise_sweeps_follow_the_device_counter():
+    """cpd_sweep returns the fit its own sweep wrote: the sweep index must be
+    the number of sweeps since the start, CP-ALS runs continue the count, and
+    no sweep runs once the fits buffer (4096) is full."""
+    print("sweep counter", flush=True)
+    t = _tensor([40, 30, 20, 10], 5000, A.DIST_UNIFORM, seed=33)
+    cands, ch = A.plan_analytic(t, 8)
+    e = A.Engine(t, 8, cands[ch], precision=A.FP64)
+    e.init_factors(5)
+    e.cpd_begin()
+    with pytest.raises(A.AdatmError):
+        e.cpd_sweep(1)
+    f0 = e.cpd_sweep(0)
+    f1 = e.cpd_sweep(1, graph=True)
+    assert np.allclose(e.fits(2), [f0, f1], rtol=0, atol=0)
+    e.init_factors(5)
+    fits = e.cpd(3, graph=True)  # starts over: sweeps 0..2
+    assert fits[0] == f0 and fits[1] == f1
+    f3 = e.cpd_sweep(3)  # continues the run
+    assert f3 == e.fits(4)[3]
+    with pytest.raises(A.AdatmError):
+        e.cpd_sweep(0)
+    with pytest.raises(A.AdatmError):
+        e.run_timed(1, 4096, graph=True)
+    e.cpd_begin()
+    for it in range(4096):
+        e.cpd_sweep(it, graph=True)
+    with pytest.raises(A.AdatmError):
+        e.cpd_sweep(4096, graph=True)
+    e.close()
