@@ -235,6 +235,15 @@ int adatm_k_adam(hipStream_t s, float *p, float const *g, float *m, float *v,
                  uint32_t n, float lr, float b1, float b2, float eps, float c1,
                  float c2);
 
+/* host-side pieces of the cost-model network shared with the allocation-free
+ * prediction (learned.c): the forward pass over caller-placed device
+ * buffers, and the network's device, stream and standardisation */
+struct adatm_mlp;
+int adatm_mlp_forward_dev(struct adatm_mlp *M, float const *x, float *const *Z,
+                          float *const *H, uint32_t B);
+void adatm_mlp_dev_info(struct adatm_mlp const *M, int *device, hipStream_t *stream,
+                        float *xmean, float *xstd, float *ymean, float *ystd);
+
 #ifdef __cplusplus
 }
 #endif

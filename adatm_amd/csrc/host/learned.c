@@ -205,17 +205,38 @@ static int batch_alloc(mlp_batch *b, uint32_t B)
   return SPLATT_SUCCESS;
 }
 
-static int forward(adatm_mlp *M, mlp_batch *b, uint32_t B)
+/* forward pass of B standardised rows x (device, B x NF) on the network's
+ * stream: Z[l] / H[l] (device, B x LOUT[l]) receive every layer's
+ * pre-activations and activations; the prediction is H[3] */
+int adatm_mlp_forward_dev(adatm_mlp *M, float const *x, float *const *Z,
+                          float *const *H, uint32_t B)
 {
-  float const *in = b->x;
+  float const *in = x;
   for (int l = 0; l < 4; ++l) {
     size_t const o = woff(l);
     if (adatm_k_dense_fwd(M->s, in, M->p + o, M->p + o + (size_t)LIN[l] * LOUT[l],
-                          b->Z[l], b->H[l], B, LIN[l], LOUT[l], l < 3))
+                          Z[l], H[l], B, LIN[l], LOUT[l], l < 3))
       return merr("forward");
-    in = b->H[l];
+    in = H[l];
   }
   return SPLATT_SUCCESS;
+}
+
+/* the network's device, stream and standardisation */
+void adatm_mlp_dev_info(adatm_mlp const *M, int *device, hipStream_t *stream,
+                        float *xmean, float *xstd, float *ymean, float *ystd)
+{
+  *device = M->device;
+  *stream = M->s;
+  memcpy(xmean, M->xmean, sizeof(M->xmean));
+  memcpy(xstd, M->xstd, sizeof(M->xstd));
+  *ymean = M->ymean;
+  *ystd = M->ystd;
+}
+
+static int forward(adatm_mlp *M, mlp_batch *b, uint32_t B)
+{
+  return adatm_mlp_forward_dev(M, b->x, b->Z, b->H, B);
 }
 
 /* gradient of the batch MSE into M->g; *loss (nullable) = mean squared error */

@@ -170,6 +170,16 @@ int adatm_mlp_train(adatm_mlp *m, float const *X, float const *y, uint32_t n,
                     uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
                     float *loss_hist);
 int adatm_mlp_predict(adatm_mlp *m, float const *X, uint32_t n, float *y);
+/* adatm_mlp_predict without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code around HIP graph
+ * capture, tight planning loops).  The activations live in a device array of
+ * the library sized for ADATM_NOALLOC_MAX_ROWS rows (6.7 MB, held by every
+ * GPU the library runs on until the process ends); larger n is rejected with
+ * SPLATT_ERROR_BADINPUT.  Calls on one GPU share that array: overlapping
+ * calls (several host threads, one GPU) race; calls in sequence are safe (a
+ * call returns after its work finished).  Same results as adatm_mlp_predict. */
+#define ADATM_NOALLOC_MAX_ROWS 4096u
+int adatm_mlp_predict_noalloc(adatm_mlp *m, float const *X, uint32_t n, float *y);
 int adatm_mlp_save(adatm_mlp *m, char const *path);
 int adatm_mlp_load(int device, char const *path, adatm_mlp **out);
 
