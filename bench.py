@@ -93,7 +93,10 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--rank", type=int, default=32)
     ap.add_argument("--ndraws", type=int, default=0, help="override draws (0 = full)")
-    ap.add_argument("--cpu-draws", type=int, default=6_000_000)
+    ap.add_argument("--cpu-draws", type=int, default=0,
+                    help="time the CPU oracle on this many draws (0 = the whole workload)")
+    ap.add_argument("--cpu1-draws", type=int, default=2_000_000,
+                    help="draws of the single-thread CPU oracle run (0 = skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
@@ -217,7 +220,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, R, A)
+        cpu = cpu_baseline(args, R, A, t)
 
     out = {
         "metric": METRIC,
@@ -384,44 +387,54 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
-def cpu_baseline(args, R, A):
+def cpu_baseline(args, R, A, t=None):
     """The oracle's C/OpenMP FP64 restatement of the reference CPU path, timed
-    on this host's cores on the first --cpu-draws draws of the same workload
-    (same generator, so a prefix of the full draw stream), one ALS sweep,
-    scaled to the full workload by the draw ratio; a workload of at most
-    --cpu-draws draws is timed whole."""
-    import numpy as np
+    on this host's cores: the whole workload (the canonical tensor `t` the GPU
+    legs used, or the first --cpu-draws draws of the same generator when that
+    is set), one ALS sweep, median of 3 after 1 warm-up, on every core; and a
+    single-thread run (the only setting in which the literal reference is
+    race-free) on the first --cpu1-draws draws, 1 warm-up + 1 timed sweep."""
     from oracle import corc
 
     full = A.named_spec(args.config, args.ndraws)
-    # a workload smaller than the sample cap is timed whole
-    spec = A.named_spec(args.config, min(args.cpu_draws, full.ndraws) if args.cpu_draws else 0)
-    s = A.Tensor.synth(spec)
-    s.canonicalize()
-    cands, ch = A.plan_analytic(s, R, ref_quirks=True, device=-1)
-    groups = cands[ch].groups(s.nmodes)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-    seq = corc.OracleSeq(s.inds(), s.vals, s.dims, groups, R)
-    U = [A.rand_matrix(spec.seed + 1, m, d, R) for m, d in enumerate(s.dims)]
-    sec = seq.time(U, 1, 3, threads)
-    log(f"cpu baseline: {s.nnz} nnz sample, {sec * 1e3:.1f} ms/sweep on {threads} threads")
-    if spec.ndraws >= full.ndraws:
-        return {
-            "value": round(sec * 1e3, 1), "unit": "ms/sweep",
-            "cores": threads, "kind": "port",
-            "sample": (f"the whole workload ({s.nnz} nnz), plan {ch}, "
-                       f"one FP64 ALS sweep, median of 3"),
-        }
-    # nonzeros of the full workload are known to the caller; scale by draws
-    scale = full.ndraws / spec.ndraws
-    return {
-        "value": round(sec * 1e3 * scale, 1), "unit": "ms/sweep (scaled)",
+
+    def timed(spec, tensor, nthreads, nrep):
+        s = tensor
+        if s is None:
+            s = A.Tensor.synth(spec)
+            s.canonicalize()
+        cands, ch = A.plan_analytic(s, R, ref_quirks=True, device=-1)
+        t0 = time.time()
+        seq = corc.OracleSeq(s.inds(), s.vals, s.dims, cands[ch].groups(s.nmodes), R)
+        build_s = time.time() - t0
+        U = [A.rand_matrix(spec.seed + 1, m, d, R) for m, d in enumerate(s.dims)]
+        sec = seq.time(U, 1, nrep, nthreads)
+        seq.close()
+        return sec, s.nnz, ch, build_s
+
+    sample = args.cpu_draws if 0 < args.cpu_draws < full.ndraws else 0
+    spec = A.named_spec(args.config, sample) if sample else full
+    sec, nnz, ch, build_s = timed(spec, None if sample else t, threads, 3)
+    log(f"cpu baseline: {nnz} nnz, {sec * 1e3:.1f} ms/sweep on {threads} threads "
+        f"(oracle CSF build {build_s:.1f}s)")
+    what = (f"the whole workload ({nnz} nnz)" if not sample else
+            f"first {sample} draws of the same generator ({nnz} nnz; not scaled)")
+    out = {
+        "value": round(sec * 1e3, 1), "unit": "ms/sweep",
         "cores": threads, "kind": "port",
-        "sample": (f"first {spec.ndraws} draws of the same generator ({s.nnz} nnz), "
-                   f"plan {ch}, one FP64 ALS sweep median of 3 = {sec * 1e3:.1f} ms, "
-                   f"scaled x{scale:.1f} by draws"),
+        "sample": f"{what}, plan {ch}, one FP64 ALS sweep, median of 3 after 1 warm-up",
     }
+    if args.cpu1_draws > 0:
+        n1 = min(args.cpu1_draws, full.ndraws)
+        sec1, nnz1, ch1, _ = timed(A.named_spec(args.config, n1), None, 1, 1)
+        log(f"cpu baseline, 1 thread: {nnz1} nnz, {sec1 * 1e3:.1f} ms/sweep")
+        out["single_thread"] = {
+            "value": round(sec1 * 1e3, 1), "unit": "ms/sweep", "cores": 1,
+            "sample": (f"first {n1} draws of the same generator ({nnz1} nnz), plan {ch1}, "
+                       "one FP64 ALS sweep after 1 warm-up (not scaled)")}
+    return out
 
 
 if __name__ == "__main__":

@@ -60,17 +60,60 @@ typedef struct {
 } ohandle;
 
 /* ---- CSF construction ---------------------------------------------------- */
-static uint32_t const *g_keys[OMAXN];
-static uint32_t g_nkeys;
-
-static int cmp_lex(void const *pa, void const *pb)
+/* Stable LSD radix sort of the positions 0..n-1 by the keys
+ * keys[0], ..., keys[nk-1] (keys[0] most significant; key k < kdim[k]):
+ * 11-bit digits, each pass a per-thread histogram over a contiguous range,
+ * bucket offsets in (digit, thread) order and an in-order scatter, so the
+ * result is the unique lexicographic order with ties kept in input order
+ * (SPLATT's tt_sort leaves coordinates in lexicographic order by the level
+ * order, src/adatm_rcsf.c:179). */
+#define RDIG 11
+static void radix_lex(uint64_t n, uint32_t nk, uint32_t const *const *keys,
+                      uint64_t const *kdim, uint32_t *perm)
 {
-  uint64_t const a = *(uint64_t const *)pa, b = *(uint64_t const *)pb;
-  for (uint32_t k = 0; k < g_nkeys; ++k) {
-    uint32_t const x = g_keys[k][a], y = g_keys[k][b];
-    if (x != y) return x < y ? -1 : 1;
+  uint32_t *tmp = (uint32_t *)malloc(n * sizeof(uint32_t));
+  int const T = omp_get_max_threads();
+  uint64_t *hist = (uint64_t *)malloc((size_t)T * (1u << RDIG) * sizeof(uint64_t));
+  #pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)n; ++j) perm[j] = (uint32_t)j;
+  uint32_t *src = perm, *dst = tmp;
+  for (int k = (int)nk - 1; k >= 0; --k) {
+    uint32_t bits = 0;
+    while (bits < 32 && (kdim[k] - 1) >> bits) ++bits;
+    uint32_t const *key = keys[k];
+    for (uint32_t sh = 0; sh < bits; sh += RDIG) {
+      uint32_t const mask = (1u << RDIG) - 1u;
+      #pragma omp parallel num_threads(T)
+      {
+        int const t = omp_get_thread_num(), nt = omp_get_num_threads();
+        uint64_t const lo = n * (uint64_t)t / nt, hi = n * (uint64_t)(t + 1) / nt;
+        uint64_t *h = hist + (size_t)t * (1u << RDIG);
+        memset(h, 0, (1u << RDIG) * sizeof(uint64_t));
+        for (uint64_t j = lo; j < hi; ++j) h[(key[src[j]] >> sh) & mask]++;
+        #pragma omp barrier
+        #pragma omp single
+        {
+          uint64_t run = 0;
+          for (uint32_t d = 0; d <= mask; ++d)
+            for (int q = 0; q < nt; ++q) {
+              uint64_t const c = hist[(size_t)q * (1u << RDIG) + d];
+              hist[(size_t)q * (1u << RDIG) + d] = run;
+              run += c;
+            }
+        }
+        for (uint64_t j = lo; j < hi; ++j) {
+          uint32_t const v = src[j];
+          dst[h[(key[v] >> sh) & mask]++] = v;
+        }
+      }
+      uint32_t *x = src;
+      src = dst;
+      dst = x;
+    }
   }
-  return (a > b) - (a < b);
+  if (src != perm) memcpy(perm, src, n * sizeof(uint32_t));
+  free(hist);
+  free(tmp);
 }
 
 static void build_group(ohandle *h, ogroup *g, uint32_t const *const *ind,
@@ -78,20 +121,25 @@ static void build_group(ohandle *h, ogroup *g, uint32_t const *const *ind,
 {
   uint32_t const N = h->N;
   uint64_t const n = h->nnz;
-  uint64_t *perm = (uint64_t *)malloc(n * sizeof(uint64_t));
-  for (uint64_t j = 0; j < n; ++j) perm[j] = j;
-  for (uint32_t k = 0; k < N; ++k) g_keys[k] = ind[g->order[k]];
-  g_nkeys = N;
-  qsort(perm, n, sizeof(uint64_t), cmp_lex);
+  uint32_t const *key[OMAXN];
+  uint64_t kdim[OMAXN];
+  for (uint32_t k = 0; k < N; ++k) {
+    key[k] = ind[g->order[k]];
+    kdim[k] = h->dims[g->order[k]];
+  }
+  uint32_t *perm = (uint32_t *)malloc(n * sizeof(uint32_t));
+  radix_lex(n, N, key, kdim, perm);
   g->vals = (double *)malloc(n * sizeof(double));
-  for (uint64_t j = 0; j < n; ++j) g->vals[j] = vals[perm[j]];
+  #pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)n; ++j) g->vals[j] = vals[perm[j]];
   /* first differing level of consecutive sorted coordinates */
   uint8_t *fd = (uint8_t *)malloc(n + 1);
   fd[0] = 0;
-  for (uint64_t j = 1; j < n; ++j) {
+  #pragma omp parallel for schedule(static)
+  for (int64_t j = 1; j < (int64_t)n; ++j) {
     uint8_t l = (uint8_t)N;
     for (uint32_t k = 0; k < N; ++k)
-      if (g_keys[k][perm[j]] != g_keys[k][perm[j - 1]]) {
+      if (key[k][perm[j]] != key[k][perm[j - 1]]) {
         l = (uint8_t)k;
         break;
       }
@@ -109,7 +157,7 @@ static void build_group(ohandle *h, ogroup *g, uint32_t const *const *ind,
     for (uint64_t j = 0; j < n; ++j)
       if (l == N - 1 || j == 0 || fd[j] <= l) {
         start[l][k] = j;
-        g->fids[l][k] = g_keys[l][perm[j]];
+        g->fids[l][k] = key[l][perm[j]];
         ++k;
       }
     start[l][g->F[l]] = n;
@@ -132,7 +180,8 @@ void *orc_build(uint32_t N, uint64_t nnz, uint32_t const *const *ind,
                 uint64_t const *dims, double const *vals, uint32_t ngrp,
                 uint32_t const *desc, uint32_t R)
 {
-  if (N < 3 || N > OMAXN || ngrp < 1 || ngrp > N || nnz == 0 || R == 0)
+  if (N < 3 || N > OMAXN || ngrp < 1 || ngrp > N || nnz == 0 || R == 0 ||
+      nnz >= (1ull << 32))
     return NULL;
   ohandle *h = (ohandle *)calloc(1, sizeof(ohandle));
   h->N = N;

@@ -45,15 +45,21 @@ def test_json_line_holds_no_nan():
     json.dumps(out, allow_nan=False)
 
 
-def test_cpu_baseline_times_a_small_workload_whole():
-    """A workload no larger than the sample cap is timed whole, unscaled; a
-    larger one is timed on a prefix of its draws and scaled by the ratio."""
+def test_cpu_baseline_is_measured_not_scaled():
+    """The CPU oracle is timed on the whole workload (the tensor the GPU legs
+    used) or, when asked, on a prefix of its draws -- never scaled -- and the
+    single-thread run reports its own sample."""
     import types
     import adatm_amd as A
-    whole = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=6_000_000,
-                                                     ndraws=40_000), 8, A)
-    assert whole["unit"] == "ms/sweep" and whole["sample"].startswith("the whole workload")
-    part = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=10_000,
+    t = A.synth_tensor(1, 40_000)
+    whole = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=0, cpu1_draws=10_000,
+                                                     ndraws=40_000), 8, A, t)
+    assert whole["unit"] == "ms/sweep" and whole["sample"].startswith(
+        f"the whole workload ({t.nnz} nnz)")
+    one = whole["single_thread"]
+    assert one["cores"] == 1 and one["sample"].startswith("first 10000 draws")
+    part = bench.cpu_baseline(types.SimpleNamespace(config=1, cpu_draws=10_000, cpu1_draws=0,
                                                     ndraws=40_000), 8, A)
-    assert part["unit"] == "ms/sweep (scaled)" and "scaled x4.0" in part["sample"]
-    assert whole["value"] > 0 and part["value"] > 0
+    assert part["unit"] == "ms/sweep" and part["sample"].startswith("first 10000 draws")
+    assert "single_thread" not in part
+    assert whole["value"] > 0 and part["value"] > 0 and one["value"] > 0
