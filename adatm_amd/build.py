@@ -88,6 +88,11 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
         exe = BINDIR / "splatt"
         _run(["gcc", *CFLAGS, str(cli), "-o", str(exe), f"-L{LIBDIR}",
               "-lsplatt_adatm", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    # the C program replaying the reference drivers' call sequences (tests)
+    replay = ROOT / "tests" / "boundary_replay.c"
+    if replay.exists():
+        _run(["gcc", *CFLAGS, str(replay), "-o", str(BINDIR / "boundary_replay"),
+              f"-L{LIBDIR}", "-lsplatt_adatm", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     if verbose:
         print(f"built {lib}")
     return lib

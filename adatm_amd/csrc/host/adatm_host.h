@@ -77,6 +77,13 @@ int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
                              adatm_plan const *plan, int precision, int device,
                              uint32_t const *use_grp, uint32_t const *use_tag,
                              adatm_engine **out);
+/* device-built engine with a given mode -> (group, tag) map */
+int adatm_engine_create_use(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                            int precision, int device, uint32_t const *use_grp,
+                            uint32_t const *use_tag, adatm_engine **out);
+/* one CSF (sorted values, fptr, fids of every level) built on the device */
+int adatm_hcsf_build_device(adatm_coo const *t, uint32_t const *order, int device,
+                            adatm_hcsf *h);
 uint32_t adatm_engine_nmodes(adatm_engine const *e);
 uint64_t adatm_engine_dim(adatm_engine const *e, uint32_t m);
 uint32_t adatm_engine_rank(adatm_engine const *e);

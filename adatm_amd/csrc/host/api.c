@@ -59,6 +59,7 @@ typedef struct reg_ent {
   int fac_on_dev[MAXN];
   uint64_t memo_fac[MAXN][MAXN]; /* per group: factor hashes at its last root */
   int memo_ok[MAXN];
+  adatm_coo coo; /* the coordinates as given, kept for device engine builds */
   struct reg_ent *next;
 } reg_ent;
 
@@ -330,11 +331,11 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
       rc = adatm_plan_exhaustive(t, hint, prec, r->device, 1, 3, cands, 256, &nc, &ch,
                                  NULL);
     } else {
-      /* the analytic choice (min pops) does not depend on the rank: every
-         product count is proportional to it; rank 1 only loosens the space
-         filter, which the engine re-checks against the device memory */
-      rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
-                               opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands,
+      /* the rank of the decomposition is not known yet: plan for the hint
+         (the analytic choice, min pops, is rank-independent; only the space
+         filter depends on the rank); fiber counts on the device */
+      rc = adatm_plan_analytic(t, hint, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
+                               opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], r->device, cands,
                                256, &nc, &ch);
     }
     if (rc == SPLATT_SUCCESS) {
@@ -345,9 +346,12 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
   } else if (alloc == SPLATT_ADAPTIVE_SPACE_EFFICIENT) {
     adatm_plan cands[256], full;
     int nc = 0, ch = 0;
-    rc = adatm_plan_analytic(t, 1, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
-                             opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], -1, cands, 256, &nc,
-                             &ch);
+    uint64_t const hint = opts[SPLATT_OPTION_ADATM_RANK_HINT] >= 1
+                              ? (uint64_t)opts[SPLATT_OPTION_ADATM_RANK_HINT]
+                              : 16;
+    rc = adatm_plan_analytic(t, hint, (int)opts[SPLATT_OPTION_ADATM_REFQUIRKS],
+                             opts[SPLATT_OPTION_ADATM_SPACE_LIMIT], r->device, cands, 256,
+                             &nc, &ch);
     if (rc == SPLATT_SUCCESS) {
       full = cands[ch];
       r->full = full;
@@ -363,8 +367,17 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
   }
   uint32_t const ng = r->plan.n_grp;
   adatm_hcsf *h = (adatm_hcsf *)calloc(ng, sizeof(adatm_hcsf));
+  /* on the device: the GPU radix sort and scans; the host build otherwise */
   for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < ng; ++g)
-    rc = adatm_hcsf_build(t, r->plan.grp[g].o_opt, h + g);
+    rc = (r->device >= 0) ? adatm_hcsf_build_device(t, r->plan.grp[g].o_opt, r->device, h + g)
+                          : adatm_hcsf_build(t, r->plan.grp[g].o_opt, h + g);
+  if (rc == SPLATT_SUCCESS && r->device >= 0) {
+    /* keep the coordinates: engines for this CSF set are built on the device */
+    rc = adatm_coo_alloc(N, t->nnz, t->dims, &r->coo);
+    for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < N; ++m)
+      memcpy(r->coo.ind[m], t->ind[m], t->nnz * sizeof(uint32_t));
+    if (rc == SPLATT_SUCCESS) memcpy(r->coo.vals, t->vals, t->nnz * sizeof(double));
+  }
   splatt_csf *arr = NULL;
   if (rc == SPLATT_SUCCESS) {
     arr = csf_array_from_hcsf(h, ng, N, t->dims);
@@ -374,6 +387,7 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
   free(h);
   if (rc != SPLATT_SUCCESS) {
     free_csf_array(arr, ng);
+    adatm_coo_free(&r->coo);
     free(r);
     return rc;
   }
@@ -547,6 +561,7 @@ void splatt_free_csf(splatt_csf *tensors, double const *const options)
   if (r) {
     n = r->n_csf;
     adatm_engine_destroy(r->eng);
+    adatm_coo_free(&r->coo);
     free(r);
   } else {
     int const alloc = options ? (int)options[SPLATT_OPTION_CSF_ALLOC]
@@ -593,6 +608,15 @@ static int ensure_engine(reg_ent *r, uint64_t rank, int prec)
   }
   memset(r->fac_on_dev, 0, sizeof(r->fac_on_dev));
   memset(r->memo_ok, 0, sizeof(r->memo_ok));
+  if (r->coo.nnz) { /* built on the device from the coordinates */
+    int const rc = adatm_engine_create_use(&r->coo, rank, &r->plan, prec, r->device,
+                                           r->use_grp, r->use_tag, &r->eng);
+    if (rc == SPLATT_SUCCESS) {
+      r->eng_rank = (uint32_t)rank;
+      r->eng_prec = prec;
+    }
+    return rc;
+  }
   adatm_hcsf *h = (adatm_hcsf *)calloc(r->n_csf, sizeof(adatm_hcsf));
   if (!h) return SPLATT_ERROR_NOMEMORY;
   int rc = SPLATT_SUCCESS;
@@ -850,6 +874,10 @@ void mttkrp_csf_adaptive(splatt_csf const *const tensors,
 }
 
 /* ---- CP-ALS ------------------------------------------------------------------ */
+static int plan_matches(reg_ent const *r, group_properties const *grp_prop,
+                        splatt_idx_t n_grp, splatt_idx_t const *use_csfs,
+                        splatt_idx_t const *use_tags);
+
 /* columns to unit 2-norm with the norms folded into lambda (SPLATT's
  * cpd_post_process) */
 static void post_process(uint32_t N, uint64_t const *dims, uint64_t R,
@@ -866,14 +894,22 @@ static void post_process(uint32_t N, uint64_t const *dims, uint64_t R,
     }
 }
 
-static int cpd_registered(reg_ent *r, splatt_idx_t nfactors,
-                          double const *opts, splatt_kruskal *out)
+/* CP-ALS on a registered CSF set (cpd_als_iterate_adaptive,
+ * src/adatm_cpd.c:256-374): initial factors from `init` (caller matrices,
+ * dims[m] x rank) or, when NULL, from the seeded generator; the final
+ * factors (unit 2-norm columns) go to out[m], their scales to lambda */
+static int cpd_core(reg_ent *r, splatt_idx_t nfactors, double const *opts,
+                    double *const *init, double **out, double *lambda, double *fit_out)
 {
   int const prec = opt_prec(opts);
   int rc = ensure_engine(r, nfactors, prec);
   if (rc != SPLATT_SUCCESS) return rc;
-  uint64_t const seed = (uint64_t)opts[SPLATT_OPTION_RANDSEED];
-  rc = adatm_engine_init_factors(r->eng, seed);
+  if (init) {
+    for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < r->N; ++m)
+      rc = adatm_engine_set_factor(r->eng, m, init[m]);
+  } else {
+    rc = adatm_engine_init_factors(r->eng, (uint64_t)opts[SPLATT_OPTION_RANDSEED]);
+  }
   if (rc != SPLATT_SUCCESS) return rc;
   memset(r->fac_on_dev, 0, sizeof(r->fac_on_dev));
   memset(r->memo_ok, 0, sizeof(r->memo_ok));
@@ -901,22 +937,39 @@ static int cpd_registered(reg_ent *r, splatt_idx_t nfactors,
       verb > SPLATT_VERBOSITY_NONE)
     fprintf(stderr, "warning: the CP-ALS normal equations were not numerically positive "
                     "definite; solved with a ridge\n");
+  rc = adatm_engine_get_lambda(r->eng, lambda);
+  for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < r->N; ++m)
+    rc = adatm_engine_get_factor(r->eng, m, out[m]);
+  if (rc != SPLATT_SUCCESS) return rc;
+  post_process(r->N, r->dims, nfactors, out, lambda);
+  *fit_out = fit;
+  return SPLATT_SUCCESS;
+}
+
+static int cpd_registered(reg_ent *r, splatt_idx_t nfactors,
+                          double const *opts, splatt_kruskal *out)
+{
   memset(out, 0, sizeof(*out));
   out->rank = nfactors;
   out->nmodes = r->N;
-  out->fit = fit;
   out->lambda = (splatt_val_t *)malloc(nfactors * sizeof(splatt_val_t));
   if (!out->lambda) return SPLATT_ERROR_NOMEMORY;
-  rc = adatm_engine_get_lambda(r->eng, out->lambda);
-  for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < r->N; ++m) {
+  for (uint32_t m = 0; m < r->N; ++m) {
     out->dims[m] = r->dims[m];
     out->factors[m] = (splatt_val_t *)malloc(r->dims[m] * nfactors * sizeof(splatt_val_t));
-    if (!out->factors[m]) return SPLATT_ERROR_NOMEMORY;
-    rc = adatm_engine_get_factor(r->eng, m, out->factors[m]);
+    if (!out->factors[m]) {
+      splatt_free_kruskal(out);
+      return SPLATT_ERROR_NOMEMORY;
+    }
   }
-  if (rc == SPLATT_SUCCESS)
-    post_process(r->N, r->dims, nfactors, out->factors, out->lambda);
-  return rc;
+  double fit = 0.0;
+  int const rc = cpd_core(r, nfactors, opts, NULL, out->factors, out->lambda, &fit);
+  if (rc != SPLATT_SUCCESS) {
+    splatt_free_kruskal(out);
+    return rc;
+  }
+  out->fit = fit;
+  return SPLATT_SUCCESS;
 }
 
 int splatt_cpd_als(splatt_csf const *const tensors, splatt_idx_t const nfactors,
@@ -950,16 +1003,146 @@ int splatt_cpd_als_adaptive(splatt_csf const *const tensors,
     adatm_set_error("splatt_cpd_als_adaptive: CSF not created by this library");
     return SPLATT_ERROR_BADINPUT;
   }
-  /* the registered plan must be the one described by the arguments */
-  if (n_grp != r->full.n_grp) return SPLATT_ERROR_BADINPUT;
+  if (!plan_matches(r, grp_prop, n_grp, use_csfs, use_tags)) {
+    adatm_set_error("splatt_cpd_als_adaptive: the plan differs from the CSFs'");
+    return SPLATT_ERROR_BADINPUT;
+  }
+  return splatt_cpd_als(tensors, nfactors, options, factored);
+}
+
+/* the registered plan is the one the arguments describe */
+static int plan_matches(reg_ent const *r, group_properties const *grp_prop,
+                        splatt_idx_t n_grp, splatt_idx_t const *use_csfs,
+                        splatt_idx_t const *use_tags)
+{
+  if (!grp_prop || !use_csfs || !use_tags || n_grp != r->full.n_grp) return 0;
   for (splatt_idx_t g = 0; g < n_grp; ++g)
     if (grp_prop[g].memo_mode != r->full.grp[g].memo_mode ||
         grp_prop[g].n_imten != r->full.grp[g].n_imten)
-      return SPLATT_ERROR_BADINPUT;
+      return 0;
   for (uint32_t m = 0; m < r->N; ++m)
-    if (use_csfs[m] != r->use_grp[m] || use_tags[m] != r->use_tag[m])
-      return SPLATT_ERROR_BADINPUT;
-  return splatt_cpd_als(tensors, nfactors, options, factored);
+    if (use_csfs[m] != r->use_grp[m] || use_tags[m] != r->use_tag[m]) return 0;
+  return 1;
+}
+
+/* The ALS loop of the reference (adatm_cpd.h:29-41, src/adatm_cpd.c:256-374)
+ * on caller matrices: mats[m] (dims[m] x nfactors, row-major) hold the
+ * initial factors and receive the final ones (unit 2-norm columns, the
+ * norms in lambda); mats[SPLATT_MAX_NMODES] is the reference's MTTKRP
+ * buffer and is not used.  `rinfo` (SPLATT's MPI rank information) is
+ * ignored and may be NULL.  Returns the final fit, or NaN on failure
+ * (message in adatm_last_error()). */
+double cpd_als_iterate_adaptive(splatt_csf const *const tensors,
+                                rcsf_seq_adaptive *const rs_seq,
+                                splatt_idx_t const n_csf,
+                                group_properties const *const grp_prop,
+                                splatt_idx_t const n_grp,
+                                splatt_idx_t const *const use_csfs,
+                                splatt_idx_t const *const use_tags, matrix_t **mats,
+                                splatt_val_t *const lambda, splatt_idx_t const nfactors,
+                                rank_info *const rinfo, double const *const opts)
+{
+  (void)rs_seq; (void)n_csf; (void)rinfo;
+  reg_ent *r = reg_find(tensors);
+  if (!r || !mats || !lambda || nfactors < 1) {
+    adatm_set_error("cpd_als_iterate_adaptive: CSF not created by this library");
+    return NAN;
+  }
+  if (!plan_matches(r, grp_prop, n_grp, use_csfs, use_tags)) {
+    adatm_set_error("cpd_als_iterate_adaptive: the plan differs from the CSFs'");
+    return NAN;
+  }
+  double *A[MAXN];
+  for (uint32_t m = 0; m < r->N; ++m) {
+    if (!mats[m] || mats[m]->I != r->dims[m] || mats[m]->J != nfactors) {
+      adatm_set_error("cpd_als_iterate_adaptive: factor %u is not %llu x %llu", m,
+                      (unsigned long long)r->dims[m], (unsigned long long)nfactors);
+      return NAN;
+    }
+    A[m] = mats[m]->vals;
+  }
+  double *dflt = opts ? NULL : splatt_default_opts();
+  double fit = 0.0;
+  int const rc = cpd_core(r, nfactors, opts ? opts : dflt, A, A, lambda, &fit);
+  free(dflt);
+  return rc == SPLATT_SUCCESS ? fit : NAN;
+}
+
+/* ---- SPLATT-internal helpers the AdaTM drivers call ---------------------- */
+/* tt_read (cmd_bench.c:234, cmd_cpd.c:139): a coordinate text file as read by
+ * splatt_csf_load; dims = largest index + 1 per mode */
+sptensor_t *tt_read(char const *const fname)
+{
+  uint32_t N;
+  uint64_t nnz;
+  splatt_idx_t **ind;
+  splatt_val_t *val;
+  if (adatm_read_tns(fname, &N, &nnz, &ind, &val) != SPLATT_SUCCESS) return NULL;
+  sptensor_t *tt = (sptensor_t *)calloc(1, sizeof(sptensor_t));
+  splatt_idx_t *dims = (splatt_idx_t *)calloc(N, sizeof(splatt_idx_t));
+  if (!tt || !dims) {
+    for (uint32_t m = 0; m < N; ++m) free(ind[m]);
+    free(ind);
+    free(val);
+    free(tt);
+    free(dims);
+    return NULL;
+  }
+  for (uint32_t m = 0; m < N; ++m)
+    for (uint64_t j = 0; j < nnz; ++j)
+      if (ind[m][j] + 1 > dims[m]) dims[m] = ind[m][j] + 1;
+  tt->nnz = nnz;
+  tt->nmodes = N;
+  tt->dims = dims;
+  tt->ind = ind;
+  tt->vals = val;
+  return tt;
+}
+
+void tt_free(sptensor_t *tt)
+{
+  if (!tt) return;
+  for (splatt_idx_t m = 0; m < tt->nmodes; ++m) free(tt->ind[m]);
+  free(tt->ind);
+  free(tt->vals);
+  free(tt->dims);
+  free(tt);
+}
+
+/* csf_free (adatm_bench.c:650) and splatt_csf_free (cmd_cpd.c:255) are
+ * SPLATT's internal names of splatt_free_csf */
+void csf_free(splatt_csf *tensors, double const *const opts)
+{
+  splatt_free_csf(tensors, opts);
+}
+
+void splatt_csf_free(splatt_csf *tensors, double const *const opts)
+{
+  splatt_free_csf(tensors, opts);
+}
+
+/* zero-filled row-major I x J matrix (SPLATT matrix.h mat_alloc) */
+matrix_t *mat_alloc(splatt_idx_t const nrows, splatt_idx_t const ncols)
+{
+  matrix_t *M = (matrix_t *)malloc(sizeof(matrix_t));
+  if (!M) return NULL;
+  M->I = nrows;
+  M->J = ncols;
+  M->rowmajor = 1;
+  size_t const n = (size_t)(nrows * ncols);
+  M->vals = (splatt_val_t *)calloc(n > 0 ? n : 1, sizeof(splatt_val_t));
+  if (!M->vals) {
+    free(M);
+    return NULL;
+  }
+  return M;
+}
+
+void mat_free(matrix_t *M)
+{
+  if (!M) return;
+  free(M->vals);
+  free(M);
 }
 
 /* The engine behind a registered CSF (NULL before the rank is known). */

@@ -1,8 +1,9 @@
 /*
  * `splatt` command-line tool: the AdaTM subcommands on one MI355X.
  *
- *   splatt bench TENSOR -a adatm [-i NITERS] [-r RANK] [-t NTHREADS] [--precision P] [--planner PL]
- *   splatt cpd   TENSOR [-i NITERS] [--tol TOL] [-r RANK] [-t NTHREADS]
+ *   splatt bench TENSOR -a adatm|csf [-a ...] [-i NITERS] [-r RANK] [-t NTHREADS]
+ *                [-s] [-w] [--tile] [--precision P] [--planner PL]
+ *   splatt cpd   TENSOR [-i NITERS] [--tol TOL] [-r RANK] [-t NTHREADS] [--tile]
  *                [--nowrite] [-v] [--precision P] [--seed S] [--planner PL]
  *
  * Flags, defaults and log lines follow the reference's drivers:
@@ -14,6 +15,12 @@
  * TENSOR is a coordinate text file (.tns: N 1-based indices and a value per
  * line) or synth:<1|2|3>[:draws] for the seeded synthetic workloads.
  * Duplicate coordinates are summed and empty indices removed on input.
+ * The reference's reordering options (-z TYPE, -p FILE, -m MODE;
+ * cmd_bench.c:84-85, 119-136) are recognised and refused: this build does
+ * not reorder.  --tile is accepted and, as in the reference's AdaTM paths
+ * (adatm_bench.c:535), has no effect.  The options of this build
+ * (--precision, --planner, --seed) are long options only, so every short
+ * option keeps the reference's meaning.
  */
 #include <argp.h>
 #include <math.h>
@@ -160,19 +167,30 @@ static void print_storage(adatm_engine *e, adatm_plan const *plan, uint32_t N)
 /* ---- bench ------------------------------------------------------------ */
 struct bench_args {
   char *tensor;
-  int adatm;
-  char *alg;
+  int adatm, csf;
+  char *badalg;
   uint32_t niters, nthreads, rank;
-  int prec, planner;
+  int prec, planner, scale, write;
+  char *reorder; /* the first reordering option given (refused) */
 };
 
+#define OPT_TILE 255
+#define OPT_PREC 251
+#define OPT_PLANNER 250
 static struct argp_option bench_opts[] = {
-    {"alg", 'a', "ALG", 0, "algorithm to benchmark (adatm)", 0},
+    {"alg", 'a', "ALG", 0, "algorithm to benchmark: adatm, csf (repeatable)", 0},
     {"iters", 'i', "NITERS", 0, "number of iterations to use (default: 5)", 0},
-    {"threads", 't', "NTHREADS", 0, "host threads (default: 1)", 0},
+    {"mode", 'm', "MODE", 0, "mode basis for hgraph reordering (not supported)", 0},
+    {"threads", 't', "NTHREADS", 0, "number of threads to use (default: 1)", 0},
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 16)", 0},
-    {"precision", 'p', "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
-    {"planner", 'P', "PLANNER", 0, "analytic | learned | exhaustive (default: analytic)", 0},
+    {"scale", 's', 0, 0, "scale threads from 1 to NTHREADS (by 2)", 0},
+    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT (no effect here)", 0},
+    {"write", 'w', 0, 0, "write results to files ALG_mode<N>.mat (for testing)", 0},
+    {"rtype", 'z', "TYPE", 0, "designate reordering type (not supported)", 0},
+    {"pfile", 'p', "FILE", 0, "partition file for reordering (not supported)", 0},
+    {"precision", OPT_PREC, "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
+    {"planner", OPT_PLANNER, "PLANNER", 0,
+     "analytic | learned | exhaustive (default: analytic)", 0},
     {0}};
 
 static int parse_planner(char const *s)
@@ -194,14 +212,24 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
   struct bench_args *a = (struct bench_args *)st->input;
   switch (key) {
   case 'a':
-    a->alg = arg;
-    a->adatm = !strcmp(arg, "adatm");
+    if (!strcmp(arg, "adatm"))
+      a->adatm = 1;
+    else if (!strcmp(arg, "csf"))
+      a->csf = 1;
+    else
+      a->badalg = arg;
     break;
   case 'i': a->niters = (uint32_t)atoi(arg); break;
   case 't': a->nthreads = (uint32_t)atoi(arg); break;
   case 'r': a->rank = (uint32_t)atoi(arg); break;
-  case 'p': a->prec = parse_prec(arg); break;
-  case 'P': a->planner = parse_planner(arg); break;
+  case 's': a->scale = 1; break;
+  case 'w': a->write = 1; break;
+  case OPT_TILE: break;
+  case 'z': case 'p': case 'm':
+    if (!a->reorder) a->reorder = key == 'z' ? "-z" : (key == 'p' ? "-p" : "-m");
+    break;
+  case OPT_PREC: a->prec = parse_prec(arg); break;
+  case OPT_PLANNER: a->planner = parse_planner(arg); break;
   case ARGP_KEY_ARG:
     if (a->tensor) argp_usage(st);
     a->tensor = arg;
@@ -214,16 +242,139 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
   return 0;
 }
 
+static void write_matrix(char const *fname, double const *A, uint64_t I,
+                         uint64_t J);
+
+/* thread counts of a run: NTHREADS, or 1, 2, 4, ..., NTHREADS with -s
+ * (cmd_bench.c:174-201); the GPU path does not use host threads, so only
+ * the log lines differ */
+static uint32_t thread_runs(struct bench_args const *a, uint32_t *th)
+{
+  uint32_t n = 0;
+  if (a->scale) {
+    for (uint32_t t = 1; t < a->nthreads; t *= 2) th[n++] = t;
+  }
+  th[n++] = a->nthreads ? a->nthreads : 1;
+  return n;
+}
+
+/* the timed loop of bench_adaptm (adatm_bench.c:606-642): iterations x modes,
+ * with -w the first iteration's MTTKRPs written to <alg>_mode<m>.mat */
+typedef int (*mttkrp_fn)(void *ctx, uint32_t m, double *out);
+
+static int bench_loop(struct bench_args const *a, adatm_coo const *t, char const *alg,
+                      mttkrp_fn f, void *ctx)
+{
+  uint32_t th[40];
+  uint32_t const nruns = thread_runs(a, th);
+  uint64_t maxd = 0;
+  for (uint32_t m = 0; m < t->nmodes; ++m) maxd = t->dims[m] > maxd ? t->dims[m] : maxd;
+  double *out = a->write ? (double *)malloc(maxd * a->rank * sizeof(double)) : NULL;
+  for (uint32_t r = 0; r < nruns; ++r) {
+    if (nruns > 1) printf("## THREADS %u\n", th[r]);
+    for (uint32_t i = 0; i < a->niters; ++i) {
+      double const t0 = now();
+      for (uint32_t m = 0; m < t->nmodes; ++m) {
+        int const w = out && r + 1 == nruns && i == 0;
+        double const tm = now();
+        if (f(ctx, m, w ? out : NULL) != SPLATT_SUCCESS) {
+          fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
+          free(out);
+          return -1;
+        }
+        printf("  mode %u %0.3fs\n", m + 1, now() - tm);
+        if (w) {
+          char fn[64];
+          snprintf(fn, sizeof(fn), "%s_mode%u.mat", alg, m + 1);
+          write_matrix(fn, out, t->dims[m], a->rank);
+        }
+      }
+      printf("    its = %3u (%0.3fs)\n", i + 1, now() - t0);
+    }
+  }
+  free(out);
+  return 0;
+}
+
+static int adatm_mttkrp_fn(void *ctx, uint32_t m, double *out)
+{
+  return adatm_engine_mttkrp((adatm_engine *)ctx, m, out);
+}
+
+/* -a csf: SPLATT's plain CSF MTTKRP (two CSFs, SPLATT's default
+ * allocation) through the public API */
+typedef struct {
+  splatt_csf *csf;
+  double *opts;
+  double **U;
+  double *scratch;
+  uint64_t rank;
+} csf_ctx;
+
+static int csf_mttkrp_fn(void *vctx, uint32_t m, double *out)
+{
+  csf_ctx *c = (csf_ctx *)vctx;
+  return splatt_mttkrp(m, c->rank, c->csf, c->U, out ? out : c->scratch, c->opts);
+}
+
+static int bench_csf(struct bench_args const *a, adatm_coo const *t)
+{
+  printf("## CSF\n");
+  csf_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.rank = a->rank;
+  c.opts = splatt_default_opts();
+  c.opts[SPLATT_OPTION_CSF_ALLOC] = SPLATT_CSF_TWOMODE;
+  c.opts[SPLATT_OPTION_ADATM_PRECISION] = a->prec;
+  splatt_idx_t *ind[ADATM_MAX_NMODES];
+  uint64_t maxd = 0;
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    ind[m] = (splatt_idx_t *)malloc(t->nnz * sizeof(splatt_idx_t));
+    for (uint64_t j = 0; j < t->nnz; ++j) ind[m][j] = t->ind[m][j];
+    maxd = t->dims[m] > maxd ? t->dims[m] : maxd;
+  }
+  int rc = splatt_csf_convert(t->nmodes, t->nnz, ind, t->vals, &c.csf, c.opts);
+  for (uint32_t m = 0; m < t->nmodes; ++m) free(ind[m]);
+  if (rc != SPLATT_SUCCESS) {
+    fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
+    free(c.opts);
+    return -1;
+  }
+  c.U = (double **)calloc(t->nmodes, sizeof(double *));
+  for (uint32_t m = 0; m < t->nmodes; ++m) {
+    c.U[m] = (double *)malloc(t->dims[m] * a->rank * sizeof(double));
+    adatm_rand_matrix(1, m, t->dims[m], a->rank, c.U[m]);
+  }
+  c.scratch = (double *)malloc(maxd * a->rank * sizeof(double));
+  rc = bench_loop(a, t, "csf", csf_mttkrp_fn, &c);
+  for (uint32_t m = 0; m < t->nmodes; ++m) free(c.U[m]);
+  free(c.U);
+  free(c.scratch);
+  splatt_free_csf(c.csf, c.opts);
+  free(c.opts);
+  return rc;
+}
+
 static int cmd_bench(int argc, char **argv)
 {
-  struct bench_args a = {NULL, 0, NULL, 5, 1, 16, SPLATT_ADATM_FP64,
-                         SPLATT_ADATM_PLAN_ANALYTIC};
-  struct argp ap = {bench_opts, bench_parse, "TENSOR -a adatm",
+  struct bench_args a;
+  memset(&a, 0, sizeof(a));
+  a.niters = 5;
+  a.nthreads = 1;
+  a.rank = 16;
+  a.prec = SPLATT_ADATM_FP64;
+  a.planner = SPLATT_ADATM_PLAN_ANALYTIC;
+  struct argp ap = {bench_opts, bench_parse, "TENSOR -a ALG",
                     "splatt-bench -- benchmark the memoized MTTKRP sequence.", 0, 0, 0};
   argp_parse(&ap, argc, argv, ARGP_IN_ORDER, 0, &a);
-  if (!a.adatm) {
-    fprintf(stderr, "SPLATT: algorithm '%s' is not recognized (only 'adatm').\n",
-            a.alg ? a.alg : "");
+  if (a.badalg || (!a.adatm && !a.csf)) {
+    fprintf(stderr, "SPLATT: algorithm '%s' is not recognized (adatm, csf).\n",
+            a.badalg ? a.badalg : "");
+    return EXIT_FAILURE;
+  }
+  if (a.reorder) {
+    fprintf(stderr, "SPLATT: %s: tensor reordering is not supported by this build.\n",
+            a.reorder);
     return EXIT_FAILURE;
   }
   adatm_coo t;
@@ -234,30 +385,24 @@ static int cmd_bench(int argc, char **argv)
   print_header(&t, a.tensor);
   printf("Benchmarking ---------------------------------------------------\n");
   printf("RANK=%u ITS=%u\n", a.rank, a.niters);
-  adatm_plan plan;
-  if (plan_for(&t, a.rank, a.planner, a.prec, &plan) != 0) return EXIT_FAILURE;
-  adatm_engine *e = NULL;
-  if (adatm_engine_create(&t, a.rank, &plan, a.prec, 0, &e) != SPLATT_SUCCESS) {
-    fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
-    return EXIT_FAILURE;
-  }
-  print_storage(e, &plan, t.nmodes);
-  adatm_engine_init_factors(e, 1);
-  for (uint32_t i = 0; i < a.niters; ++i) {
-    double const t0 = now();
-    for (uint32_t m = 0; m < t.nmodes; ++m) {
-      double const tm = now();
-      if (adatm_engine_mttkrp(e, m, NULL) != SPLATT_SUCCESS) {
-        fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
-        return EXIT_FAILURE;
-      }
-      printf("  mode %u %0.3fs\n", m + 1, now() - tm);
+  int rc = 0;
+  if (a.csf) rc = bench_csf(&a, &t);
+  if (rc == 0 && a.adatm) {
+    printf("## ADATM\n");
+    adatm_plan plan;
+    if (plan_for(&t, a.rank, a.planner, a.prec, &plan) != 0) return EXIT_FAILURE;
+    adatm_engine *e = NULL;
+    if (adatm_engine_create(&t, a.rank, &plan, a.prec, 0, &e) != SPLATT_SUCCESS) {
+      fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
+      return EXIT_FAILURE;
     }
-    printf("    its = %3u (%0.3fs)\n", i + 1, now() - t0);
+    print_storage(e, &plan, t.nmodes);
+    adatm_engine_init_factors(e, 1);
+    rc = bench_loop(&a, &t, "adatm", adatm_mttkrp_fn, e);
+    adatm_engine_destroy(e);
   }
-  adatm_engine_destroy(e);
   adatm_coo_free(&t);
-  return EXIT_SUCCESS;
+  return rc == 0 ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 
 /* ---- cpd -------------------------------------------------------------- */
@@ -277,12 +422,14 @@ static struct argp_option cpd_opts[] = {
     {"iters", 'i', "NITERS", 0, "maximum number of iterations to use (default: 50)", 0},
     {"tol", OPT_TOL, "TOLERANCE", 0, "minimum change for convergence (default: 1e-5)", 0},
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 10)", 0},
-    {"threads", 't', "NTHREADS", 0, "host threads (default: #cores)", 0},
+    {"threads", 't', "NTHREADS", 0, "number of threads to use (default: #cores)", 0},
+    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT (no effect here)", 0},
     {"nowrite", OPT_NOWRITE, 0, 0, "do not write output to file", 0},
     {"verbose", 'v', 0, 0, "turn on verbose output (default: no)", 0},
-    {"precision", 'p', "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
+    {"precision", OPT_PREC, "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
     {"seed", OPT_SEED, "SEED", 0, "factor initialisation seed (default: time)", 0},
-    {"planner", 'P', "PLANNER", 0, "analytic | learned | exhaustive (default: analytic)", 0},
+    {"planner", OPT_PLANNER, "PLANNER", 0,
+     "analytic | learned | exhaustive (default: analytic)", 0},
     {0}};
 
 static error_t cpd_parse(int key, char *arg, struct argp_state *st)
@@ -296,9 +443,10 @@ static error_t cpd_parse(int key, char *arg, struct argp_state *st)
   case 't': a->nthreads = (uint32_t)atoi(arg); break;
   case OPT_NOWRITE: a->write = 0; break;
   case 'v': a->verbose += 1; break;
-  case 'p': a->prec = parse_prec(arg); break;
+  case OPT_TILE: break;
+  case OPT_PREC: a->prec = parse_prec(arg); break;
   case OPT_SEED: a->seed = strtoull(arg, NULL, 10); break;
-  case 'P': a->planner = parse_planner(arg); break;
+  case OPT_PLANNER: a->planner = parse_planner(arg); break;
   case ARGP_KEY_ARG:
     if (a->tensor) argp_usage(st);
     a->tensor = arg;

@@ -910,6 +910,114 @@ int adatm_count_fibers_device_many(adatm_coo const *t, uint32_t const *orders,
   return rc;
 }
 
+/* One CSF in host memory built on the device (the SPLATT API's arrays):
+ * stable radix sort by `order`, first differing levels, per level the node
+ * flags, their scan, ids and starts, child pointers; the same kernels and
+ * arithmetic as the engine's group build, so the structure is the engine's.
+ * Fills h->F, fptr, fids (root ids included) and the values in sorted order;
+ * h->par stays NULL. */
+int adatm_hcsf_build_device(adatm_coo const *t, uint32_t const *order, int device,
+                            adatm_hcsf *h)
+{
+  memset(h, 0, sizeof(*h));
+  uint32_t const N = t->nmodes;
+  uint64_t const n = t->nnz;
+  if (N < 2 || N > MAXN || n == 0 || n >= (1ull << 32)) return SPLATT_ERROR_BADINPUT;
+  HIPCHK(hipSetDevice(device), "hipSetDevice");
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+  dcoo c;
+  memset(&c, 0, sizeof(c));
+  uint32_t *perm = NULL, *keys = NULL, *stemp = NULL, *tot = NULL, *ptr = NULL;
+  uint32_t *sc[MAXN] = {0}, *ids = NULL;
+  uint32_t *flags[2] = {0}, *pos[2] = {0}, *start[2] = {0};
+  uint8_t *fd = NULL;
+  double *v64 = NULL;
+  uint32_t Fprev = 0;
+  int rc = dcoo_upload(&c, t, 1);
+  if (rc != SPLATT_SUCCESS) goto out;
+  rc = SPLATT_ERROR_NOMEMORY;
+  uint64_t const tw = adatm_k_sort_temp_words(n) + adatm_k_scan_temp_words(n);
+  if (hipMalloc((void **)&perm, n * 4) || hipMalloc((void **)&keys, n * 4) ||
+      hipMalloc((void **)&stemp, tw * 4) || hipMalloc((void **)&fd, n) ||
+      hipMalloc((void **)&tot, 16) || hipMalloc((void **)&v64, n * 8) ||
+      hipMalloc((void **)&ids, n * 4) || hipMalloc((void **)&ptr, (n + 1) * 4))
+    goto out;
+  for (uint32_t l = 0; l < N; ++l)
+    if (hipMalloc((void **)&sc[l], n * 4)) goto out;
+  for (int k = 0; k < 2; ++k)
+    if (hipMalloc((void **)&flags[k], n * 4) || hipMalloc((void **)&pos[k], n * 4) ||
+        hipMalloc((void **)&start[k], n * 4))
+      goto out;
+  h->N = N;
+  h->nnz = n;
+  memcpy(h->order, order, N * sizeof(uint32_t));
+  for (uint32_t l = 0; l < N; ++l) {
+    h->fids[l] = (uint32_t *)malloc(n * sizeof(uint32_t)); /* trimmed below */
+    if (!h->fids[l]) goto out;
+  }
+  h->vals = (double *)malloc(n * sizeof(double));
+  if (!h->vals) goto out;
+  if ((rc = dcoo_lexsort(s, &c, order, perm, keys, stemp)) != SPLATT_SUCCESS) goto out;
+  rc = SPLATT_ERROR_DEVICE;
+  for (uint32_t l = 0; l < N; ++l)
+    if (adatm_k_gather(s, sc[l], c.ind[order[l]], perm, n)) goto out;
+  if (adatm_k_gather_vals(s, KP_FP64, v64, c.vals, perm, n)) goto out;
+  if (hipMemcpyAsync(h->vals, v64, n * 8, hipMemcpyDeviceToHost, s) != hipSuccess) goto out;
+  if (adatm_k_first_diff(s, (uint32_t const *const *)sc, N, n, fd)) goto out;
+  for (uint32_t l = 0; l < N; ++l) {
+    int const cur = l & 1, prv = cur ^ 1;
+    uint32_t Fl = (uint32_t)n;
+    if (l + 1 < N) {
+      if (adatm_k_level_flags(s, fd, n, l, flags[cur])) goto out;
+      if (dscan(s, flags[cur], pos[cur], n, stemp, tot, &Fl) != SPLATT_SUCCESS) goto out;
+      if (adatm_k_level_emit(s, flags[cur], pos[cur], sc[l], n, ids, start[cur])) goto out;
+      if (hipMemcpyAsync(h->fids[l], ids, (size_t)Fl * 4, hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+        goto out;
+    } else if (hipMemcpyAsync(h->fids[l], sc[l], n * 4, hipMemcpyDeviceToHost, s) !=
+               hipSuccess) {
+      goto out;
+    }
+    h->F[l] = Fl;
+    if (l >= 1) {
+      /* child pointers of level l-1 into level l */
+      if (adatm_k_fptr(s, start[prv], Fprev, l + 1 < N ? pos[cur] : NULL, Fl, ptr)) goto out;
+      h->fptr[l - 1] = (uint32_t *)malloc(((size_t)Fprev + 1) * sizeof(uint32_t));
+      if (!h->fptr[l - 1] ||
+          hipMemcpyAsync(h->fptr[l - 1], ptr, ((size_t)Fprev + 1) * 4,
+                         hipMemcpyDeviceToHost, s) != hipSuccess)
+        goto out;
+    }
+    /* the downloads read ids/ptr before the next level overwrites them */
+    if (hipStreamSynchronize(s) != hipSuccess) goto out;
+    Fprev = Fl;
+  }
+  for (uint32_t l = 0; l + 1 < N; ++l) {
+    uint32_t *shrunk = (uint32_t *)realloc(h->fids[l], (h->F[l] ? h->F[l] : 1) * 4);
+    if (shrunk) h->fids[l] = shrunk;
+  }
+  rc = SPLATT_SUCCESS;
+out:
+  if (hipStreamSynchronize(s) != hipSuccess && rc == SPLATT_SUCCESS)
+    rc = hip_fail(hipGetLastError(), "device CSF build");
+  hipFree(perm); hipFree(keys); hipFree(stemp); hipFree(fd); hipFree(tot);
+  hipFree(v64); hipFree(ids); hipFree(ptr);
+  for (uint32_t l = 0; l < N; ++l) hipFree(sc[l]);
+  for (int k = 0; k < 2; ++k) {
+    hipFree(flags[k]);
+    hipFree(pos[k]);
+    hipFree(start[k]);
+  }
+  dcoo_free(&c);
+  hipStreamDestroy(s);
+  if (rc != SPLATT_SUCCESS) {
+    if (!adatm_last_error()[0]) adatm_set_error("device CSF build failed (%d)", rc);
+    adatm_hcsf_free(h);
+  }
+  return rc;
+}
+
 /* Work list of a segmented reduction over segments cp[0..nseg] on the device
  * (light items start where the bucket cp/tile changes, long segments are cut
  * into tile-sized pieces with carry slots). */
@@ -1278,10 +1386,33 @@ int adatm_engine_create(adatm_coo const *t, uint64_t rank,
   return adatm_engine_create_tiled(t, rank, plan, precision, device, 0, 0, out);
 }
 
+static int engine_create_dev(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                             int precision, int device, uint32_t red_tile,
+                             uint32_t scat_tile, uint32_t const *use_grp,
+                             uint32_t const *use_tag, adatm_engine **out);
+
 int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
                               adatm_plan const *plan, int precision, int device,
                               uint32_t red_tile, uint32_t scat_tile,
                               adatm_engine **out)
+{
+  return engine_create_dev(t, rank, plan, precision, device, red_tile, scat_tile, NULL,
+                           NULL, out);
+}
+
+/* built on the device with the caller's mode -> (group, tag) map (the SPLATT
+ * layer's plain and space-efficient CSF sets) */
+int adatm_engine_create_use(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                            int precision, int device, uint32_t const *use_grp,
+                            uint32_t const *use_tag, adatm_engine **out)
+{
+  return engine_create_dev(t, rank, plan, precision, device, 0, 0, use_grp, use_tag, out);
+}
+
+static int engine_create_dev(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                             int precision, int device, uint32_t red_tile,
+                             uint32_t scat_tile, uint32_t const *use_grp,
+                             uint32_t const *use_tag, adatm_engine **out)
 {
   *out = NULL;
   uint32_t const N = t->nmodes;
@@ -1298,7 +1429,7 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
   e->red_tile = red_tile;
   e->scat_tile = scat_tile;
   int rc = engine_begin(e, N, t->dims, t->nnz, rank, plan, precision, device,
-                        NULL, NULL);
+                        use_grp, use_tag);
   dcoo c;
   memset(&c, 0, sizeof(c));
   if (rc == SPLATT_SUCCESS) rc = dcoo_upload(&c, t, 1);

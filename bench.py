@@ -99,6 +99,8 @@ def main() -> int:
                     help="draws of the single-thread CPU oracle run (0 = skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-convert", action="store_true",
+                    help="skip timing splatt_csf_convert on the workload")
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default work-item tiles instead of the autotuner's")
@@ -218,6 +220,10 @@ def main() -> int:
     mttkrp_bytes = sum(pk[16 + c] for c in MTTKRP_CLASSES)
     traffic = measured_traffic(args.config, template_name(dcls, dinfo, t.nmodes))
 
+    convert_s = None if args.no_convert else splatt_convert_seconds(A, t)
+    if convert_s is not None:
+        log(f"splatt_csf_convert (device plan and CSF build): {convert_s:.2f}s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, R, A, t)
@@ -268,6 +274,7 @@ def main() -> int:
         "fit_learned_fp32": r32["fit_last"],
         "fit_plain_fp64": r64["fit_last"] if r64 else None,
         "mttkrp_rel_err_fp32_vs_fp64": errs,
+        "splatt_csf_convert_s": round(convert_s, 3) if convert_s is not None else None,
         "bf16_rows_ms_per_sweep": (round(res["learned_bf16"]["ms_per_sweep"], 4)
                                    if "learned_bf16" in res else None),
         "fit_bf16_rows": res["learned_bf16"]["fit_last"] if "learned_bf16" in res else None,
@@ -278,6 +285,40 @@ def main() -> int:
     }
     print(json.dumps(finite(out), allow_nan=False), flush=True)
     return 0
+
+
+def splatt_convert_seconds(A, t) -> float:
+    """Wall time of the SPLATT API's splatt_csf_convert on the workload's
+    coordinates (64-bit indices, default options: the analytic plan with
+    device fiber counts, every group's CSF built on the device and handed
+    back as SPLATT arrays)."""
+    import ctypes as C
+    import numpy as np
+
+    L = A.lib()
+    L.splatt_default_opts.restype = C.POINTER(C.c_double)
+    L.splatt_csf_convert.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.POINTER(C.c_uint64)),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_double)]
+    L.splatt_free_csf.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.splatt_free_opts.argtypes = [C.POINTER(C.c_double)]
+    cols = [np.ascontiguousarray(t.ind(m), dtype=np.uint64) for m in range(t.nmodes)]
+    ip = (C.POINTER(C.c_uint64) * t.nmodes)(
+        *[c.ctypes.data_as(C.POINTER(C.c_uint64)) for c in cols])
+    v = np.ascontiguousarray(t.vals, dtype=np.float64)
+    opts = L.splatt_default_opts()
+    out = C.c_void_p()
+    t0 = time.perf_counter()
+    rc = L.splatt_csf_convert(t.nmodes, t.nnz, ip, v.ctypes.data_as(C.POINTER(C.c_double)),
+                              C.byref(out), opts)
+    sec = time.perf_counter() - t0
+    if rc != A.SPLATT_SUCCESS:
+        log(f"splatt_csf_convert failed: {L.adatm_last_error().decode()}")
+        sec = None
+    else:
+        L.splatt_free_csf(out, opts)
+    L.splatt_free_opts(opts)
+    return sec
 
 
 C5_SEED = 0xADA7A005

@@ -10,7 +10,8 @@
  * Additions (all appended, no existing value moves):
  *   - options SPLATT_OPTION_ADATM_* after SPLATT_OPTION_COMM
  *     (SPLATT_OPTION_NOPTIONS grows; arrays come from splatt_default_opts()),
- *   - error code SPLATT_ERROR_DEVICE after SPLATT_ERROR_NOMEMORY,
+ *   - error codes SPLATT_ERROR_DEVICE, SPLATT_ERROR_NUMERIC after
+ *     SPLATT_ERROR_NOMEMORY,
  *   - sptensor_t / matrix_t, which SPLATT keeps in internal headers,
  *   - declarations of the AdaTM entry points that the reference defines but
  *     never declares publicly (splatt_cpd_als_adaptive, planner functions).
@@ -365,6 +366,37 @@ void mttkrp_csf_adaptive(splatt_csf const *const tensors,
                          group_properties *const grp_prop, int const n_grp,
                          splatt_idx_t const use_csf, splatt_idx_t const use_tag,
                          double const *const opts);
+
+/* SPLATT's MPI rank information (splatt_mpi.h, not distributed with the
+ * reference); accepted for signature compatibility and ignored. */
+typedef struct rank_info rank_info;
+
+/* The ALS loop (reference adatm_cpd.h:29-41, adatm_cpd.c:256-374) on caller
+ * matrices: mats[m] (dims[m] x nfactors) hold the initial factors and
+ * receive the final ones (unit 2-norm columns, the norms in lambda); the
+ * plan must be the one the CSFs were built with.  Returns the final fit (NaN
+ * on failure, message in adatm_last_error()). */
+double cpd_als_iterate_adaptive(splatt_csf const *const tensors,
+                                rcsf_seq_adaptive *const rs_seq,
+                                splatt_idx_t const n_csf,
+                                group_properties const *const grp_prop,
+                                splatt_idx_t const n_grp,
+                                splatt_idx_t const *const use_csfs,
+                                splatt_idx_t const *const use_tags, matrix_t **mats,
+                                splatt_val_t *const lambda, splatt_idx_t const nfactors,
+                                rank_info *const rinfo, double const *const opts);
+
+/* SPLATT-internal helpers the AdaTM drivers call (cmd_bench.c:234-266,
+ * cmd_cpd.c:139-154, 255, adatm_bench.c:650): tt_read reads the coordinate
+ * text format of splatt_csf_load (dims = largest index + 1); csf_free and
+ * splatt_csf_free are splatt_free_csf; mat_alloc gives a zeroed row-major
+ * matrix. */
+sptensor_t *tt_read(char const *const fname);
+void tt_free(sptensor_t *tt);
+void csf_free(splatt_csf *tensors, double const *const opts);
+void splatt_csf_free(splatt_csf *tensors, double const *const opts);
+matrix_t *mat_alloc(splatt_idx_t const nrows, splatt_idx_t const ncols);
+void mat_free(matrix_t *M);
 
 /* CP-ALS over a memoization plan (reference adatm_cpd.c:205-253). */
 int splatt_cpd_als_adaptive(splatt_csf const *const tensors,

@@ -99,7 +99,10 @@ def lex_order(inds: np.ndarray, order) -> np.ndarray:
 
 
 def count_fibers(inds: np.ndarray, order) -> list[int]:
-    """F_l = number of distinct length-(l+1) prefixes (count_nfibs)."""
+    """F_l = number of distinct length-(l+1) prefixes for l < N-1, and
+    F_{N-1} = nnz: every nonzero is a leaf, repeated coordinates included
+    (count_nfibs, src/adatm_rcsf.c:410-500, with nfibs[N-1] = nnz set at
+    src/adatm_bench.c:181)."""
     N = inds.shape[0]
     p = lex_order(inds, order)
     s = inds[list(order)][:, p]
@@ -108,9 +111,10 @@ def count_fibers(inds: np.ndarray, order) -> list[int]:
         return [0] * N
     diff = np.zeros(s.shape[1], dtype=bool)
     diff[0] = True
-    for l in range(N):
+    for l in range(N - 1):
         diff[1:] |= s[l, 1:] != s[l, :-1]
         F.append(int(diff.sum()))
+    F.append(int(s.shape[1]))
     return F
 
 

@@ -182,3 +182,79 @@ def test_space_efficient_plans(lib, groups, N, kept, grp, tag):
     q, g, t = A.plan_space_efficient(_mkplan(groups, N), N)
     assert [q.grp[i].memo_mode for i in range(q.n_grp)] == kept
     assert g == grp and t == tag
+
+
+# ---------------------------------------------------------------------------
+# Hand-derived known answers of the analytic model (adatm_bench.c:145-272,
+# adatm_rcsf.c:329-406) on two tiny tensors, rank 2 (8-byte values and ids).
+# Per-level fiber counts F_l (distinct length-(l+1) prefixes; F_{N-1} = nnz)
+# are counted by hand for every level order the candidates use.  With those:
+#   csf_bytes(F) = 16 nnz + sum_{l<N-1} (16 F_l + 8)
+#   csf_ops(F)   = 2R sum_l F_l
+#   rcsf_bytes   = sum_{i<n_imten} F[b - i] R 8   (D6: the reference reads
+#                  level b-i; the memo it allocates is level b-1-i)
+#   rcsf_ops     = sum_{i=1..nre} 2R (F[end-1] if i < end else F[i]),
+#                  end = b - n_imten + 1, nre = next memo mode - memo mode - 1
+# D5: only the first candidate of a run of equal group counts recounts; the
+# later ones keep its csf_bytes/csf_ops and read F of its LAST group.  Every
+# candidate at N = 4 and 5 has two groups, so all but the first are stale.
+KA_N4 = dict(
+    dims=[5, 4, 3, 2],
+    coords=[(0, 0, 0, 0), (0, 0, 1, 1), (0, 1, 1, 0), (1, 0, 0, 0), (1, 0, 0, 1),
+            (2, 3, 2, 1), (4, 1, 0, 0)],
+    # F by order: [0,1,2,3] (4,5,6,7); [3,2,1,0] (2,5,6,7);
+    #             [0,1,3,2] (4,5,7,7); [2,3,1,0] (3,5,6,7)
+    fibers={(0, 1, 2, 3): [4, 5, 6, 7], (3, 2, 1, 0): [2, 5, 6, 7],
+            (0, 1, 3, 2): [4, 5, 7, 7], (2, 3, 1, 0): [3, 5, 6, 7]},
+    # c0 [(0,3,2) o=[0,1,2,3], (3,1,0) o=[3,2,1,0]]: csf 376+344, ops 88+80;
+    #    rcsf bytes (F[3]+F[2]) 16 = 208; ops end=2: F[1]4 + F[2]4 = 20+24
+    # c1 [(0,3,1),(3,1,0)] stale, F of [3,2,1,0]: bytes F[3]16 = 112;
+    #    ops end=3: F[2]4 twice = 48
+    # c2 [(0,2,1) o=[0,1,3,2], (2,2,1) o=[2,3,1,0]] stale, F of [3,2,1,0]:
+    #    bytes F[2]16 per group = 96+96; ops end=2, nre=1: F[1]4 per group = 20+20
+    quirks=[(928, 212), (832, 216), (912, 208)], quirks_choice=2,
+    # corrected: recounted per candidate, memo level b-1-i
+    # c0 bytes (F[2]+F[1])16 = 176; c1 F[2]16 = 96, ops 24+24;
+    # c2 csf 392+360, ops 92+84, bytes F_C[1]16 + F_D[1]16, ops 20+20
+    fresh=[(896, 212), (816, 216), (912, 216)], fresh_choice=0)
+KA_N5 = dict(
+    dims=[6, 5, 4, 3, 2],
+    coords=[(0, 0, 0, 0, 0), (0, 0, 1, 2, 1), (0, 2, 1, 0, 0), (3, 0, 0, 0, 1),
+            (3, 0, 0, 1, 1), (5, 4, 3, 2, 0)],
+    fibers={(0, 1, 2, 3, 4): [3, 4, 5, 6, 6], (4, 3, 2, 1, 0): [2, 5, 6, 6, 6],
+            (0, 1, 2, 4, 3): [3, 4, 5, 5, 6], (3, 4, 2, 1, 0): [3, 5, 6, 6, 6]},
+    # c0 [(0,4,3) [0..4], (4,1,0) [4..0]]: csf 416+432, ops 96+100;
+    #    bytes (F[4]+F[3]+F[2])16 = 272; ops end=2, nre=3: 16+20+24
+    # c1 [(0,4,2),..] stale F=(2,5,6,6,6): bytes (6+6)16; ops end=3: 24+24+24
+    # c2 [(0,4,1),..] stale: bytes 6*16; ops end=4: 24*3
+    # c3 [(0,3,2) [0,1,2,4,3], (3,2,1) [3,4,2,1,0]] stale: g0 bytes (F[3]+F[2])16
+    #    = 192, ops end=2, nre=2: F[1]4 + F[2]4 = 20+24; g1 bytes F[2]16 = 96,
+    #    ops end=2, nre=1: F[1]4 = 20
+    quirks=[(1120, 256), (1040, 268), (944, 268), (1136, 260)], quirks_choice=0,
+    # corrected: c0 bytes (F[3]+F[2]+F[1])16 = 240; c1 (6+5)16, ops 20+20+24;
+    # c2 6*16, ops 24*3; c3 csf 400+448, ops 92+104, g0 bytes (5+4)16, ops 16+20,
+    # g1 bytes 5*16, ops 20
+    fresh=[(1088, 256), (1024, 260), (944, 268), (1072, 252)], fresh_choice=3)
+
+
+@pytest.mark.parametrize("ka", [KA_N4, KA_N5], ids=["N4", "N5"])
+def test_analytic_model_known_answers(ka):
+    dims, coords = ka["dims"], ka["coords"]
+    N = len(dims)
+    inds = np.array(coords, dtype=np.int64).T
+    vals = np.arange(1, inds.shape[1] + 1, dtype=np.float64)
+    for o, F in ka["fibers"].items():  # the hand counts themselves
+        assert O.count_fibers(inds, list(o)) == F, o
+    t = A.Tensor.from_arrays(inds, vals, dims)
+    for quirks, key in ((True, "quirks"), (False, "fresh")):
+        cands, ch = A.plan_analytic(t, 2, ref_quirks=quirks, device=-1)
+        assert [(p.pspace, p.pops) for p in cands] == ka[key], key
+        assert ch == ka[key + "_choice"], key
+        # the Python restatement gives the same numbers
+        configs = O.enumerate_plans(N, dims)
+        pred = O.predict(configs, lambda o: O.count_fibers(inds, list(o)), inds.shape[1], N,
+                         2, quirks=quirks)
+        assert pred == ka[key] and O.choose(pred) == ka[key + "_choice"], key
+    # the orders the candidates use are the ones counted above
+    used = {tuple(g[3]) for c in O.enumerate_plans(N, dims) for g in c}
+    assert used == set(ka["fibers"])

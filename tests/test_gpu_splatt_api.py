@@ -199,10 +199,12 @@ def test_planner_options(planner, tmp_path, monkeypatch):
 
 def test_cli_bench_and_cpd(tmp_path):
     exe = str(A.CLI_PATH)
-    r = subprocess.run([exe, "bench", "synth:1:200000", "-a", "adatm", "-i", "2", "-r", "16"],
+    print("cli: bench on the full 4-mode workload", flush=True)
+    r = subprocess.run([exe, "bench", "synth:1", "-a", "adatm", "-i", "2", "-r", "16"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "CSF-STORAGE" in r.stdout and "vCSF-STORAGE" in r.stdout
+    assert "NNZ=2000000" in r.stdout
     assert r.stdout.count("  mode ") == 8 and "its =   2" in r.stdout
     r = subprocess.run([exe, "cpd", "synth:1:200000", "-r", "8", "-i", "5", "--tol", "0",
                         "--seed", "3"], capture_output=True, text=True, timeout=300,
@@ -219,3 +221,112 @@ def test_cli_bench_and_cpd(tmp_path):
                            capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
         assert label in r.stdout and "its =   1" in r.stdout
+
+
+def test_cli_outputs_match_the_oracle(tmp_path):
+    """`splatt bench -a adatm -a csf -w` writes each algorithm's MTTKRPs
+    (factors from seed 1) and `splatt cpd` its fits and factors: both against
+    the oracle on the canonical form of the fixture file (the CLI sums
+    duplicates and drops empty indices on input).  The reference's
+    reordering options are refused, -s scales the thread count by 2."""
+    exe = str(A.CLI_PATH)
+    tns = os.path.join(os.path.dirname(__file__), "golden", "small4.tns")
+    a = np.loadtxt(tns, comments="#", ndmin=2)
+    inds0 = a[:, :4].T.astype(np.int64) - 1
+    ci, cv, cd, _ = O.canonicalize(inds0, a[:, 4].copy(), [int(x) + 1 for x in inds0.max(axis=1)])
+    R = 8
+    r = subprocess.run([exe, "bench", tns, "-a", "adatm", "-a", "csf", "-w", "-i", "1",
+                        "-r", str(R)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    U = [A.rand_matrix(1, m, d, R) for m, d in enumerate(cd)]
+    for alg in ("adatm", "csf"):
+        for m in range(4):
+            M = np.loadtxt(tmp_path / f"{alg}_mode{m + 1}.mat").reshape(cd[m], R)
+            assert O.rel_fro(M, O.coo_mttkrp(ci, cv, U, m)) <= 1e-7, (alg, m)
+    r = subprocess.run([exe, "cpd", tns, "-r", str(R), "-i", "10", "--tol", "0", "--seed", "3"],
+                       capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    fits = [float(l.split("fit = ")[1].split()[0]) for l in r.stdout.splitlines()
+            if "fit = " in l]
+    U3 = [A.rand_matrix(3, m, d, R) for m, d in enumerate(cd)]
+    ofits, oA, olam = O.cpd_als(lambda m, F: O.coo_mttkrp(ci, cv, F, m), U3, 10,
+                                float((cv ** 2).sum()))
+    assert len(fits) == 10 and np.max(np.abs(np.array(fits) - ofits)) <= 1e-5
+    lam = np.loadtxt(tmp_path / "lambda.mat")
+    for m in range(4):
+        s_ = np.sqrt((oA[m] ** 2).sum(axis=0))
+        olam = olam * s_
+        F = np.loadtxt(tmp_path / f"mode{m + 1}.mat")
+        assert O.rel_fro(F, oA[m] / s_) <= 1e-6, m
+    assert O.rel_fro(lam, olam) <= 1e-6
+    for opt in (["-p", "part.txt"], ["-z", "graph"], ["-m", "2"]):
+        r = subprocess.run([exe, "bench", tns, "-a", "adatm", *opt], capture_output=True,
+                           text=True, timeout=60, cwd=tmp_path)
+        assert r.returncode != 0 and "reordering is not supported" in r.stderr, opt
+    r = subprocess.run([exe, "bench", tns, "-a", "splatt"], capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode != 0 and "not recognized" in r.stderr
+    r = subprocess.run([exe, "bench", tns, "-a", "adatm", "-s", "-t", "4", "-i", "1",
+                        "--tile"], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert [l for l in r.stdout.splitlines() if l.startswith("## THREADS")] == [
+        "## THREADS 1", "## THREADS 2", "## THREADS 4"]
+
+
+class CsfSparsity(C.Structure):
+    _fields_ = [("nfibs", C.c_uint64 * MAXM), ("fptr", C.POINTER(C.c_uint64) * MAXM),
+                ("fids", C.POINTER(C.c_uint64) * MAXM), ("vals", C.POINTER(C.c_double))]
+
+
+class SplattCsf(C.Structure):
+    _fields_ = [("nnz", C.c_uint64), ("nmodes", C.c_uint64), ("dims", C.c_uint64 * MAXM),
+                ("dim_perm", C.c_uint64 * MAXM), ("which_tile", C.c_int),
+                ("ntiles", C.c_uint64), ("tile_dims", C.c_uint64 * MAXM),
+                ("pt", C.POINTER(CsfSparsity))]
+
+
+def _csf_arrays(ptr, n):
+    """(dim_perm, nfibs, fptr, fids, vals) of the n CSFs of a SPLATT array"""
+    arr = C.cast(ptr, C.POINTER(SplattCsf))
+    out = []
+    for g in range(n):
+        c = arr[g]
+        N, nnz = int(c.nmodes), int(c.nnz)
+        pt = c.pt[0]
+        F = [int(pt.nfibs[l]) for l in range(N)]
+        fptr = [np.ctypeslib.as_array(pt.fptr[l], shape=(F[l] + 1,)).copy()
+                for l in range(N - 1)]
+        fids = [np.ctypeslib.as_array(pt.fids[l], shape=(F[l],)).copy() for l in range(N)]
+        vals = np.ctypeslib.as_array(pt.vals, shape=(nnz,)).copy()
+        out.append(([int(c.dim_perm[l]) for l in range(N)], F, fptr, fids, vals))
+    return out
+
+
+@pytest.mark.parametrize("alloc", [A.ADAPTIVE_TIME_EFFICIENT, A.CSF_TWOMODE, A.CSF_ALLMODE])
+def test_device_built_csf_equals_host_built(alloc):
+    """splatt_csf_convert builds the SPLATT arrays on the GPU (radix sort and
+    scans) by default: plan, fptr, fids and value order are bit-identical to
+    the host build's (device -1), with duplicates and empty slices."""
+    print(f"device vs host CSF alloc={alloc}", flush=True)
+    L = _lib()
+    dims = [60, 45, 30, 20, 9]
+    inds, vals = _tensor(17, dims, 50000)
+    built = []
+    for dev in (0, -1):
+        opts = L.splatt_default_opts()
+        opts[A.OPT_CSF_ALLOC] = alloc
+        opts[A.OPT_DEVICE] = dev
+        csf = _convert(L, inds, vals, opts)
+        n = {A.CSF_TWOMODE: 2, A.CSF_ALLMODE: 5}.get(alloc)
+        if n is None:
+            t = A.Tensor.from_arrays(inds, vals, [int(x) + 1 for x in inds.max(axis=1)])
+            cands, ch = A.plan_analytic(t, 16, device=-1)
+            n = cands[ch].n_grp
+        built.append(_csf_arrays(csf, n))
+        L.splatt_free_csf(csf, opts)
+        L.splatt_free_opts(opts)
+    assert len(built[0]) == len(built[1])
+    for a, b in zip(*built):
+        assert a[0] == b[0] and a[1] == b[1]
+        for x, y in zip(a[2] + a[3] + [a[4]], b[2] + b[3] + [b[4]]):
+            assert np.array_equal(x, y)
