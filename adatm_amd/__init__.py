@@ -24,6 +24,11 @@ SPLATT_ERROR_DEVICE = 4
 SPLATT_ERROR_NUMERIC = 5
 
 FP64, FP32, BF16ROWS = 0, 1, 2
+ENGINE_STAGED = 0x1
+
+
+def ENGINE_STAGED_GROUP(g: int) -> int:
+    return 0x100 << g
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16rows": BF16ROWS}
 
 DIST_UNIFORM, DIST_ZIPF, DIST_CLUSTER = 0, 1, 2
@@ -98,9 +103,9 @@ class Plan(C.Structure):
 _lib = None
 
 
-# timer(ctx, reduction tile, scatter tile, batch, &ms) of adatm_autotune_search
+# timer(ctx, reduction tile, scatter tile, batch, flags, &ms) of adatm_autotune_search
 TILE_TIMER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
-                         C.POINTER(C.c_double))
+                         C.c_uint32, C.POINTER(C.c_double))
 
 
 def lib() -> C.CDLL:
@@ -136,9 +141,12 @@ def lib() -> C.CDLL:
         "adatm_engine_destroy": (None, [C.c_void_p]),
         "adatm_engine_create_tiled": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
                                                 C.c_uint32, C.c_uint32, P(C.c_void_p)]),
+        "adatm_engine_create_ex": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
+                                             C.c_uint32, C.c_uint32, C.c_uint32,
+                                             P(C.c_void_p)]),
         "adatm_autotune_tiles": (C.c_int, [P(Coo), C.c_uint64, P(Plan), C.c_int, C.c_int,
                                            C.c_int, P(C.c_uint32), P(C.c_uint32),
-                                           P(C.c_uint32), P(C.c_double)]),
+                                           P(C.c_uint32), P(C.c_uint32), P(C.c_double)]),
         "adatm_engine_set_batch": (C.c_int, [C.c_void_p, C.c_uint32]),
         "adatm_engine_set_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_engine_get_factor": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
@@ -199,8 +207,9 @@ def lib() -> C.CDLL:
         "adatm_engine_cpd_sweep": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int,
                                              P(C.c_double)]),
         "adatm_engine_solve_status": (C.c_int, [C.c_void_p, P(C.c_int)]),
-        "adatm_autotune_search": (C.c_int, [TILE_TIMER, C.c_void_p, P(C.c_uint32),
-                                            P(C.c_uint32), P(C.c_uint32), P(C.c_double)]),
+        "adatm_autotune_search": (C.c_int, [TILE_TIMER, C.c_void_p, C.c_uint32, P(C.c_uint32),
+                                            P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                            P(C.c_double)]),
         "adatm_k_bf16_bits": (C.c_uint16, [C.c_float]),
     }
     for name, (res, args) in sig.items():
@@ -534,29 +543,29 @@ def plan_exhaustive(t: Tensor, rank: int, precision: int = FP32, device: int = 0
 
 def autotune_tiles(t: Tensor, rank: int, plan: Plan, precision: int = FP32,
                    device: int = 0, nrep: int = 3):
-    """((reduction tile, scatter tile, children per batch), best ms per
-    sweep) by timing; pass the triple as Engine(..., tiles=...)."""
-    r = C.c_uint32()
-    s = C.c_uint32()
-    b = C.c_uint32()
+    """((reduction tile, scatter tile, children per batch, creation flags),
+    best ms per sweep) by timing; pass the tuple as Engine(..., tiles=...)."""
+    r, s_, b, f = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
     ms = C.c_double()
     check(lib().adatm_autotune_tiles(C.byref(t.c), rank, C.byref(plan), precision, device,
-                                     nrep, C.byref(r), C.byref(s), C.byref(b), C.byref(ms)),
+                                     nrep, C.byref(r), C.byref(s_), C.byref(b), C.byref(f),
+                                     C.byref(ms)),
           "autotune_tiles")
-    return (r.value, s.value, b.value), ms.value
+    return (r.value, s_.value, b.value, f.value), ms.value
 
 
-def autotune_search(timer):
-    """The autotuner's search driven by timer(red_tile, scat_tile, batch) ->
-    ms; returns ((red_tile, scat_tile, batch), best ms)."""
-    def cb(_ctx, r, s_, b, ms):
-        ms[0] = float(timer(int(r), int(s_), int(b)))
+def autotune_search(timer, groups: int = 0):
+    """The autotuner's search driven by timer(red_tile, scat_tile, batch,
+    flags) -> ms over the memo groups in the mask `groups`; returns
+    ((red_tile, scat_tile, batch, flags), best ms)."""
+    def cb(_ctx, r, s_, b, f, ms):
+        ms[0] = float(timer(int(r), int(s_), int(b), int(f)))
         return SPLATT_SUCCESS
     fn = TILE_TIMER(cb)
-    r, s_, b, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
-    check(lib().adatm_autotune_search(fn, None, C.byref(r), C.byref(s_), C.byref(b),
-                                      C.byref(ms)), "autotune_search")
-    return (r.value, s_.value, b.value), ms.value
+    r, s_, b, f, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+    check(lib().adatm_autotune_search(fn, None, groups, C.byref(r), C.byref(s_), C.byref(b),
+                                      C.byref(f), C.byref(ms)), "autotune_search")
+    return (r.value, s_.value, b.value, f.value), ms.value
 
 
 def bf16_bits(x: float) -> int:
@@ -574,16 +583,22 @@ class Engine:
     """A memoization plan resident on one GPU (wraps adatm_engine)."""
 
     def __init__(self, t: Tensor, rank: int, plan: Plan, precision: int = FP64,
-                 device: int = 0, tiles=(0, 0)):
+                 device: int = 0, tiles=(0, 0), staged: bool = False, flags: int = 0):
+        """staged: root MTTKRPs stop at every memo level (ENGINE_STAGED)
+        instead of one leaf-to-root pass writing the memo levels beside;
+        flags: any other ENGINE_* creation flags"""
         self.h = C.c_void_p()
         self.N = t.nmodes
         self.R = int(rank)
         self.dims = t.dims
         self.plan = plan
         self.precision = precision
-        check(lib().adatm_engine_create_tiled(C.byref(t.c), rank, C.byref(plan), precision,
-                                              device, int(tiles[0]), int(tiles[1]),
-                                              C.byref(self.h)), "engine_create")
+        if len(tiles) > 3:  # the autotuner's creation flags
+            flags |= int(tiles[3])
+        check(lib().adatm_engine_create_ex(C.byref(t.c), rank, C.byref(plan), precision,
+                                           device, int(tiles[0]), int(tiles[1]),
+                                           (ENGINE_STAGED if staged else 0) | flags,
+                                           C.byref(self.h)), "engine_create")
         if len(tiles) > 2 and tiles[2]:
             self.set_batch(int(tiles[2]))
 

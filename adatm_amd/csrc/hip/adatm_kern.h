@@ -64,6 +64,16 @@ typedef struct {
   uint8_t const *cfd;
   uint32_t const *ids[ADATM_KMAXFUSE + 1];
   void const *U[ADATM_KMAXFUSE + 1];
+  /* side outputs (leaf passes): sout[j] != NULL makes the pass also write
+   * the rows of relative level j (the running sum of each node when it
+   * closes, before the fold with its own factor row) into sout[j] by node
+   * index.  A node that the item's ends cut writes its partial row to
+   * spart[j] at slot shead[k*K+j] (the item's first node, cut before the
+   * item) or stail[k*K+j] (its last node, cut after it); ~0 = not cut. */
+  uint32_t side;
+  void *sout[ADATM_KMAXFUSE + 1];
+  void *spart[ADATM_KMAXFUSE + 1];
+  uint32_t const *shead, *stail;
 } adatm_kchain;
 
 /* Z_top(p) = sum over bottom descendants c of p of in(c) o U_K[ids_K[c]],
@@ -216,6 +226,21 @@ int adatm_k_fix_emit(hipStream_t s, uint32_t const *hslot, uint32_t nheavy,
 int adatm_k_items_anc(hipStream_t s, uint32_t const *start, uint32_t nitems,
                       uint32_t lb, uint32_t lt, uint32_t const *const *par,
                       uint32_t *anc);
+/* side-level cuts of a work list (see k_side_count / k_side_emit): per item
+ * the partial rows of the level-L nodes its ends cut, their slots in item
+ * order, and the runs of slots that belong to one node */
+int adatm_k_side_count(hipStream_t s, uint32_t const *start, uint32_t nitems,
+                       uint32_t lb, uint32_t L, uint64_t Fb, uint8_t const *cfd,
+                       uint32_t const *const *par, uint32_t *cnt, uint32_t *nf,
+                       uint32_t *nl, uint32_t *fl);
+int adatm_k_side_emit(hipStream_t s, uint32_t nitems, uint32_t K, uint32_t j,
+                      uint32_t const *cnt, uint32_t const *base, uint32_t const *nf,
+                      uint32_t const *nl, uint32_t const *fl, uint32_t *head,
+                      uint32_t *tail, uint32_t *snode);
+int adatm_k_run_flags(hipStream_t s, uint32_t const *v, uint64_t n, uint32_t *flags);
+int adatm_k_run_emit(hipStream_t s, uint32_t const *flags, uint32_t const *pos,
+                     uint32_t const *v, uint64_t n, uint32_t nrun, uint32_t *val,
+                     uint32_t *first);
 int adatm_k_rowptr(hipStream_t s, uint32_t const *skey, uint64_t n,
                    uint32_t nrows, uint32_t *rowptr);
 int adatm_k_scatter_rids(hipStream_t s, uint32_t const *node, uint64_t n,

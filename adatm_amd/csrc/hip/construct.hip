@@ -454,6 +454,109 @@ __global__ void k_items_anc(uint32_t const *__restrict__ start, uint32_t nitems,
   }
 }
 
+/* Nodes of a side level L (lt < L < lb, written by a fused pass beside its
+ * top level) cut by the ends of work item k: the item's first node (the
+ * level-L ancestor of its first position a) is cut when position a-1 shares
+ * it (cfd[a-1] > L), its last node (of position b-1) when position b does
+ * (cfd[b-1] > L).  A cut node gets one partial row per item it spans; a
+ * single node cut at either end one.  cnt[k] = the item's partial rows,
+ * nf[k] / nl[k] its first / last node, fl[k] = 1 (head cut) | 2 (tail cut). */
+__global__ void k_side_count(uint32_t const *__restrict__ start, uint32_t nitems,
+                             uint32_t lb, uint32_t L, uint64_t Fb,
+                             uint8_t const *__restrict__ cfd, par_ptrs par,
+                             uint32_t *__restrict__ cnt, uint32_t *__restrict__ nf,
+                             uint32_t *__restrict__ nl, uint32_t *__restrict__ fl)
+{
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; k < nitems; k += st) {
+    uint32_t const a = start[k], b = start[k + 1];
+    if (b <= a) {
+      cnt[k] = 0;
+      nf[k] = nl[k] = 0;
+      fl[k] = 0;
+      continue;
+    }
+    uint32_t x = a, y = b - 1;
+    for (uint32_t l = lb; l > L; --l) {
+      x = par.p[l][x];
+      y = par.p[l][y];
+    }
+    bool const h = a > 0 && cfd[a - 1] > L;
+    bool const t = (uint64_t)b < Fb && cfd[b - 1] > L;
+    cnt[k] = (x == y) ? ((h || t) ? 1u : 0u) : (uint32_t)h + (uint32_t)t;
+    nf[k] = x;
+    nl[k] = y;
+    fl[k] = (h ? 1u : 0u) | (t ? 2u : 0u);
+  }
+}
+
+/* Partial-row slots in item order, the head's before the tail's, from
+ * base = exclusive scan of cnt (so the partials of one cut node are
+ * consecutive slots); head[k*K+j] / tail[k*K+j] = the slot of item k's
+ * head-cut first / tail-cut last node at relative level j (one slot when
+ * they are the same node), ~0 when not cut; snode[slot] = its node. */
+__global__ void k_side_emit(uint32_t nitems, uint32_t K, uint32_t j,
+                            uint32_t const *__restrict__ cnt,
+                            uint32_t const *__restrict__ base,
+                            uint32_t const *__restrict__ nf,
+                            uint32_t const *__restrict__ nl,
+                            uint32_t const *__restrict__ fl,
+                            uint32_t *__restrict__ head, uint32_t *__restrict__ tail,
+                            uint32_t *__restrict__ snode)
+{
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; k < nitems; k += st) {
+    uint32_t hs = NONE32, ts = NONE32;
+    uint32_t const f = fl[k];
+    if (cnt[k]) {
+      uint32_t q = base[k];
+      if (nf[k] == nl[k]) {
+        hs = (f & 1u) ? q : NONE32;
+        ts = (f & 2u) ? q : NONE32;
+        snode[q] = nf[k];
+      } else {
+        if (f & 1u) {
+          hs = q;
+          snode[q++] = nf[k];
+        }
+        if (f & 2u) {
+          ts = q;
+          snode[q] = nl[k];
+        }
+      }
+    }
+    head[k * K + j] = hs;
+    tail[k * K + j] = ts;
+  }
+}
+
+/* runs of equal values: flags[s] = 1 where a run starts */
+__global__ void k_run_flags(uint32_t const *__restrict__ v, uint64_t n,
+                            uint32_t *__restrict__ flags)
+{
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; q < n; q += st) flags[q] = (q == 0 || v[q] != v[q - 1]) ? 1u : 0u;
+}
+
+/* run r = its value and first index; first[nrun] = n */
+__global__ void k_run_emit(uint32_t const *__restrict__ flags,
+                           uint32_t const *__restrict__ pos,
+                           uint32_t const *__restrict__ v, uint64_t n, uint32_t nrun,
+                           uint32_t *__restrict__ val, uint32_t *__restrict__ first)
+{
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const st = (uint64_t)gridDim.x * blockDim.x;
+  for (; q < n; q += st)
+    if (flags[q]) {
+      val[pos[q]] = v[q];
+      first[pos[q]] = (uint32_t)q;
+    }
+  if (q == n) first[nrun] = (uint32_t)n; /* the thread just past the end */
+}
+
 /* rowptr of ids sorted ascending: first position of every row */
 __global__ void k_rowptr(uint32_t const *__restrict__ skey, uint64_t n,
                          uint32_t nrows, uint32_t *__restrict__ rowptr)
@@ -708,6 +811,44 @@ int adatm_k_items_anc(hipStream_t s, uint32_t const *start, uint32_t nitems,
   for (uint32_t k = 0; k < ADATM_KMAXL; ++k) p.p[k] = par[k];
   hipLaunchKernelGGL(k_items_anc, grid_for(nitems), dim3(256), 0, s, start, nitems,
                      lb, lt, p, anc);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_side_count(hipStream_t s, uint32_t const *start, uint32_t nitems,
+                       uint32_t lb, uint32_t L, uint64_t Fb, uint8_t const *cfd,
+                       uint32_t const *const *par, uint32_t *cnt, uint32_t *nf,
+                       uint32_t *nl, uint32_t *fl)
+{
+  par_ptrs p;
+  for (uint32_t k = 0; k < ADATM_KMAXL; ++k) p.p[k] = par[k];
+  hipLaunchKernelGGL(k_side_count, grid_for(nitems), dim3(256), 0, s, start, nitems, lb, L,
+                     Fb, cfd, p, cnt, nf, nl, fl);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_side_emit(hipStream_t s, uint32_t nitems, uint32_t K, uint32_t j,
+                      uint32_t const *cnt, uint32_t const *base, uint32_t const *nf,
+                      uint32_t const *nl, uint32_t const *fl, uint32_t *head,
+                      uint32_t *tail, uint32_t *snode)
+{
+  hipLaunchKernelGGL(k_side_emit, grid_for(nitems), dim3(256), 0, s, nitems, K, j, cnt,
+                     base, nf, nl, fl, head, tail, snode);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_run_flags(hipStream_t s, uint32_t const *v, uint64_t n, uint32_t *flags)
+{
+  hipLaunchKernelGGL(k_run_flags, grid_for(n), dim3(256), 0, s, v, n, flags);
+  return herr(hipGetLastError());
+}
+
+int adatm_k_run_emit(hipStream_t s, uint32_t const *flags, uint32_t const *pos,
+                     uint32_t const *v, uint64_t n, uint32_t nrun, uint32_t *val,
+                     uint32_t *first)
+{
+  /* one block more than the elements need, so a thread lands on q == n */
+  dim3 g = grid_for(n + 1);
+  hipLaunchKernelGGL(k_run_emit, g, dim3(256), 0, s, flags, pos, v, n, nrun, val, first);
   return herr(hipGetLastError());
 }
 

@@ -51,7 +51,8 @@ namespace {
  * those of the intermediate nodes they belong to -- are issued together,
  * with the per-child metadata read from the holding lane (ds_bpermute; for
  * R > 32, one group per wave, v_readlane into scalars). */
-template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V,
+          bool SIDE>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -102,6 +103,18 @@ __global__ __launch_bounds__(256) void k_reduce(
     for (int v = 0; v < V; ++v) acc[j][v] = (T)0;
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
+  /* SIDE: the partial-row slots of the nodes this item's ends cut, and the
+     item's first node, at every relative level written beside the top */
+  uint32_t shd[K], stl[K], nfst[K];
+  if constexpr (SIDE) {
+#pragma unroll
+    for (int j = 1; j < K; ++j) {
+      bool const on = ch.sout[j] != nullptr;
+      shd[j] = on ? ch.shead[(size_t)k * K + j] : NONE32;
+      stl[j] = on ? ch.stail[(size_t)k * K + j] : NONE32;
+      nfst[j] = node[j];
+    }
+  }
 
   /* window metadata of child w0 + col (past the item's end: its last
      child); the item's last child closes every level */
@@ -128,6 +141,7 @@ __global__ __launch_bounds__(256) void k_reduce(
        is that node's output row (j = 0) or id (j >= 1). */
     uint64_t cm[K];
     uint32_t wn[K];
+    uint32_t wnode[K]; /* SIDE: node index of the lane's child at level j */
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       cm[j] = ((uint64_t)__ballot(wf <= lt + (uint32_t)j) >> gb) & valid;
@@ -136,6 +150,7 @@ __global__ __launch_bounds__(256) void k_reduce(
         wn[0] = ch.orow ? ch.orow[n] : n;
       else
         wn[j] = ch.ids[j][n];
+      if constexpr (SIDE) wnode[j] = n;
       node[j] += (uint32_t)__popcll(cm[j]);
     }
     uint32_t nid = 0, nf = 0;
@@ -177,6 +192,21 @@ __global__ __launch_bounds__(256) void k_reduce(
         for (int j = K - 1; j >= 1; --j) {
           /* fold the level-j node into its parent with its own factor row */
           bool const c = (cb[j] >> i) & 1u;
+          if constexpr (SIDE) {
+            /* a closing node of a side level: its row, or the partial row
+               of a node this item's ends cut */
+            if (c && ch.sout[j]) {
+              uint32_t const nd = getl(wnode[j], s0 + (uint32_t)i);
+              T *dst;
+              if (nd == nfst[j] && shd[j] != NONE32)
+                dst = (T *)ch.spart[j] + (size_t)shd[j] * R;
+              else if (w0 + s0 + (uint32_t)i + 1u == b && stl[j] != NONE32)
+                dst = (T *)ch.spart[j] + (size_t)stl[j] * R;
+              else
+                dst = (T *)ch.sout[j] + (size_t)nd * R;
+              strow<T, V>(dst + cv, acc[j]);
+            }
+          }
 #pragma unroll
           for (int v = 0; v < V; ++v) {
             T const t = acc[j][v] * uc[j][i][v];
@@ -465,7 +495,8 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
   return hipGetLastError();
 }
 
-template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, int V>
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, int V,
+          bool SIDE = false>
 void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                   void const *xin, void *out, void *carry, uint32_t R)
 {
@@ -475,14 +506,14 @@ void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
   dim3 const grid((w->nitems + per_block - 1) / per_block);
   if constexpr (V == 1) {
     if (gpw == 1) {
-      hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, true, 1>), grid, dim3(256),
-                         0, s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
-                         (T const *)xin, (T *)out, (T *)carry, R, gpw);
+      hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, true, 1, SIDE>), grid,
+                         dim3(256), 0, s, w->nitems, w->start, w->seg, w->slot, w->anc,
+                         *ch, (T const *)xin, (T *)out, (T *)carry, R, gpw);
       return;
     }
   }
-  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false, V>), grid, dim3(256), 0,
-                     s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false, V, SIDE>), grid, dim3(256),
+                     0, s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
                      (T const *)xin, (T *)out, (T *)carry, R, gpw);
 }
 
@@ -503,6 +534,23 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
   uint32_t const G = wide ? R / VW : R;
   /* a batch never spans windows: no more children per batch than G */
   uint32_t const rb = (ch->rb > G) ? (G >= 8 ? 8u : 4u) : ch->rb;
+  if constexpr (LEAF && K >= 2) {
+    /* leaf passes writing side levels: batches of 4 or 8, not pipelined */
+    if (ch->side) {
+      bool const r8 = K <= 3 && rb != 4 && G >= 8;
+      if (wide) {
+        if (r8)
+          go_reduce_rb<T, TU, LEAF, K, 8, false, VW, true>(s, w, ch, xin, out, carry, R);
+        else
+          go_reduce_rb<T, TU, LEAF, K, 4, false, VW, true>(s, w, ch, xin, out, carry, R);
+      } else if (r8) {
+        go_reduce_rb<T, TU, LEAF, K, 8, false, 1, true>(s, w, ch, xin, out, carry, R);
+      } else {
+        go_reduce_rb<T, TU, LEAF, K, 4, false, 1, true>(s, w, ch, xin, out, carry, R);
+      }
+      return;
+    }
+  }
   if (wide) {
     if constexpr (K <= 3) {
       if (rb == 4)

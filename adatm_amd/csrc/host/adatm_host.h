@@ -41,42 +41,11 @@ int adatm_hcsf_build(adatm_coo const *t, uint32_t const *order,
                      adatm_hcsf *out);
 void adatm_hcsf_free(adatm_hcsf *c);
 
-/* Work decomposition of a segmented reduction: children [0, nchild) in
- * segments seg p = [ptr[p], ptr[p+1]).  Light items cover whole segments
- * seg[k] .. seg[k+1]-1 with at most `tile` children; a segment longer than
- * `tile` is cut into pieces of `tile` children, each writing a carry slot
- * that a fix-up pass sums in piece order (deterministic). */
-typedef struct {
-  uint32_t nitems;
-  uint32_t nheavy;
-  uint32_t nslots;
-  uint32_t *start; /* nitems+1: first child of item */
-  uint32_t *seg;   /* nitems+1: first segment of item */
-  uint32_t *slot;  /* nitems: carry slot of a heavy piece, ADATM_NONE else */
-  uint32_t *hseg;  /* nheavy: the split segment */
-  uint32_t *hslot; /* nheavy+1: its slots [hslot[h], hslot[h+1]) */
-} adatm_segwork;
-
-int adatm_segwork_build(uint32_t const *ptr, uint64_t nseg, uint32_t tile,
-                        adatm_segwork *w);
-void adatm_segwork_free(adatm_segwork *w);
-
-/* Counting sort of node ids (values < nrows): perm = stable order by id,
- * rowptr (nrows+1) = segment of each id in perm. */
-int adatm_sort_by_id(uint32_t const *ids, uint64_t n, uint64_t nrows,
-                     uint32_t *perm, uint32_t *rowptr);
-
 /* ---- planner (planner.c) ----------------------------------------------- */
 int adatm_plan_enumerate(uint32_t nmodes, uint64_t const *dims,
                          adatm_plan *cands, int maxc);
 
 /* ---- engine internals shared with the SPLATT layer (engine.c) ---------- */
-int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
-                             uint64_t const *dims, uint64_t nnz,
-                             double ttnormsq, uint64_t rank,
-                             adatm_plan const *plan, int precision, int device,
-                             uint32_t const *use_grp, uint32_t const *use_tag,
-                             adatm_engine **out);
 /* device-built engine with a given mode -> (group, tag) map */
 int adatm_engine_create_use(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                             int precision, int device, uint32_t const *use_grp,

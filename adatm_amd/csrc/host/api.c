@@ -234,47 +234,6 @@ static splatt_csf *csf_array_from_hcsf(adatm_hcsf const *h, uint32_t n,
   return out;
 }
 
-/* u32 host CSF (with parent arrays) of a registered SPLATT CSF */
-static int hcsf_from_csf(splatt_csf const *c, adatm_hcsf *h)
-{
-  uint32_t const N = (uint32_t)c->nmodes;
-  memset(h, 0, sizeof(*h));
-  h->N = N;
-  h->nnz = c->nnz;
-  csf_sparsity const *pt = c->pt;
-  for (uint32_t l = 0; l < N; ++l) {
-    h->order[l] = (uint32_t)c->dim_perm[l];
-    h->F[l] = pt->nfibs[l];
-  }
-  for (uint32_t l = 0; l < N; ++l) {
-    uint64_t const F = h->F[l];
-    h->fids[l] = (uint32_t *)malloc((F ? F : 1) * sizeof(uint32_t));
-    if (!h->fids[l]) goto oom;
-    for (uint64_t i = 0; i < F; ++i)
-      h->fids[l][i] = pt->fids[l] ? (uint32_t)pt->fids[l][i] : (uint32_t)i;
-    if (l + 1 < N) {
-      h->fptr[l] = (uint32_t *)malloc((F + 1) * sizeof(uint32_t));
-      if (!h->fptr[l]) goto oom;
-      for (uint64_t i = 0; i <= F; ++i) h->fptr[l][i] = (uint32_t)pt->fptr[l][i];
-    }
-    if (l >= 1) {
-      h->par[l] = (uint32_t *)malloc((F ? F : 1) * sizeof(uint32_t));
-      if (!h->par[l]) goto oom;
-    }
-  }
-  for (uint32_t l = 0; l + 1 < N; ++l)
-    for (uint64_t p = 0; p < h->F[l]; ++p)
-      for (uint32_t c2 = h->fptr[l][p]; c2 < h->fptr[l][p + 1]; ++c2)
-        h->par[l + 1][c2] = (uint32_t)p;
-  h->vals = (double *)malloc(c->nnz * sizeof(double));
-  if (!h->vals) goto oom;
-  memcpy(h->vals, pt->vals, c->nnz * sizeof(double));
-  return SPLATT_SUCCESS;
-oom:
-  adatm_hcsf_free(h);
-  return SPLATT_ERROR_NOMEMORY;
-}
-
 static void free_csf_array(splatt_csf *t, uint32_t n)
 {
   if (!t) return;
@@ -608,26 +567,14 @@ static int ensure_engine(reg_ent *r, uint64_t rank, int prec)
   }
   memset(r->fac_on_dev, 0, sizeof(r->fac_on_dev));
   memset(r->memo_ok, 0, sizeof(r->memo_ok));
-  if (r->coo.nnz) { /* built on the device from the coordinates */
-    int const rc = adatm_engine_create_use(&r->coo, rank, &r->plan, prec, r->device,
-                                           r->use_grp, r->use_tag, &r->eng);
-    if (rc == SPLATT_SUCCESS) {
-      r->eng_rank = (uint32_t)rank;
-      r->eng_prec = prec;
-    }
-    return rc;
+  if (!r->coo.nnz) {
+    adatm_set_error("CSF set built with SPLATT_OPTION_ADATM_DEVICE < 0 (host arrays "
+                    "only): MTTKRP and CP-ALS need a GPU");
+    return SPLATT_ERROR_BADINPUT;
   }
-  adatm_hcsf *h = (adatm_hcsf *)calloc(r->n_csf, sizeof(adatm_hcsf));
-  if (!h) return SPLATT_ERROR_NOMEMORY;
-  int rc = SPLATT_SUCCESS;
-  for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < r->n_csf; ++g)
-    rc = hcsf_from_csf(r->csf + g, h + g);
-  if (rc == SPLATT_SUCCESS)
-    rc = adatm_engine_create_hcsf(h, r->N, r->dims, r->nnz, r->ttnormsq, rank,
-                                  &r->plan, prec, r->device, r->use_grp,
-                                  r->use_tag, &r->eng);
-  for (uint32_t g = 0; g < r->n_csf; ++g) adatm_hcsf_free(h + g);
-  free(h);
+  /* built on the device from the coordinates */
+  int const rc = adatm_engine_create_use(&r->coo, rank, &r->plan, prec, r->device,
+                                         r->use_grp, r->use_tag, &r->eng);
   if (rc == SPLATT_SUCCESS) {
     r->eng_rank = (uint32_t)rank;
     r->eng_prec = prec;

@@ -444,100 +444,6 @@ done:
   return rc;
 }
 
-void adatm_segwork_free(adatm_segwork *w)
-{
-  if (!w) return;
-  free(w->start); free(w->seg); free(w->slot); free(w->hseg); free(w->hslot);
-  memset(w, 0, sizeof(*w));
-}
-
-int adatm_segwork_build(uint32_t const *ptr, uint64_t nseg, uint32_t tile,
-                        adatm_segwork *w)
-{
-  memset(w, 0, sizeof(*w));
-  uint64_t const nchild = ptr[nseg];
-  /* upper bound on items: one per segment plus one per tile of children */
-  uint64_t const cap = nseg + nchild / (tile ? tile : 1) + 2;
-  w->start = (uint32_t *)malloc((cap + 1) * sizeof(uint32_t));
-  w->seg = (uint32_t *)malloc((cap + 1) * sizeof(uint32_t));
-  w->slot = (uint32_t *)malloc((cap + 1) * sizeof(uint32_t));
-  uint64_t hcap = nchild / (tile ? tile : 1) + 1;
-  w->hseg = (uint32_t *)malloc(hcap * sizeof(uint32_t));
-  w->hslot = (uint32_t *)malloc((hcap + 1) * sizeof(uint32_t));
-  if (!w->start || !w->seg || !w->slot || !w->hseg || !w->hslot) {
-    adatm_segwork_free(w);
-    return SPLATT_ERROR_NOMEMORY;
-  }
-  uint64_t ni = 0, nh = 0, ns = 0;
-  uint64_t cur_seg = 0, cur_cnt = 0;
-  int open = 0; /* current light item holds >= 1 segment */
-  for (uint64_t p = 0; p < nseg; ++p) {
-    uint64_t const cnt = (uint64_t)ptr[p + 1] - ptr[p];
-    if (cnt > tile) {
-      if (open) {
-        w->start[ni] = ptr[cur_seg];
-        w->seg[ni] = (uint32_t)cur_seg;
-        w->slot[ni] = ADATM_NONE;
-        ++ni;
-        open = 0;
-      }
-      uint64_t const pieces = (cnt + tile - 1) / tile;
-      w->hseg[nh] = (uint32_t)p;
-      w->hslot[nh] = (uint32_t)ns;
-      ++nh;
-      for (uint64_t q = 0; q < pieces; ++q) {
-        w->start[ni] = (uint32_t)(ptr[p] + q * tile);
-        w->seg[ni] = (uint32_t)p;
-        w->slot[ni] = (uint32_t)ns++;
-        ++ni;
-      }
-      cur_seg = p + 1;
-      cur_cnt = 0;
-    } else {
-      if (open && cur_cnt + cnt > tile) {
-        w->start[ni] = ptr[cur_seg];
-        w->seg[ni] = (uint32_t)cur_seg;
-        w->slot[ni] = ADATM_NONE;
-        ++ni;
-        cur_seg = p;
-        cur_cnt = 0;
-      }
-      if (!open) {
-        cur_seg = p;
-        open = 1;
-      }
-      cur_cnt += cnt;
-    }
-  }
-  if (open) {
-    w->start[ni] = ptr[cur_seg];
-    w->seg[ni] = (uint32_t)cur_seg;
-    w->slot[ni] = ADATM_NONE;
-    ++ni;
-  }
-  w->start[ni] = (uint32_t)nchild;
-  w->seg[ni] = (uint32_t)nseg;
-  w->slot[ni] = ADATM_NONE;
-  w->hslot[nh] = (uint32_t)ns;
-  w->nitems = (uint32_t)ni;
-  w->nheavy = (uint32_t)nh;
-  w->nslots = (uint32_t)ns;
-  return SPLATT_SUCCESS;
-}
-
-int adatm_sort_by_id(uint32_t const *ids, uint64_t n, uint64_t nrows,
-                     uint32_t *perm, uint32_t *rowptr)
-{
-  memset(rowptr, 0, (nrows + 1) * sizeof(uint32_t));
-  for (uint64_t j = 0; j < n; ++j) rowptr[ids[j] + 1]++;
-  for (uint64_t i = 0; i < nrows; ++i) rowptr[i + 1] += rowptr[i];
-  uint32_t *cur = (uint32_t *)malloc((nrows + 1) * sizeof(uint32_t));
-  if (!cur) return SPLATT_ERROR_NOMEMORY;
-  memcpy(cur, rowptr, (nrows + 1) * sizeof(uint32_t));
-  for (uint64_t j = 0; j < n; ++j) perm[cur[ids[j]]++] = (uint32_t)j;
-  free(cur);
-  return SPLATT_SUCCESS;
-}
 
 /* Fiber counts on the device (radix sort + boundary scan). */
 int adatm_count_fibers_device(adatm_coo const *t, uint32_t const *order,

@@ -42,10 +42,15 @@ typedef struct {
   uint32_t nslots;
 } dwork;
 
-/* one fused reduction pass of a group: level lb (children) -> lt (output) */
+/* one fused reduction pass of a group: level lb (children) -> lt (output);
+ * a leaf pass may also write memo levels between them (side levels) */
 typedef struct {
   uint32_t lb, lt;
   dwork w;
+  uint32_t side[ADATM_KMAXFUSE + 1]; /* relative level j is written beside the top */
+  dwork sw[ADATM_KMAXFUSE + 1];      /* merge of the partial rows of its cut nodes */
+  size_t soff[ADATM_KMAXFUSE + 1];   /* its first partial row in the engine's spart */
+  uint32_t *shead, *stail;           /* per item and relative level: partial slots */
 } dpass;
 
 /* reuse scatter of level d: id-sorted nodes with their ancestors' rows */
@@ -101,6 +106,9 @@ struct adatm_engine {
   void *tmp[2];
   void *carry, *carry2;
   size_t tmp_bytes, carry_bytes, carry2_bytes;
+  void *spart;       /* partial rows of side-level nodes cut by work items */
+  size_t spart_bytes;
+  uint32_t flags;    /* ADATM_ENGINE_STAGED, ADATM_ENGINE_STAGED_GROUP(g) */
   double ttnormsq;
   size_t dev_bytes;
   uint32_t fits_cap;
@@ -159,38 +167,6 @@ static int dupload(adatm_engine *e, void **p, void const *h, size_t bytes)
   int rc = dalloc(e, p, bytes);
   if (rc != SPLATT_SUCCESS) return rc;
   if (bytes) HIPCHK(hipMemcpy(*p, h, bytes, hipMemcpyHostToDevice), "upload");
-  return SPLATT_SUCCESS;
-}
-
-static int upload_work(adatm_engine *e, adatm_segwork const *w,
-                       uint32_t const *anc, size_t nanc, dwork *d)
-{
-  int rc;
-  memset(d, 0, sizeof(*d));
-  if (anc && nanc &&
-      (rc = dupload(e, (void **)&d->kw.anc, anc, nanc * sizeof(uint32_t))) !=
-          SPLATT_SUCCESS)
-    return rc;
-  d->kw.nitems = w->nitems;
-  d->kw.nheavy = w->nheavy;
-  d->nslots = w->nslots;
-  if ((rc = dupload(e, (void **)&d->kw.start, w->start,
-                    (w->nitems + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
-    return rc;
-  if ((rc = dupload(e, (void **)&d->kw.seg, w->seg,
-                    (w->nitems + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
-    return rc;
-  if ((rc = dupload(e, (void **)&d->kw.slot, w->slot,
-                    (w->nitems + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
-    return rc;
-  if (w->nheavy) {
-    if ((rc = dupload(e, (void **)&d->kw.hseg, w->hseg,
-                      w->nheavy * sizeof(uint32_t))) != SPLATT_SUCCESS)
-      return rc;
-    if ((rc = dupload(e, (void **)&d->kw.hslot, w->hslot,
-                      (w->nheavy + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
-      return rc;
-  }
   return SPLATT_SUCCESS;
 }
 
@@ -254,6 +230,11 @@ static double bytes_pass(adatm_engine const *e, dgroup const *g, uint32_t lb,
     b += (double)e->dims[g->order[l]] * R * w;
   b += (double)g->F[lt] * R * z; /* output rows */
   if (lt == 0 && g->orow) b += (double)g->F[0] * 4.0;
+  dpass const *P = NULL;
+  for (uint32_t i = 0; i < g->npass; ++i)
+    if (g->pass[i].lb == lb && g->pass[i].lt == lt) P = g->pass + i;
+  for (uint32_t j = 1; P && j < lb - lt; ++j)
+    if (P->side[j]) b += (double)g->F[lt + j] * R * z; /* memo rows written beside */
   return b;
 }
 
@@ -305,6 +286,14 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
     ch.ids[j] = g->ids[lt + j];
     ch.U[j] = gather_factor(e, g->order[lt + j]);
   }
+  for (uint32_t j = 1; j < ch.K; ++j)
+    if (P->side[j]) {
+      ch.side = 1;
+      ch.sout[j] = g->memo[lt + j];
+      ch.spart[j] = (char *)e->spart + P->soff[j] * e->R * e->vsz;
+    }
+  ch.shead = P->shead;
+  ch.stail = P->stail;
   int const leaf = (lb == e->N - 1);
   ev_begin(e, leaf ? KC_LEAF : KC_LEVEL, (int)ch.K, bytes_pass(e, g, lb, lt));
   KCHK(adatm_k_reduce(e->stream, e->prec, leaf, &P->w.kw, &ch, in, dst,
@@ -317,13 +306,28 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
          "merge");
     ev_end(e);
   }
+  /* side levels: the cut nodes' partial rows summed in item order */
+  for (uint32_t j = 1; j < ch.K; ++j)
+    if (P->side[j] && P->sw[j].kw.nchunk) {
+      ev_begin(e, KC_MERGE, 0, merge_bytes(e, &P->sw[j]));
+      KCHK(adatm_k_fixup(e->stream, e->prec, &P->sw[j].kw, ch.spart[j], e->carry2,
+                         g->memo[lt + j], e->R),
+           "side merge");
+      ev_end(e);
+    }
   return SPLATT_SUCCESS;
 }
 
-/* levels a fused pass may stop at walking up from lb: the next materialised
- * level, at most ADATM_KMAXFUSE levels up */
-static uint32_t pass_top(dgroup const *g, uint32_t lb, uint32_t floor_lvl)
+/* top of a root-chain pass from lb: staged, the next materialised level at
+ * most ADATM_KMAXFUSE levels up; otherwise a leaf pass goes on through the
+ * memo levels (written beside its top) up to ADATM_KMAXFUSE levels */
+static uint32_t pass_top(adatm_engine const *e, dgroup const *g, uint32_t lb,
+                         uint32_t floor_lvl)
 {
+  uint32_t const gi = (uint32_t)(g - e->g);
+  if (!(e->flags & (ADATM_ENGINE_STAGED | ADATM_ENGINE_STAGED_GROUP(gi))) &&
+      lb == e->N - 1)
+    return lb - floor_lvl > ADATM_KMAXFUSE ? lb - ADATM_KMAXFUSE : floor_lvl;
   uint32_t lt = lb - 1;
   while (lt > floor_lvl && !is_memo_level(g, lt) && lb - lt < ADATM_KMAXFUSE)
     --lt;
@@ -339,7 +343,7 @@ static int mttkrp_root(adatm_engine *e, dgroup *g, uint32_t mode)
   int pp = 0;
   uint32_t lb = e->N - 1;
   while (lb > 0) {
-    uint32_t const lt = pass_top(g, lb, 0);
+    uint32_t const lt = pass_top(e, g, lb, 0);
     void *dst;
     if (lt == 0)
       dst = e->M;
@@ -454,208 +458,6 @@ static int engine_alloc_common(adatm_engine *e)
   return SPLATT_SUCCESS;
 }
 
-/* work list of the fused pass lb -> lt of a host CSF */
-static int build_pass(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
-                      uint32_t lb, uint32_t lt)
-{
-  if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
-  if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
-  if (!g->cfd[lb]) {
-    /* first level at which node c and c+1 of level lb differ */
-    uint64_t const F = h->F[lb];
-    uint8_t *f = (uint8_t *)malloc(F ? F : 1);
-    if (!f) return SPLATT_ERROR_NOMEMORY;
-    #pragma omp parallel for schedule(static)
-    for (int64_t c = 0; c < (int64_t)F; ++c) {
-      uint32_t l = 0;
-      if ((uint64_t)c + 1 < F) {
-        uint32_t x = (uint32_t)c, y = (uint32_t)c + 1;
-        l = lb;
-        while (l > 0 && h->par[l][x] != h->par[l][y]) {
-          x = h->par[l][x];
-          y = h->par[l][y];
-          --l;
-        }
-      }
-      f[c] = (uint8_t)l;
-    }
-    int const rc = dupload(e, (void **)&g->cfd[lb], f, F);
-    free(f);
-    if (rc != SPLATT_SUCCESS) return rc;
-  }
-  uint64_t const np = h->F[lt];
-  uint32_t *cp = (uint32_t *)malloc((np + 1) * sizeof(uint32_t));
-  if (!cp) return SPLATT_ERROR_NOMEMORY;
-  /* first bottom descendant of each top node (fptr composition) */
-  #pragma omp parallel for schedule(static)
-  for (int64_t p = 0; p <= (int64_t)np; ++p) {
-    uint64_t v = (uint64_t)p;
-    for (uint32_t l = lt; l < lb; ++l) v = h->fptr[l][v];
-    cp[p] = (uint32_t)v;
-  }
-  adatm_segwork w;
-  int rc = adatm_segwork_build(cp, np, e->red_tile, &w);
-  free(cp);
-  if (rc != SPLATT_SUCCESS) return rc;
-  uint32_t const K = lb - lt;
-  if (lt == 0 && g->orow)
-    for (uint32_t hh = 0; hh < w.nheavy; ++hh) w.hseg[hh] = h->fids[0][w.hseg[hh]];
-  uint32_t *anc = NULL;
-  if (K > 1) {
-    anc = (uint32_t *)malloc((size_t)w.nitems * (K - 1) * sizeof(uint32_t));
-    if (!anc) {
-      adatm_segwork_free(&w);
-      return SPLATT_ERROR_NOMEMORY;
-    }
-    #pragma omp parallel for schedule(static)
-    for (int64_t k = 0; k < (int64_t)w.nitems; ++k) {
-      uint32_t v = w.start[k];
-      for (uint32_t l = lb; l > lt + 1; --l) {
-        v = h->par[l][v];
-        anc[(size_t)k * (K - 1) + (l - 1 - lt) - 1] = v;
-      }
-    }
-  }
-  dpass *P = g->pass + g->npass;
-  P->lb = lb;
-  P->lt = lt;
-  rc = upload_work(e, &w, anc, anc ? (size_t)w.nitems * (K - 1) : 0, &P->w);
-  if (rc == SPLATT_SUCCESS) rc = dwork_fixlists(e, &P->w);
-  size_t const cb = (size_t)w.nslots * e->R * e->vsz;
-  if (cb > e->carry_bytes) e->carry_bytes = cb;
-  free(anc);
-  adatm_segwork_free(&w);
-  if (rc == SPLATT_SUCCESS) g->npass++;
-  return rc;
-}
-
-/* id-sorted level-d nodes with the factor-row ids of their ancestors */
-static int build_scatter(adatm_engine *e, dgroup *g, adatm_hcsf const *h,
-                         uint32_t d)
-{
-  dscat *S = g->scat + d;
-  if (S->node) return SPLATT_SUCCESS;
-  if (d < 1 || d > ADATM_KMAXSCAT) {
-    adatm_set_error("engine: reuse level %u beyond %d", d, ADATM_KMAXSCAT);
-    return SPLATT_ERROR_BADINPUT;
-  }
-  uint64_t const nrows = e->dims[g->order[d]];
-  uint64_t const Fd = h->F[d];
-  uint32_t *perm = (uint32_t *)malloc(Fd * sizeof(uint32_t));
-  uint32_t *rowptr = (uint32_t *)malloc((nrows + 1) * sizeof(uint32_t));
-  uint32_t *rid = (uint32_t *)malloc((size_t)d * Fd * sizeof(uint32_t));
-  int rc = SPLATT_ERROR_NOMEMORY;
-  adatm_segwork w;
-  memset(&w, 0, sizeof(w));
-  if (!perm || !rowptr || !rid) goto out;
-  rc = adatm_sort_by_id(h->fids[d], Fd, nrows, perm, rowptr);
-  if (rc != SPLATT_SUCCESS) goto out;
-  #pragma omp parallel for schedule(static)
-  for (int64_t q = 0; q < (int64_t)Fd; ++q) {
-    uint32_t v = perm[q];
-    for (uint32_t l = d; l >= 1; --l) {
-      v = h->par[l][v]; /* ancestor at level l-1 */
-      rid[(size_t)(l - 1) * Fd + q] = h->fids[l - 1][v];
-    }
-  }
-  rc = adatm_segwork_build(rowptr, nrows, e->scat_tile, &w);
-  if (rc != SPLATT_SUCCESS) goto out;
-  S->d = d;
-  rc = dupload(e, (void **)&S->node, perm, Fd * sizeof(uint32_t));
-  for (uint32_t l = 0; rc == SPLATT_SUCCESS && l < d; ++l)
-    rc = dupload(e, (void **)&S->rid[l], rid + (size_t)l * Fd,
-                 Fd * sizeof(uint32_t));
-  if (rc == SPLATT_SUCCESS)
-    rc = dupload(e, (void **)&S->rowptr, rowptr, (nrows + 1) * sizeof(uint32_t));
-  if (rc == SPLATT_SUCCESS) rc = upload_work(e, &w, NULL, 0, &S->w);
-  if (rc == SPLATT_SUCCESS) rc = dwork_fixlists(e, &S->w);
-  {
-    size_t const cb = (size_t)w.nslots * e->R * e->vsz;
-    if (cb > e->carry_bytes) e->carry_bytes = cb;
-  }
-out:
-  adatm_segwork_free(&w);
-  free(perm);
-  free(rowptr);
-  free(rid);
-  return rc;
-}
-
-static int engine_add_group(adatm_engine *e, uint32_t gi, adatm_hcsf const *h)
-{
-  uint32_t const N = e->N, R = e->R;
-  dgroup *g = e->g + gi;
-  adatm_group const *pg = e->plan.grp + gi;
-  int rc;
-  g->memo_mode = pg->memo_mode;
-  g->b = pg->begin_imten;
-  g->nim = pg->n_imten;
-  memcpy(g->order, h->order, N * sizeof(uint32_t));
-  memcpy(g->F, h->F, N * sizeof(uint64_t));
-  /* values in the path precision */
-  if (e->vsz == 8) {
-    if ((rc = dupload(e, &g->vals, h->vals, h->nnz * 8)) != SPLATT_SUCCESS)
-      return rc;
-  } else {
-    float *tmp = (float *)malloc(h->nnz * sizeof(float));
-    if (!tmp) return SPLATT_ERROR_NOMEMORY;
-    #pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < (int64_t)h->nnz; ++j) tmp[j] = (float)h->vals[j];
-    rc = dupload(e, &g->vals, tmp, h->nnz * sizeof(float));
-    free(tmp);
-    if (rc != SPLATT_SUCCESS) return rc;
-  }
-  for (uint32_t l = 1; l < N; ++l)
-    if ((rc = dupload(e, (void **)&g->ids[l], h->fids[l],
-                      h->F[l] * sizeof(uint32_t))) != SPLATT_SUCCESS)
-      return rc;
-  for (uint32_t l = 0; l + 1 < N; ++l)
-    if ((rc = dupload(e, (void **)&g->ptr[l], h->fptr[l],
-                      (h->F[l] + 1) * sizeof(uint32_t))) != SPLATT_SUCCESS)
-      return rc;
-  /* root ids: identity after empty-index compaction, else a row map */
-  {
-    int ident = (h->F[0] == e->dims[g->order[0]]);
-    for (uint64_t n = 0; ident && n < h->F[0]; ++n) ident = (h->fids[0][n] == n);
-    g->zero_root = (h->F[0] < e->dims[g->order[0]]);
-    if (!ident &&
-        (rc = dupload(e, (void **)&g->orow, h->fids[0],
-                      h->F[0] * sizeof(uint32_t))) != SPLATT_SUCCESS)
-      return rc;
-  }
-  /* memo levels */
-  for (uint32_t l = 1; l + 1 < N; ++l)
-    if (is_memo_level(g, l) &&
-        (rc = dalloc(e, &g->memo[l], g->F[l] * R * e->vsz)) != SPLATT_SUCCESS)
-      return rc;
-  /* the root chain's passes */
-  for (uint32_t lb = N - 1; lb > 0;) {
-    uint32_t const lt = pass_top(g, lb, 0);
-    if ((rc = build_pass(e, g, h, lb, lt)) != SPLATT_SUCCESS) return rc;
-    if (lt > 0 && !is_memo_level(g, lt)) {
-      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
-      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
-    }
-    lb = lt;
-  }
-  /* non-root modes: propagation passes (memo or leaves) and scatters */
-  for (uint32_t m = 0; m < N; ++m) {
-    if (e->use_grp[m] != gi || e->use_tag[m] == 0) continue;
-    uint32_t const d = e->level_of[gi][m];
-    uint32_t lb = (e->use_tag[m] == 1) ? reuse_source(g, d) : N - 1;
-    while (lb > d) {
-      uint32_t const lt = lb > d + ADATM_KMAXFUSE
-                              ? lb - ADATM_KMAXFUSE : d;
-      if ((rc = build_pass(e, g, h, lb, lt)) != SPLATT_SUCCESS) return rc;
-      size_t const tb = (size_t)g->F[lt] * R * e->vsz;
-      if (tb > e->tmp_bytes) e->tmp_bytes = tb;
-      lb = lt;
-    }
-    if ((rc = build_scatter(e, g, h, d)) != SPLATT_SUCCESS) return rc;
-  }
-  return SPLATT_SUCCESS;
-}
-
 static int engine_finish(adatm_engine *e)
 {
   int rc;
@@ -666,6 +468,8 @@ static int engine_finish(adatm_engine *e)
     return rc;
   if ((rc = dalloc(e, &e->carry2, e->carry2_bytes ? e->carry2_bytes : 64)) !=
       SPLATT_SUCCESS)
+    return rc;
+  if ((rc = dalloc(e, &e->spart, e->spart_bytes ? e->spart_bytes : 64)) != SPLATT_SUCCESS)
     return rc;
   return SPLATT_SUCCESS;
 }
@@ -749,32 +553,6 @@ static int engine_begin(adatm_engine *e, uint32_t N, uint64_t const *dims,
   for (int i = 0; i < 2048; ++i)
     HIPCHK(hipEventCreate(&e->ev[i]), "hipEventCreate");
   return engine_alloc_common(e);
-}
-
-int adatm_engine_create_hcsf(adatm_hcsf const *groups, uint32_t N,
-                             uint64_t const *dims, uint64_t nnz,
-                             double ttnormsq, uint64_t rank,
-                             adatm_plan const *plan, int precision, int device,
-                             uint32_t const *use_grp, uint32_t const *use_tag,
-                             adatm_engine **out)
-{
-  adatm_engine *e = (adatm_engine *)calloc(1, sizeof(adatm_engine));
-  if (!e) return SPLATT_ERROR_NOMEMORY;
-  int rc = engine_begin(e, N, dims, nnz, rank, plan, precision, device,
-                        use_grp, use_tag);
-  for (uint32_t gi = 0; rc == SPLATT_SUCCESS && gi < plan->n_grp; ++gi)
-    rc = engine_add_group(e, gi, groups + gi);
-  if (rc == SPLATT_SUCCESS) rc = engine_finish(e);
-  e->ttnormsq = ttnormsq;
-  if (rc != SPLATT_SUCCESS) {
-    char msg[512];
-    snprintf(msg, sizeof(msg), "%s", adatm_last_error());
-    adatm_engine_destroy(e);
-    adatm_set_error("%s", msg);
-    return rc;
-  }
-  *out = e;
-  return SPLATT_SUCCESS;
 }
 
 /* ======================================================================== *
@@ -1133,8 +911,79 @@ typedef struct {
   uint32_t *stemp, *tot;
 } dbuild;
 
+/* Side levels of a leaf pass: for every memo level L it crosses, the nodes
+ * its work items' ends cut, their partial-row slots (in item order, so each
+ * cut node's slots are consecutive) and the merge list of the cut nodes
+ * (node = memo row, slot range), chunked like the split-parent merge. */
+static int build_side(adatm_engine *e, dgroup *g, dbuild *B, dpass *P)
+{
+  hipStream_t const s = e->stream;
+  uint32_t const K = P->lb - P->lt, n = P->w.kw.nitems;
+  uint32_t *cnt = NULL, *base = NULL, *nf = NULL, *nl = NULL, *fl = NULL;
+  uint32_t *snode = NULL, *flags = NULL, *pos = NULL;
+  size_t total = 0;
+  int rc = SPLATT_ERROR_NOMEMORY;
+  if ((rc = dalloc(e, (void **)&P->shead, (size_t)n * K * 4)) != SPLATT_SUCCESS ||
+      (rc = dalloc(e, (void **)&P->stail, (size_t)n * K * 4)) != SPLATT_SUCCESS)
+    return rc;
+  rc = SPLATT_ERROR_NOMEMORY;
+  /* cut nodes lie in split top parents only: at most two per piece */
+  if (hipMalloc((void **)&cnt, (size_t)n * 4) || hipMalloc((void **)&base, (size_t)n * 4) ||
+      hipMalloc((void **)&nf, (size_t)n * 4) || hipMalloc((void **)&nl, (size_t)n * 4) ||
+      hipMalloc((void **)&fl, (size_t)n * 4) ||
+      hipMalloc((void **)&snode, (size_t)2 * n * 4) ||
+      hipMalloc((void **)&flags, (size_t)2 * n * 4) ||
+      hipMalloc((void **)&pos, (size_t)2 * n * 4))
+    goto out;
+  for (uint32_t j = 1; j < K; ++j) {
+    if (!P->side[j]) continue;
+    uint32_t const L = P->lt + j;
+    uint32_t nsl = 0, nrun = 0;
+    rc = SPLATT_ERROR_DEVICE;
+    if (adatm_k_side_count(s, P->w.kw.start, n, P->lb, L, g->F[P->lb], g->cfd[P->lb],
+                           (uint32_t const *const *)B->par, cnt, nf, nl, fl))
+      goto out;
+    if (dscan(s, cnt, base, n, B->stemp, B->tot, &nsl) != SPLATT_SUCCESS) goto out;
+    if (adatm_k_side_emit(s, n, K, j, cnt, base, nf, nl, fl, (uint32_t *)P->shead,
+                          (uint32_t *)P->stail, snode))
+      goto out;
+    P->soff[j] = total;
+    total += nsl;
+    dwork *W = P->sw + j;
+    memset(W, 0, sizeof(*W));
+    if (nsl) {
+      if (adatm_k_run_flags(s, snode, nsl, flags)) goto out;
+      if (dscan(s, flags, pos, nsl, B->stemp, B->tot, &nrun) != SPLATT_SUCCESS) goto out;
+      if ((rc = dalloc(e, (void **)&W->kw.hseg, (size_t)nrun * 4)) != SPLATT_SUCCESS ||
+          (rc = dalloc(e, (void **)&W->kw.hslot, ((size_t)nrun + 1) * 4)) != SPLATT_SUCCESS)
+        goto out;
+      rc = SPLATT_ERROR_DEVICE;
+      if (adatm_k_run_emit(s, flags, pos, snode, nsl, nrun, (uint32_t *)W->kw.hseg,
+                           (uint32_t *)W->kw.hslot))
+        goto out;
+      W->kw.nheavy = nrun;
+      W->nslots = nsl;
+      if ((rc = dwork_fixlists(e, W)) != SPLATT_SUCCESS) goto out;
+    }
+  }
+  {
+    size_t const sb = total * e->R * e->vsz;
+    if (sb > e->spart_bytes) e->spart_bytes = sb;
+  }
+  rc = SPLATT_SUCCESS;
+out:
+  hipStreamSynchronize(s);
+  hipFree(cnt); hipFree(base); hipFree(nf); hipFree(nl); hipFree(fl);
+  hipFree(snode); hipFree(flags); hipFree(pos);
+  if (rc != SPLATT_SUCCESS && !adatm_last_error()[0])
+    adatm_set_error("side-level cut lists failed");
+  return rc;
+}
+
+/* work list of the pass lb -> lt; a root-chain leaf pass also writes the
+ * memo levels it crosses (side levels) */
 static int build_pass_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t lb,
-                          uint32_t lt)
+                          uint32_t lt, int root)
 {
   if (find_pass(g, lb, lt)) return SPLATT_SUCCESS;
   if (g->npass >= MAXPASS) return SPLATT_ERROR_BADINPUT;
@@ -1178,8 +1027,16 @@ static int build_pass_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t lb,
                            (uint32_t const *const *)B->par, (uint32_t *)P->w.kw.anc),
          "item ancestors");
   }
-  g->npass++;
-  return SPLATT_SUCCESS;
+  g->npass++; /* from here on the destroy path frees what the pass holds */
+  if (root && lb == e->N - 1 && K > 1 && P->w.kw.nitems) {
+    int any = 0;
+    for (uint32_t j = 1; j < K; ++j) {
+      P->side[j] = (uint32_t)is_memo_level(g, lt + j);
+      any |= (int)P->side[j];
+    }
+    if (any) rc = build_side(e, g, B, P);
+  }
+  return rc;
 }
 
 static int build_scatter_dev(adatm_engine *e, dgroup *g, dbuild *B, uint32_t d)
@@ -1336,8 +1193,8 @@ static int engine_add_group_dev(adatm_engine *e, uint32_t gi, dcoo const *c)
       goto out;
   /* root chain */
   for (uint32_t lb = N - 1; lb > 0;) {
-    uint32_t const lt = pass_top(g, lb, 0);
-    if ((rc = build_pass_dev(e, g, &B, lb, lt)) != SPLATT_SUCCESS) goto out;
+    uint32_t const lt = pass_top(e, g, lb, 0);
+    if ((rc = build_pass_dev(e, g, &B, lb, lt, 1)) != SPLATT_SUCCESS) goto out;
     if (lt > 0 && !is_memo_level(g, lt)) {
       size_t const tb = (size_t)g->F[lt] * R * e->vsz;
       if (tb > e->tmp_bytes) e->tmp_bytes = tb;
@@ -1351,7 +1208,7 @@ static int engine_add_group_dev(adatm_engine *e, uint32_t gi, dcoo const *c)
     uint32_t lb = (e->use_tag[m] == 1) ? reuse_source(g, d) : N - 1;
     while (lb > d) {
       uint32_t const lt = lb > d + ADATM_KMAXFUSE ? lb - ADATM_KMAXFUSE : d;
-      if ((rc = build_pass_dev(e, g, &B, lb, lt)) != SPLATT_SUCCESS) goto out;
+      if ((rc = build_pass_dev(e, g, &B, lb, lt, 0)) != SPLATT_SUCCESS) goto out;
       size_t const tb = (size_t)g->F[lt] * R * e->vsz;
       if (tb > e->tmp_bytes) e->tmp_bytes = tb;
       lb = lt;
@@ -1388,7 +1245,7 @@ int adatm_engine_create(adatm_coo const *t, uint64_t rank,
 
 static int engine_create_dev(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                              int precision, int device, uint32_t red_tile,
-                             uint32_t scat_tile, uint32_t const *use_grp,
+                             uint32_t scat_tile, uint32_t flags, uint32_t const *use_grp,
                              uint32_t const *use_tag, adatm_engine **out);
 
 int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
@@ -1396,8 +1253,23 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
                               uint32_t red_tile, uint32_t scat_tile,
                               adatm_engine **out)
 {
-  return engine_create_dev(t, rank, plan, precision, device, red_tile, scat_tile, NULL,
+  return engine_create_dev(t, rank, plan, precision, device, red_tile, scat_tile, 0, NULL,
                            NULL, out);
+}
+
+int adatm_engine_create_ex(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                           int precision, int device, uint32_t red_tile,
+                           uint32_t scat_tile, uint32_t flags, adatm_engine **out)
+{
+  uint32_t const known = ADATM_ENGINE_STAGED |
+                         (ADATM_ENGINE_STAGED_GROUP(ADATM_MAX_NMODES) -
+                          ADATM_ENGINE_STAGED_GROUP(0));
+  if (flags & ~known) {
+    adatm_set_error("engine: unknown creation flags 0x%x", flags);
+    return SPLATT_ERROR_BADINPUT;
+  }
+  return engine_create_dev(t, rank, plan, precision, device, red_tile, scat_tile, flags,
+                           NULL, NULL, out);
 }
 
 /* built on the device with the caller's mode -> (group, tag) map (the SPLATT
@@ -1406,12 +1278,13 @@ int adatm_engine_create_use(adatm_coo const *t, uint64_t rank, adatm_plan const 
                             int precision, int device, uint32_t const *use_grp,
                             uint32_t const *use_tag, adatm_engine **out)
 {
-  return engine_create_dev(t, rank, plan, precision, device, 0, 0, use_grp, use_tag, out);
+  return engine_create_dev(t, rank, plan, precision, device, 0, 0, 0, use_grp, use_tag,
+                           out);
 }
 
 static int engine_create_dev(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                              int precision, int device, uint32_t red_tile,
-                             uint32_t scat_tile, uint32_t const *use_grp,
+                             uint32_t scat_tile, uint32_t flags, uint32_t const *use_grp,
                              uint32_t const *use_tag, adatm_engine **out)
 {
   *out = NULL;
@@ -1428,6 +1301,7 @@ static int engine_create_dev(adatm_coo const *t, uint64_t rank, adatm_plan const
   if (!e) return SPLATT_ERROR_NOMEMORY;
   e->red_tile = red_tile;
   e->scat_tile = scat_tile;
+  e->flags = flags;
   int rc = engine_begin(e, N, t->dims, t->nnz, rank, plan, precision, device,
                         use_grp, use_tag);
   dcoo c;
@@ -1471,7 +1345,13 @@ void adatm_engine_destroy(adatm_engine *e)
       for (uint32_t k = 0; k < ADATM_KMAXSCAT; ++k) hipFree(S->rid[k]);
       free_work(&S->w);
     }
-    for (uint32_t i = 0; i < g->npass; ++i) free_work(&g->pass[i].w);
+    for (uint32_t i = 0; i < g->npass; ++i) {
+      dpass *P = g->pass + i;
+      free_work(&P->w);
+      for (uint32_t j = 0; j <= ADATM_KMAXFUSE; ++j) free_work(&P->sw[j]);
+      hipFree(P->shead);
+      hipFree(P->stail);
+    }
     hipFree(g->orow);
   }
   for (uint32_t m = 0; m < MAXN; ++m) {
@@ -1491,6 +1371,7 @@ void adatm_engine_destroy(adatm_engine *e)
   hipFree(e->tmp[1]);
   hipFree(e->carry);
   hipFree(e->carry2);
+  hipFree(e->spart);
   if (e->ev[0])
     for (int i = 0; i < 2048; ++i)
       if (e->ev[i]) hipEventDestroy(e->ev[i]);
@@ -1884,7 +1765,7 @@ int adatm_engine_stats(adatm_engine *e, uint64_t *nfibs, double *seq_bytes,
     }
     while (lb > stop) {
       uint32_t const lt = (e->use_tag[m] == 0)
-                              ? pass_top(g, lb, 0)
+                              ? pass_top(e, g, lb, 0)
                               : (lb > stop + ADATM_KMAXFUSE
                                      ? lb - ADATM_KMAXFUSE : stop);
       B += bytes_pass(e, g, lb, lt);
@@ -2062,21 +1943,24 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
   return SPLATT_SUCCESS;
 }
 
-/* Work-item tiles and reduction batch of one plan chosen by timing: the
- * reduction tile over {32, 64, 128, 256, 512} with the scatter tile at its
- * default, then the scatter tile over {16, 32, 64, 128, 256} with the best
- * reduction tile, then (when the scatter tile moved) the reduction tile
- * again with the new scatter tile; then, with the best tiles, the default
- * batch against the children per batch over {4, 8, 16}, the pipelined 4 and
- * 8 and the wide 4 and 8.  The defaults are timed first and a candidate
- * displaces the best so far only when it is more than TUNE_MARGIN faster:
- * near-equal choices differ by run-to-run noise, and without the margin the
- * choice (and with it the kernel that dominates the sweep) flips between
- * runs.  The order of the calls to `timer` is part of the contract (the
- * unit tests drive it with a table of timings). */
+/* Work-item tiles, root-pass form and reduction batch of one plan chosen by
+ * timing: the reduction tile over {32, 64, 128, 256, 512} with the scatter
+ * tile at its default, then the scatter tile over {16, 32, 64, 128, 256}
+ * with the best reduction tile, then (when the scatter tile moved) the
+ * reduction tile again with the new scatter tile; then, with the best
+ * tiles, each memo group's root pass staged (against the default, one pass
+ * to the root), keeping what is faster, group by group; then the
+ * default batch against the children per batch over {4, 8, 16}, the
+ * pipelined 4 and 8 and the wide 4 and 8.  The defaults are timed first and
+ * a candidate displaces the best so far only when it is more than
+ * TUNE_MARGIN faster: near-equal choices differ by run-to-run noise, and
+ * without the margin the choice (and with it the kernel that dominates the
+ * sweep) flips between runs.  The order of the calls to `timer` is part of
+ * the contract (the unit tests drive it with a table of timings). */
 #define TUNE_MARGIN 0.99
-int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
-                          uint32_t *scat_tile, uint32_t *batch, double *best_ms)
+int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t groups,
+                          uint32_t *red_tile, uint32_t *scat_tile, uint32_t *batch,
+                          uint32_t *flags, double *best_ms)
 {
   if (!timer || !red_tile || !scat_tile) return SPLATT_ERROR_BADINPUT;
   /* defaults first */
@@ -2096,7 +1980,7 @@ int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
       if (pass == 1 && sc == SCAT_TILE) continue; /* timed in the first pass */
       if (pass == 2 && r == r1) continue;         /* timed in the second pass */
       double ms = INFINITY;
-      int const rc = timer(ctx, r, sc, 0, &ms);
+      int const rc = timer(ctx, r, sc, 0, 0, &ms);
       if (rc != SPLATT_SUCCESS) return rc;
       if (ms < bt * TUNE_MARGIN) {
         bt = ms;
@@ -2105,13 +1989,26 @@ int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
       }
     }
   }
+  /* root passes: each memo group staged in turn (keeping what helped) */
+  uint32_t bf = 0;
+  for (uint32_t g = 0; g < ADATM_MAX_NMODES; ++g) {
+    if (!((groups >> g) & 1u)) continue;
+    uint32_t const f = bf | ADATM_ENGINE_STAGED_GROUP(g);
+    double ms = INFINITY;
+    int const rc = timer(ctx, br, bs, 0, f, &ms);
+    if (rc != SPLATT_SUCCESS) return rc;
+    if (ms < bt * TUNE_MARGIN) {
+      bt = ms;
+      bf = f;
+    }
+  }
   static uint32_t const BT[7] = {4, 8, 16, 4 | ADATM_BATCH_PIPELINED,
                                  8 | ADATM_BATCH_PIPELINED, 4 | ADATM_BATCH_WIDE,
                                  8 | ADATM_BATCH_WIDE};
   uint32_t bb = 0;
   for (int i = 0; i < 7; ++i) {
     double ms = INFINITY;
-    int const rc = timer(ctx, br, bs, BT[i], &ms);
+    int const rc = timer(ctx, br, bs, BT[i], bf, &ms);
     if (rc != SPLATT_SUCCESS) return rc;
     if (ms < bt * TUNE_MARGIN) {
       bt = ms;
@@ -2121,33 +2018,36 @@ int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
   *red_tile = br;
   *scat_tile = bs;
   if (batch) *batch = bb;
+  if (flags) *flags = bf;
   if (best_ms) *best_ms = bt;
   return SPLATT_SUCCESS;
 }
 
 /* the device timer of adatm_autotune_tiles: CP-ALS sweeps of an engine
- * built with the tiles (kept while only the batch changes) */
+ * built with the tiles and flags (kept while only the batch changes) */
 typedef struct {
   adatm_coo const *t;
   uint64_t rank;
   adatm_plan const *plan;
   int precision, device, nrep;
   adatm_engine *e;
-  uint32_t red, scat;
+  uint32_t red, scat, flags;
 } tune_ctx;
 
-static int tune_time(void *vctx, uint32_t red, uint32_t scat, uint32_t batch, double *ms)
+static int tune_time(void *vctx, uint32_t red, uint32_t scat, uint32_t batch,
+                     uint32_t flags, double *ms)
 {
   tune_ctx *c = (tune_ctx *)vctx;
-  if (!c->e || c->red != red || c->scat != scat) {
+  if (!c->e || c->red != red || c->scat != scat || c->flags != flags) {
     adatm_engine_destroy(c->e);
     c->e = NULL;
-    int rc = adatm_engine_create_tiled(c->t, c->rank, c->plan, c->precision, c->device,
-                                       red, scat, &c->e);
+    int rc = adatm_engine_create_ex(c->t, c->rank, c->plan, c->precision, c->device, red,
+                                    scat, flags, &c->e);
     if (rc == SPLATT_SUCCESS) rc = adatm_engine_init_factors(c->e, 1);
     if (rc != SPLATT_SUCCESS) return rc;
     c->red = red;
     c->scat = scat;
+    c->flags = flags;
   }
   int const rc = adatm_engine_set_batch(c->e, batch);
   if (rc != SPLATT_SUCCESS) return rc;
@@ -2156,7 +2056,8 @@ static int tune_time(void *vctx, uint32_t red, uint32_t scat, uint32_t batch, do
 
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
-                         uint32_t *scat_tile, uint32_t *batch, double *best_ms)
+                         uint32_t *scat_tile, uint32_t *batch, uint32_t *flags,
+                         double *best_ms)
 {
   tune_ctx c;
   memset(&c, 0, sizeof(c));
@@ -2166,7 +2067,12 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
   c.precision = precision;
   c.device = device;
   c.nrep = nrep;
-  int const rc = adatm_autotune_search(tune_time, &c, red_tile, scat_tile, batch, best_ms);
+  /* the memo groups' root passes may be fused or staged; others have one form */
+  uint32_t groups = 0;
+  for (uint32_t g = 0; g < plan->n_grp; ++g)
+    if (plan->grp[g].n_imten > 0) groups |= 1u << g;
+  int const rc = adatm_autotune_search(tune_time, &c, groups, red_tile, scat_tile, batch,
+                                       flags, best_ms);
   adatm_engine_destroy(c.e);
   return rc;
 }

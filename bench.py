@@ -81,6 +81,24 @@ def finite(x):
     return x
 
 
+def root_pass_name(flags: int, ngrp: int) -> str:
+    """the root-pass form of every memo group"""
+    import adatm_amd as A
+    if flags & A.ENGINE_STAGED:
+        return "staged (a pass per memo level)"
+    return ", ".join(f"g{g}: " + ("staged" if flags & A.ENGINE_STAGED_GROUP(g) else "fused")
+                     for g in range(ngrp))
+
+
+def root_flags(choice: str, tiles) -> int:
+    """engine creation flags of the root passes: the explicit choice, else
+    the autotuner's (the fourth entry of its tuple)"""
+    import adatm_amd as A
+    if choice:
+        return {"fused": 0, "staged": A.ENGINE_STAGED}[choice]
+    return int(tiles[3]) if len(tiles) > 3 else 0
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -99,14 +117,18 @@ def main() -> int:
                     help="draws of the single-thread CPU oracle run (0 = skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--root-pass", default="", choices=["", "fused", "staged"],
+                    help="root MTTKRP of every memo group: one leaf-to-root pass writing the "
+                         "memo levels beside (fused) or passes stopping at every memo level "
+                         "(staged); default: the autotuner's choice per group")
     ap.add_argument("--no-convert", action="store_true",
                     help="skip timing splatt_csf_convert on the workload")
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default work-item tiles instead of the autotuner's")
-    ap.add_argument("--tiles", default="", metavar="RED,SCAT[,BATCH]",
-                    help="use these work-item tiles (and reduction batch) for every engine "
-                         "(no autotuning)")
+    ap.add_argument("--tiles", default="", metavar="RED,SCAT[,BATCH[,FLAGS]]",
+                    help="use these work-item tiles (reduction batch, engine creation flags) "
+                         "for every engine (no autotuning)")
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
     ap.add_argument("--hw-queues", type=int, default=0,
@@ -177,7 +199,9 @@ def main() -> int:
         elif not args.no_autotune:
             tiles, tms = A.autotune_tiles(t, R, pl, prec, 0, 3)
             log(f"{name}: autotuned tiles {tiles} ({tms:.3f} ms/sweep, {time.time() - t2:.1f}s)")
-        e = A.Engine(t, R, pl, precision=prec, device=0, tiles=tiles)
+        e = A.Engine(t, R, pl, precision=prec, device=0, tiles=tiles[:3],
+                     flags=root_flags(args.root_pass, tiles))
+        tiles = (tuple(tiles[:3]) + (0, 0, 0))[:3] + (root_flags(args.root_pass, tiles),)
         e.init_factors(spec.seed + 1)
         log(f"{name} engine built ({time.time() - t2:.1f}s)")
         if args.warmup > 0:
@@ -189,6 +213,7 @@ def main() -> int:
         launch_log = e.launches()
         F, seq_bytes, seq_flops, dev_bytes = e.stats()
         res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=float(fits[-1]), tiles=list(tiles),
+                         root_pass=root_pass_name(tiles[3], pl.n_grp),
                          seq_ms=seq_ms, pk=pk, seq_bytes=seq_bytes,
                          seq_flops=seq_flops, dev_bytes=dev_bytes,
                          nfibs=F.tolist(), launches=launch_log)
@@ -251,6 +276,7 @@ def main() -> int:
             "learned_plan": lplan.groups(t.nmodes), "learned_plan_index": lplan_idx,
             "analytic_plan": plan.groups(t.nmodes), "analytic_plan_index": ch,
             "tiles": {k: v["tiles"] for k, v in res.items()},
+            "root_pass": {k: v["root_pass"] for k, v in res.items()},
         },
         "roofline": {
             "bound": "hbm", "kernel": launch_name(dcls, dinfo),

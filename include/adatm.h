@@ -223,6 +223,18 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
                               adatm_plan const *plan, int precision, int device,
                               uint32_t red_tile, uint32_t scat_tile,
                               adatm_engine **out);
+/* The same with creation flags for the root MTTKRP of a memo group: by
+ * default the leaf pass goes on to the root in one launch (up to 8 levels)
+ * and writes the memo levels it crosses beside its output;
+ * ADATM_ENGINE_STAGED (every group) or ADATM_ENGINE_STAGED_GROUP(g) (group g)
+ * makes it a chain of passes that stop at every memo level (each memo level
+ * written by one pass and read back by the next).  Results differ only in
+ * summation grouping. */
+#define ADATM_ENGINE_STAGED 0x1u
+#define ADATM_ENGINE_STAGED_GROUP(g) (0x100u << (g))
+int adatm_engine_create_ex(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
+                           int precision, int device, uint32_t red_tile,
+                           uint32_t scat_tile, uint32_t flags, adatm_engine **out);
 /* Children per batch of the fused reductions (4, 8, 16; 0 = default 8, or 4
  * for passes fusing more than 3 levels, with leaf passes in the wide layout
  * below where it applies; 16 applies to passes of <= 2 levels, 8 at most to
@@ -237,25 +249,31 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
  * batch 4 or 8, not pipelined. */
 #define ADATM_BATCH_WIDE 0x200u
 int adatm_engine_set_batch(adatm_engine *e, uint32_t batch);
-/* Autotuner: the tiles and batch of one plan chosen by timing CP-ALS sweeps
- * (nrep, median) of engines built with each candidate; a candidate replaces
- * the defaults (timed first) only when it is more than 1% faster.  batch
- * (nullable) gets the value for adatm_engine_set_batch (0 = default). */
+/* Autotuner: the tiles, root-pass form (flags for adatm_engine_create_ex)
+ * and batch of one plan, chosen by timing CP-ALS sweeps (nrep, median) of
+ * engines built with each candidate; a candidate replaces the defaults
+ * (timed first) only when it is more than 1% faster.  batch and flags are
+ * nullable. */
 int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                          int precision, int device, int nrep, uint32_t *red_tile,
-                         uint32_t *scat_tile, uint32_t *batch, double *best_ms);
+                         uint32_t *scat_tile, uint32_t *batch, uint32_t *flags,
+                         double *best_ms);
 /* The autotuner's search over any timer: timer(ctx, red_tile, scat_tile,
- * batch, &ms) times one candidate (batch as adatm_engine_set_batch) and
- * returns SPLATT_SUCCESS; the reduction tiles {128, 32, 64, 256, 512} (scatter
- * tile 64), the scatter tiles {16, 32, 128, 256} (best reduction tile), the
- * reduction tiles again when the scatter tile moved, then the batches {4, 8,
- * 16, 4|PIPELINED, 8|PIPELINED, 4|WIDE, 8|WIDE} on the best tiles, in that
- * order; a candidate wins only when more than 1% faster than the best so far.
- * adatm_autotune_tiles is this search over device CP-ALS sweeps. */
+ * batch, flags, &ms) times one candidate (batch as adatm_engine_set_batch,
+ * flags as adatm_engine_create_ex) and returns SPLATT_SUCCESS; in this
+ * order: the reduction tiles {128, 32, 64, 256, 512} (scatter tile 64), the
+ * scatter tiles {16, 32, 128, 256} (best reduction tile), the reduction tiles
+ * again when the scatter tile moved, then for each group g in the mask
+ * `groups` (the memo groups) the flags so far | ADATM_ENGINE_STAGED_GROUP(g),
+ * then the batches {4, 8, 16, 4|PIPELINED, 8|PIPELINED, 4|WIDE, 8|WIDE} with
+ * the best flags; a candidate wins only when more than 1% faster than the
+ * best so far.  adatm_autotune_tiles is this search over device CP-ALS
+ * sweeps. */
 typedef int (*adatm_tile_timer)(void *ctx, uint32_t red_tile, uint32_t scat_tile,
-                                uint32_t batch, double *ms);
-int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t *red_tile,
-                          uint32_t *scat_tile, uint32_t *batch, double *best_ms);
+                                uint32_t batch, uint32_t flags, double *ms);
+int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t groups,
+                          uint32_t *red_tile, uint32_t *scat_tile, uint32_t *batch,
+                          uint32_t *flags, double *best_ms);
 void adatm_engine_destroy(adatm_engine *e);
 
 /* Factor matrices are exchanged in FP64 row-major (dims[m] x rank). */

@@ -169,33 +169,40 @@ def test_autotune_search_order_and_margin(lib):
     """The autotuner's search on a table of timings: the default tiles are
     timed first; a candidate wins only when > 1% faster; when the scatter
     tile moves, the reduction tiles are timed again with it (skipping the one
-    already timed); the batches are timed last on the best tiles."""
+    already timed); then the root-pass forms, then the batches on the best
+    tiles and form."""
     P, W = A.BATCH_PIPELINED, A.BATCH_WIDE
-    table = {(128, 64, 0): 10.0, (32, 64, 0): 9.95,  # within 1%: not taken
-             (64, 64, 0): 9.0, (256, 64, 0): 11.0, (512, 64, 0): 12.0,
-             (64, 16, 0): 9.5, (64, 32, 0): 8.0, (64, 128, 0): 8.5, (64, 256, 0): 9.0,
-             (128, 32, 0): 8.05, (32, 32, 0): 7.0, (256, 32, 0): 7.0, (512, 32, 0): 9.0}
+    G0, G2 = A.ENGINE_STAGED_GROUP(0), A.ENGINE_STAGED_GROUP(2)
+    table = {(128, 64, 0, 0): 10.0, (32, 64, 0, 0): 9.95,  # within 1%: not taken
+             (64, 64, 0, 0): 9.0, (256, 64, 0, 0): 11.0, (512, 64, 0, 0): 12.0,
+             (64, 16, 0, 0): 9.5, (64, 32, 0, 0): 8.0, (64, 128, 0, 0): 8.5,
+             (64, 256, 0, 0): 9.0, (128, 32, 0, 0): 8.05, (32, 32, 0, 0): 7.0,
+             (256, 32, 0, 0): 7.0, (512, 32, 0, 0): 9.0,
+             (32, 32, 0, G0): 6.5, (32, 32, 0, G0 | G2): 6.0}
     calls = []
 
-    def timer(r, s, b):
-        calls.append((r, s, b))
-        return table.get((r, s, b), 7.0 if b == 8 | W else 6.97)
+    def timer(r, s, b, f):
+        calls.append((r, s, b, f))
+        return table.get((r, s, b, f), 6.0 if b == 8 | W else 5.97)
 
-    (r, s, b), ms = A.autotune_search(timer)
-    assert calls[:5] == [(128, 64, 0), (32, 64, 0), (64, 64, 0), (256, 64, 0), (512, 64, 0)]
-    assert calls[5:9] == [(64, 16, 0), (64, 32, 0), (64, 128, 0), (64, 256, 0)]
+    (r, s, b, f), ms = A.autotune_search(timer, groups=0b101)
+    assert calls[:5] == [(128, 64, 0, 0), (32, 64, 0, 0), (64, 64, 0, 0), (256, 64, 0, 0),
+                         (512, 64, 0, 0)]
+    assert calls[5:9] == [(64, 16, 0, 0), (64, 32, 0, 0), (64, 128, 0, 0), (64, 256, 0, 0)]
     # the scatter tile moved to 32: every reduction tile but 64 again with it
-    assert calls[9:13] == [(128, 32, 0), (32, 32, 0), (256, 32, 0), (512, 32, 0)]
+    assert calls[9:13] == [(128, 32, 0, 0), (32, 32, 0, 0), (256, 32, 0, 0), (512, 32, 0, 0)]
+    # group 0 staged (6.5 < 7.0: kept), then group 2 on top of it (6.0: kept)
+    assert calls[13:15] == [(32, 32, 0, G0), (32, 32, 0, G0 | G2)]
     batches = [4, 8, 16, 4 | P, 8 | P, 4 | W, 8 | W]
-    assert calls[13:] == [(32, 32, bt) for bt in batches]
-    # 256/32 ties 32/32 (not 1% faster); every batch within 1% of 7.0 loses
-    assert (r, s, b) == (32, 32, 0) and ms == 7.0
+    assert calls[15:] == [(32, 32, bt, G0 | G2) for bt in batches]
+    # 256/32 ties 32/32 (not 1% faster); every batch within 1% of 6.0 loses
+    assert (r, s, b, f) == (32, 32, 0, G0 | G2) and ms == 6.0
     # the scatter tile stays at its default: no third pass; a batch wins
     calls.clear()
-    (r, s, b), ms = A.autotune_search(lambda r, s, b: calls.append((r, s, b)) or
-                                      (5.0 if (r, s, b) == (128, 64, 8) else 10.0))
-    assert len(calls) == 5 + 4 + 7
-    assert (r, s, b) == (128, 64, 8) and ms == 5.0
+    (r, s, b, f), ms = A.autotune_search(lambda r, s, b, f: calls.append((r, s, b, f)) or
+                                         (5.0 if (r, s, b, f) == (128, 64, 8, 0) else 10.0))
+    assert len(calls) == 5 + 4 + 7  # no memo group in the mask
+    assert (r, s, b, f) == (128, 64, 8, 0) and ms == 5.0
 
 
 def test_stepwise_cpd_refuses_bad_arguments(lib):

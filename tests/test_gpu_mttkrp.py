@@ -41,15 +41,20 @@ CASES = [
 
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("prec", [A.FP64, A.FP32])
-def test_every_plan_every_mode(case, prec):
+@pytest.mark.parametrize("staged", [False, True])
+def test_every_plan_every_mode(case, prec, staged):
+    """Every candidate plan and mode against the COO brute force, with the
+    memo levels written beside one leaf-to-root pass (default; small tiles so
+    that many memo nodes are cut by work items) or by a chain of passes."""
     dims, nd, dist, R = CASES[case]
-    print(f"case={case} dims={dims} R={R} prec={prec}", flush=True)
+    print(f"case={case} dims={dims} R={R} prec={prec} staged={staged}", flush=True)
     t = _tensor(dims, nd, dist, seed=100 + case)
     U = _factors(t, R)
     inds, vals = t.inds().astype(np.int64), t.vals.copy()
     ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
     for plan in A.plan_candidates(t.nmodes, t.dims):
-        e = A.Engine(t, R, plan, precision=prec)
+        e = A.Engine(t, R, plan, precision=prec, staged=staged,
+                     tiles=(0, 0) if staged else (16, 0))
         for m in range(t.nmodes):
             e.set_factor(m, U[m])
         for m in range(t.nmodes):
@@ -183,9 +188,11 @@ def test_tiles_and_autotuner():
         for m in range(4):
             assert O.rel_fro(e.mttkrp(m), ref[m]) <= 1e-12, (tiles, m)
         e.close()
-    (rt, st, bt), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
+    (rt, st, bt, ft), ms = A.autotune_tiles(t, R, cands[ch], A.FP32, 0, 2)
     assert rt in (32, 64, 128, 256, 512) and st in (16, 32, 64, 128, 256)
     assert bt & ~(A.BATCH_PIPELINED | A.BATCH_WIDE) in (0, 4, 8, 16) and ms > 0
+    groups = sum(1 << g for g in range(cands[ch].n_grp) if cands[ch].grp[g].n_imten)
+    assert ft & ~sum(A.ENGINE_STAGED_GROUP(g) for g in range(16) if groups >> g & 1) == 0
 
 
 @pytest.mark.parametrize("prec,R", [(A.FP64, 16), (A.FP32, 16), (A.FP32, 32), (A.FP32, 48)])
@@ -400,3 +407,29 @@ def test_stepwise_sweeps_follow_the_device_counter():
     with pytest.raises(A.AdatmError):
         e.cpd_sweep(4096, graph=True)
     e.close()
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.FP32])
+def test_memo_levels_written_beside_the_root_pass(prec):
+    """The memo levels the one-pass root MTTKRP writes (reuse modes read
+    them) equal those of the staged chain: every mode of every plan agrees
+    between the two engines (FP64 to rounding, FP32 within 1e-6), over
+    tiles that cut long fibers into many pieces, and CP-ALS fits agree."""
+    print(f"fused vs staged prec={prec}", flush=True)
+    t = _tensor([300, 200, 100, 50, 20], 120000, A.DIST_ZIPF, seed=41)
+    R = 16
+    U = _factors(t, R, seed=8)
+    for plan in A.plan_candidates(t.nmodes, t.dims):
+        out = {}
+        for staged, tiles in ((True, (0, 0)), (False, (8, 8)), (False, (64, 0)),
+                              (False, (512, 0))):
+            e = A.Engine(t, R, plan, precision=prec, staged=staged, tiles=tiles)
+            for m in range(t.nmodes):
+                e.set_factor(m, U[m])
+            out[(staged, tiles)] = [e.mttkrp(m) for m in range(t.nmodes)]
+            e.close()
+        ref = out[(True, (0, 0))]
+        for key, M in out.items():
+            for m in range(t.nmodes):
+                err = O.rel_fro(M[m], ref[m])
+                assert err <= (1e-13 if prec == A.FP64 else 1e-6), (plan.groups(5), key, m, err)
