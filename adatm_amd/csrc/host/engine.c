@@ -1948,8 +1948,9 @@ int adatm_engine_set_batch(adatm_engine *e, uint32_t batch)
  * tile at its default, then the scatter tile over {16, 32, 64, 128, 256}
  * with the best reduction tile, then (when the scatter tile moved) the
  * reduction tile again with the new scatter tile; then, with the best
- * tiles, each memo group's root pass staged (against the default, one pass
- * to the root), keeping what is faster, group by group; then the
+ * tiles, every memo group's root pass staged (against the default, one
+ * pass to the root), then each group's form flipped in turn, keeping what
+ * is faster; then the
  * default batch against the children per batch over {4, 8, 16}, the
  * pipelined 4 and 8 and the wide 4 and 8.  The defaults are timed first and
  * a candidate displaces the best so far only when it is more than
@@ -1989,11 +1990,14 @@ int adatm_autotune_search(adatm_tile_timer timer, void *ctx, uint32_t groups,
       }
     }
   }
-  /* root passes: each memo group staged in turn (keeping what helped) */
-  uint32_t bf = 0;
-  for (uint32_t g = 0; g < ADATM_MAX_NMODES; ++g) {
-    if (!((groups >> g) & 1u)) continue;
-    uint32_t const f = bf | ADATM_ENGINE_STAGED_GROUP(g);
+  /* root passes: every memo group staged, then each group's form flipped in
+     turn from the better of the two (keeping what helped) */
+  uint32_t bf = 0, all = 0;
+  for (uint32_t g = 0; g < ADATM_MAX_NMODES; ++g)
+    if ((groups >> g) & 1u) all |= ADATM_ENGINE_STAGED_GROUP(g);
+  for (int k = -1; all && k < ADATM_MAX_NMODES; ++k) {
+    if (k >= 0 && !((groups >> k) & 1u)) continue;
+    uint32_t const f = (k < 0) ? all : bf ^ ADATM_ENGINE_STAGED_GROUP(k);
     double ms = INFINITY;
     int const rc = timer(ctx, br, bs, 0, f, &ms);
     if (rc != SPLATT_SUCCESS) return rc;

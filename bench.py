@@ -81,6 +81,9 @@ def finite(x):
     return x
 
 
+PREC_NAME = {0: "fp64", 1: "fp32", 2: "bf16rows"}
+
+
 def root_pass_name(flags: int, ngrp: int) -> str:
     """the root-pass form of every memo group"""
     import adatm_amd as A
@@ -126,6 +129,9 @@ def main() -> int:
     ap.add_argument("--model", default="", help="cost model file (default: the trained one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default work-item tiles instead of the autotuner's")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner instead of using its recorded choice for the "
+                         "named workload (adatm_amd/planner/tuned_tiles.json)")
     ap.add_argument("--tiles", default="", metavar="RED,SCAT[,BATCH[,FLAGS]]",
                     help="use these work-item tiles (reduction batch, engine creation flags) "
                          "for every engine (no autotuning)")
@@ -191,13 +197,28 @@ def main() -> int:
         runs.append(("plain_fp64", plan, A.FP64))
     res = {}
     Mref = {}
+    tile_src = {}
+    tuned_table = {}
+    try:
+        with open(os.path.join(ROOT, "adatm_amd", "planner", "tuned_tiles.json")) as f:
+            tuned_table = json.load(f)
+    except OSError:
+        pass
     for name, pl, prec in runs:
         t2 = time.time()
         tiles = (0, 0, 0)
+        pidx = lplan_idx if pl is lplan else ch
+        tuned = tuned_table.get(f"C{args.config}/R{R}/plan{pidx}/{PREC_NAME[prec]}")
         if args.tiles:
             tiles = tuple(int(x) for x in args.tiles.split(","))
+            tile_src[name] = "--tiles"
+        elif tuned and not args.autotune and not args.ndraws:
+            tiles = tuple(tuned[:4])
+            tile_src[name] = "adatm_amd/planner/tuned_tiles.json (autotuner, adatm_amd/tune_named.py)"
+            log(f"{name}: tuned tiles {tiles} (autotuner's {tuned[4]:.3f} ms/sweep)")
         elif not args.no_autotune:
             tiles, tms = A.autotune_tiles(t, R, pl, prec, 0, 3)
+            tile_src[name] = "autotuned in this run"
             log(f"{name}: autotuned tiles {tiles} ({tms:.3f} ms/sweep, {time.time() - t2:.1f}s)")
         e = A.Engine(t, R, pl, precision=prec, device=0, tiles=tiles[:3],
                      flags=root_flags(args.root_pass, tiles))
@@ -276,6 +297,7 @@ def main() -> int:
             "learned_plan": lplan.groups(t.nmodes), "learned_plan_index": lplan_idx,
             "analytic_plan": plan.groups(t.nmodes), "analytic_plan_index": ch,
             "tiles": {k: v["tiles"] for k, v in res.items()},
+            "tiles_source": tile_src,
             "root_pass": {k: v["root_pass"] for k, v in res.items()},
         },
         "roofline": {

@@ -263,9 +263,10 @@ int adatm_autotune_tiles(adatm_coo const *t, uint64_t rank, adatm_plan const *pl
  * flags as adatm_engine_create_ex) and returns SPLATT_SUCCESS; in this
  * order: the reduction tiles {128, 32, 64, 256, 512} (scatter tile 64), the
  * scatter tiles {16, 32, 128, 256} (best reduction tile), the reduction tiles
- * again when the scatter tile moved, then for each group g in the mask
- * `groups` (the memo groups) the flags so far | ADATM_ENGINE_STAGED_GROUP(g),
- * then the batches {4, 8, 16, 4|PIPELINED, 8|PIPELINED, 4|WIDE, 8|WIDE} with
+ * again when the scatter tile moved, then the flags staging every group in
+ * the mask `groups` (the memo groups), then for each such group g the best
+ * flags so far with g's ADATM_ENGINE_STAGED_GROUP(g) bit flipped, then the
+ * batches {4, 8, 16, 4|PIPELINED, 8|PIPELINED, 4|WIDE, 8|WIDE} with
  * the best flags; a candidate wins only when more than 1% faster than the
  * best so far.  adatm_autotune_tiles is this search over device CP-ALS
  * sweeps. */

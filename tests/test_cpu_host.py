@@ -178,7 +178,7 @@ def test_autotune_search_order_and_margin(lib):
              (64, 16, 0, 0): 9.5, (64, 32, 0, 0): 8.0, (64, 128, 0, 0): 8.5,
              (64, 256, 0, 0): 9.0, (128, 32, 0, 0): 8.05, (32, 32, 0, 0): 7.0,
              (256, 32, 0, 0): 7.0, (512, 32, 0, 0): 9.0,
-             (32, 32, 0, G0): 6.5, (32, 32, 0, G0 | G2): 6.0}
+             (32, 32, 0, G0 | G2): 6.5, (32, 32, 0, G2): 6.0}
     calls = []
 
     def timer(r, s, b, f):
@@ -191,12 +191,13 @@ def test_autotune_search_order_and_margin(lib):
     assert calls[5:9] == [(64, 16, 0, 0), (64, 32, 0, 0), (64, 128, 0, 0), (64, 256, 0, 0)]
     # the scatter tile moved to 32: every reduction tile but 64 again with it
     assert calls[9:13] == [(128, 32, 0, 0), (32, 32, 0, 0), (256, 32, 0, 0), (512, 32, 0, 0)]
-    # group 0 staged (6.5 < 7.0: kept), then group 2 on top of it (6.0: kept)
-    assert calls[13:15] == [(32, 32, 0, G0), (32, 32, 0, G0 | G2)]
+    # both memo groups staged (6.5 < 7.0: kept), then group 0 back to one
+    # pass (6.0: kept), then group 2 back too (7.0: not kept)
+    assert calls[13:16] == [(32, 32, 0, G0 | G2), (32, 32, 0, G2), (32, 32, 0, 0)]
     batches = [4, 8, 16, 4 | P, 8 | P, 4 | W, 8 | W]
-    assert calls[15:] == [(32, 32, bt, G0 | G2) for bt in batches]
+    assert calls[16:] == [(32, 32, bt, G2) for bt in batches]
     # 256/32 ties 32/32 (not 1% faster); every batch within 1% of 6.0 loses
-    assert (r, s, b, f) == (32, 32, 0, G0 | G2) and ms == 6.0
+    assert (r, s, b, f) == (32, 32, 0, G2) and ms == 6.0
     # the scatter tile stays at its default: no third pass; a batch wins
     calls.clear()
     (r, s, b, f), ms = A.autotune_search(lambda r, s, b, f: calls.append((r, s, b, f)) or
