@@ -177,8 +177,13 @@ __global__ __launch_bounds__(256) void k_reduce(
           ldrow(xin + ((size_t)(w0 + src) * R + cv), xv[i]);
 #pragma unroll
         for (int j = 1; j < K; ++j) {
+          /* a level-j node's row is used once, when the node closes: a
+             child that does not close it loads row 0 instead (the group's
+             close bit; no branch, and the line stays cached), so the node's
+             row is fetched once per node rather than once per child */
           uint32_t const nid_j = getl(wn[j], src);
-          ldrow((TU const *)ch.U[j] + (nid_j * R + cv), uc[j][i]);
+          uint32_t const row_j = ((cm[j] >> (s0 + (uint32_t)i)) & 1u) ? nid_j : 0u;
+          ldrow((TU const *)ch.U[j] + (row_j * R + cv), uc[j][i]);
         }
       }
     };
