@@ -45,6 +45,7 @@ BATCH_PIPELINED = 0x100
 # flag of Engine.set_batch: 16-byte row pieces per lane (128-bit gathers)
 BATCH_WIDE = 0x200
 CSF_ONEMODE, CSF_TWOMODE, CSF_ALLMODE = 0, 1, 2
+NOTILE, DENSETILE = 0, 1
 ADAPTIVE, ADAPTIVE_TIME_EFFICIENT, ADAPTIVE_SPACE_EFFICIENT = 3, 4, 5
 
 
@@ -211,6 +212,11 @@ def lib() -> C.CDLL:
                                             P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                             P(C.c_double)]),
         "adatm_k_bf16_bits": (C.c_uint16, [C.c_float]),
+        "adatm_perm_nvtxs": (C.c_int, [P(Coo), C.c_int, C.c_uint32, C.c_int,
+                                       P(C.c_uint64)]),
+        "adatm_perm_compute": (C.c_int, [P(Coo), C.c_int, C.c_uint32, P(C.c_uint32),
+                                         C.c_uint64, C.c_int, P(P(C.c_uint32))]),
+        "adatm_coo_relabel": (None, [P(Coo), P(P(C.c_uint32))]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -363,6 +369,36 @@ def plan_candidates(nmodes: int, dims) -> list[Plan]:
     arr = (Plan * 256)()
     n = lib().adatm_plan_candidates(nmodes, _u64ptr(d), arr, 256)
     return [arr[i] for i in range(n)]
+
+
+PERM_GRAPH, PERM_HGRAPH, PERM_FIBSCHED, PERM_ERROR = 0, 1, 2, 3
+
+
+def perm_nvtxs(t: Tensor, kind: int, mode: int = 0, device: int = -1) -> int:
+    """Vertices of the partition that relabelling `kind` reads."""
+    n = C.c_uint64(0)
+    check(lib().adatm_perm_nvtxs(C.byref(t.c), kind, mode, device, C.byref(n)),
+          "perm_nvtxs")
+    return int(n.value)
+
+
+def perm_compute(t: Tensor, kind: int, parts, mode: int = 0,
+                 device: int = -1) -> list[np.ndarray]:
+    """New label of every index (old -> new), one array per mode."""
+    parts = np.ascontiguousarray(parts, dtype=np.uint32)
+    out = [np.zeros(max(d, 1), dtype=np.uint32) for d in t.dims]
+    ptrs = (C.POINTER(C.c_uint32) * t.nmodes)(*[_uptr(a) for a in out])
+    check(lib().adatm_perm_compute(C.byref(t.c), kind, mode, _uptr(parts), parts.size,
+                                   device, ptrs), "perm_compute")
+    return [a[:d] for a, d in zip(out, t.dims)]
+
+
+def relabel(t: Tensor, perms) -> None:
+    """t.ind[m][j] = perms[m][t.ind[m][j]] (None leaves a mode alone)."""
+    keep = [None if p is None else np.ascontiguousarray(p, dtype=np.uint32) for p in perms]
+    ptrs = (C.POINTER(C.c_uint32) * t.nmodes)(
+        *[C.POINTER(C.c_uint32)() if p is None else _uptr(p) for p in keep])
+    lib().adatm_coo_relabel(C.byref(t.c), ptrs)
 
 
 def count_fibers(t: Tensor, order, device: int = -1) -> list[int]:

@@ -199,53 +199,142 @@ static int plain_plan(int alloc, uint32_t N, uint64_t const *dims,
 }
 
 /* ---- host CSF (u32) <-> SPLATT layout ------------------------------------ */
-static splatt_csf *csf_array_from_hcsf(adatm_hcsf const *h, uint32_t n,
-                                       uint32_t N, uint64_t const *dims)
+/* one tile's arrays from the subtree of levels lv0.. of h rooted at the
+ * level-lv0 nodes [lo, hi) (lv0 = 0, whole range: the untiled CSF) */
+static int fill_sparsity(csf_sparsity *pt, adatm_hcsf const *h, uint32_t lv0, uint64_t lo,
+                         uint64_t hi)
 {
-  splatt_csf *out = (splatt_csf *)calloc(n, sizeof(splatt_csf));
-  if (!out) return NULL;
-  for (uint32_t g = 0; g < n; ++g) {
-    splatt_csf *c = out + g;
-    c->nnz = h[g].nnz;
-    c->nmodes = N;
-    for (uint32_t m = 0; m < N; ++m) {
-      c->dims[m] = dims[m];
-      c->dim_perm[m] = h[g].order[m];
-      c->tile_dims[m] = 1;
+  uint32_t const N = h->N - lv0;
+  for (uint32_t l = 0; l < N; ++l) {
+    uint32_t const hl = lv0 + l;
+    uint64_t const F = hi - lo;
+    pt->nfibs[l] = F;
+    pt->fids[l] = (splatt_idx_t *)malloc((F ? F : 1) * sizeof(splatt_idx_t));
+    if (!pt->fids[l]) return SPLATT_ERROR_NOMEMORY;
+    for (uint64_t i = 0; i < F; ++i) pt->fids[l][i] = h->fids[hl][lo + i];
+    if (l + 1 < N) {
+      uint32_t const *fp = h->fptr[hl];
+      pt->fptr[l] = (splatt_idx_t *)malloc((F + 1) * sizeof(splatt_idx_t));
+      if (!pt->fptr[l]) return SPLATT_ERROR_NOMEMORY;
+      for (uint64_t i = 0; i <= F; ++i) pt->fptr[l][i] = fp[lo + i] - fp[lo];
+      uint64_t const nlo = fp[lo], nhi = fp[hi];
+      lo = nlo;
+      hi = nhi;
+    } else {
+      pt->vals = (splatt_val_t *)malloc((F ? F : 1) * sizeof(splatt_val_t));
+      if (!pt->vals) return SPLATT_ERROR_NOMEMORY;
+      memcpy(pt->vals, h->vals + lo, F * sizeof(double));
     }
-    c->which_tile = SPLATT_NOTILE;
-    c->ntiles = 1;
-    c->pt = (csf_sparsity *)calloc(1, sizeof(csf_sparsity));
-    if (!c->pt) return out;
-    csf_sparsity *pt = c->pt;
-    for (uint32_t l = 0; l < N; ++l) {
-      uint64_t const F = h[g].F[l];
-      pt->nfibs[l] = F;
-      pt->fids[l] = (splatt_idx_t *)malloc((F ? F : 1) * sizeof(splatt_idx_t));
-      for (uint64_t i = 0; i < F; ++i) pt->fids[l][i] = h[g].fids[l][i];
-      if (l + 1 < N) {
-        pt->fptr[l] = (splatt_idx_t *)malloc((F + 1) * sizeof(splatt_idx_t));
-        for (uint64_t i = 0; i <= F; ++i) pt->fptr[l][i] = h[g].fptr[l][i];
-      }
-    }
-    pt->vals = (splatt_val_t *)malloc(h[g].nnz * sizeof(splatt_val_t));
-    memcpy(pt->vals, h[g].vals, h[g].nnz * sizeof(double));
   }
-  return out;
+  return SPLATT_SUCCESS;
+}
+
+static void csf_head(splatt_csf *c, uint32_t N, uint64_t nnz, uint64_t const *dims,
+                     uint32_t const *order)
+{
+  c->nnz = nnz;
+  c->nmodes = N;
+  for (uint32_t m = 0; m < N; ++m) {
+    c->dims[m] = dims[m];
+    c->dim_perm[m] = order[m];
+    c->tile_dims[m] = 1;
+  }
+  c->which_tile = SPLATT_NOTILE;
+  c->ntiles = 1;
+}
+
+static int csf_from_hcsf(splatt_csf *c, adatm_hcsf const *h, uint64_t const *dims)
+{
+  csf_head(c, h->N, h->nnz, dims, h->order);
+  c->pt = (csf_sparsity *)calloc(1, sizeof(csf_sparsity));
+  if (!c->pt) return SPLATT_ERROR_NOMEMORY;
+  return fill_sparsity(c->pt, h, 0, 0, h->F[0]);
+}
+
+/*
+ * SPLATT's dense tiling (SPLATT_DENSETILE; p_csf_alloc_densetile and
+ * tt_densetile of SPLATT v1.1.1, selected by --tile at cmd_bench.c:151 and
+ * cmd_cpd.c:90).  Every level at depth >= `depth` is cut into `ntile`
+ * equal index ranges (tile size max(dim / ntile, 1), the last range takes
+ * the rest), the others into one; tile ids are row-major over the modes
+ * (the last mode fastest) and each tile holds the CSF of its nonzeros in
+ * the untiled CSF order.  Built on the device as one CSF with the tile id
+ * as an extra leading level, then cut per tile.  The MTTKRP of a tiled set
+ * runs on the GPU's own (untiled) structures: tiling only schedules CPU
+ * threads and does not change the result beyond summation order.
+ */
+#define MAX_TILES (1u << 20)
+static int csf_densetile(splatt_csf *c, adatm_coo const *t, uint32_t const *order,
+                         uint64_t ntile, uint64_t depth, int device)
+{
+  uint32_t const N = t->nmodes;
+  csf_head(c, N, t->nnz, t->dims, order);
+  uint64_t td[MAXN], tsz[MAXN], ntiles = 1;
+  for (uint32_t m = 0; m < N; ++m) td[m] = 1;
+  for (uint32_t d = (uint32_t)depth; d < N; ++d) td[order[d]] = ntile ? ntile : 1;
+  for (uint32_t m = 0; m < N; ++m) {
+    ntiles *= td[m];
+    if (ntiles > MAX_TILES) {
+      adatm_set_error("dense tiling: more than %u tiles", MAX_TILES);
+      return SPLATT_ERROR_BADINPUT;
+    }
+    tsz[m] = t->dims[m] / td[m] ? t->dims[m] / td[m] : 1;
+    c->tile_dims[m] = td[m];
+  }
+  c->which_tile = SPLATT_DENSETILE;
+  c->ntiles = ntiles;
+  c->pt = (csf_sparsity *)calloc(ntiles, sizeof(csf_sparsity));
+  if (!c->pt) return SPLATT_ERROR_NOMEMORY;
+  /* the tile id as mode N, sorted first */
+  adatm_coo x = *t;
+  x.nmodes = N + 1;
+  x.dims[N] = ntiles;
+  x.ind[N] = (uint32_t *)malloc(t->nnz * sizeof(uint32_t));
+  if (!x.ind[N]) return SPLATT_ERROR_NOMEMORY;
+  #pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)t->nnz; ++j) {
+    uint64_t id = 0;
+    for (uint32_t m = 0; m < N; ++m) {
+      uint64_t cm = t->ind[m][j] / tsz[m];
+      if (cm >= td[m]) cm = td[m] - 1;
+      id = id * td[m] + cm;
+    }
+    x.ind[N][j] = (uint32_t)id;
+  }
+  uint32_t xo[MAXN + 1];
+  xo[0] = N;
+  memcpy(xo + 1, order, N * sizeof(uint32_t));
+  adatm_hcsf h;
+  int rc = device >= 0 ? adatm_hcsf_build_device(&x, xo, device, &h)
+                       : adatm_hcsf_build(&x, xo, &h);
+  free(x.ind[N]);
+  if (rc != SPLATT_SUCCESS) return rc;
+  for (uint64_t k = 0; rc == SPLATT_SUCCESS && k < h.F[0]; ++k)
+    rc = fill_sparsity(c->pt + h.fids[0][k], &h, 1, h.fptr[0][k], h.fptr[0][k + 1]);
+  /* empty tiles: no fibers, a two-entry root pointer (SPLATT's layout) */
+  for (uint64_t tl = 0; rc == SPLATT_SUCCESS && tl < ntiles; ++tl)
+    if (!c->pt[tl].fids[0]) {
+      c->pt[tl].fptr[0] = (splatt_idx_t *)calloc(2, sizeof(splatt_idx_t));
+      if (!c->pt[tl].fptr[0]) rc = SPLATT_ERROR_NOMEMORY;
+    }
+  adatm_hcsf_free(&h);
+  return rc;
 }
 
 static void free_csf_array(splatt_csf *t, uint32_t n)
 {
   if (!t) return;
   for (uint32_t g = 0; g < n; ++g) {
-    csf_sparsity *pt = t[g].pt;
-    if (!pt) continue;
-    for (uint32_t l = 0; l < SPLATT_MAX_NMODES; ++l) {
-      free(pt->fptr[l]);
-      free(pt->fids[l]);
+    if (!t[g].pt) continue;
+    for (uint64_t k = 0; k < t[g].ntiles; ++k) {
+      csf_sparsity *pt = t[g].pt + k;
+      for (uint32_t l = 0; l < SPLATT_MAX_NMODES; ++l) {
+        free(pt->fptr[l]);
+        free(pt->fids[l]);
+      }
+      free(pt->vals);
     }
-    free(pt->vals);
-    free(pt);
+    free(t[g].pt);
   }
   free(t);
 }
@@ -325,11 +414,26 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
     return rc;
   }
   uint32_t const ng = r->plan.n_grp;
-  adatm_hcsf *h = (adatm_hcsf *)calloc(ng, sizeof(adatm_hcsf));
+  /* SPLATT's CSF sets may be tiled; AdaTM's never are (adatm_rcsf.c:304) */
+  int const tiled = !given && (int)opts[SPLATT_OPTION_TILE] == SPLATT_DENSETILE &&
+                    (alloc == SPLATT_CSF_ONEMODE || alloc == SPLATT_CSF_TWOMODE ||
+                     alloc == SPLATT_CSF_ALLMODE);
+  splatt_csf *arr = (splatt_csf *)calloc(ng, sizeof(splatt_csf));
+  if (!arr) rc = SPLATT_ERROR_NOMEMORY;
   /* on the device: the GPU radix sort and scans; the host build otherwise */
-  for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < ng; ++g)
-    rc = (r->device >= 0) ? adatm_hcsf_build_device(t, r->plan.grp[g].o_opt, r->device, h + g)
-                          : adatm_hcsf_build(t, r->plan.grp[g].o_opt, h + g);
+  for (uint32_t g = 0; rc == SPLATT_SUCCESS && g < ng; ++g) {
+    if (tiled) {
+      double const nt = opts[SPLATT_OPTION_NTHREADS], dp = opts[SPLATT_OPTION_TILEDEPTH];
+      rc = csf_densetile(arr + g, t, r->plan.grp[g].o_opt, nt >= 1 ? (uint64_t)nt : 1,
+                         dp >= 0 ? (uint64_t)dp : 0, r->device);
+      continue;
+    }
+    adatm_hcsf h;
+    rc = (r->device >= 0) ? adatm_hcsf_build_device(t, r->plan.grp[g].o_opt, r->device, &h)
+                          : adatm_hcsf_build(t, r->plan.grp[g].o_opt, &h);
+    if (rc == SPLATT_SUCCESS) rc = csf_from_hcsf(arr + g, &h, t->dims);
+    adatm_hcsf_free(&h);
+  }
   if (rc == SPLATT_SUCCESS && r->device >= 0) {
     /* keep the coordinates: engines for this CSF set are built on the device */
     rc = adatm_coo_alloc(N, t->nnz, t->dims, &r->coo);
@@ -337,13 +441,6 @@ static int build_from_coo(adatm_coo const *t, double const *opts,
       memcpy(r->coo.ind[m], t->ind[m], t->nnz * sizeof(uint32_t));
     if (rc == SPLATT_SUCCESS) memcpy(r->coo.vals, t->vals, t->nnz * sizeof(double));
   }
-  splatt_csf *arr = NULL;
-  if (rc == SPLATT_SUCCESS) {
-    arr = csf_array_from_hcsf(h, ng, N, t->dims);
-    if (!arr) rc = SPLATT_ERROR_NOMEMORY;
-  }
-  for (uint32_t g = 0; g < ng; ++g) adatm_hcsf_free(h + g);
-  free(h);
   if (rc != SPLATT_SUCCESS) {
     free_csf_array(arr, ng);
     adatm_coo_free(&r->coo);

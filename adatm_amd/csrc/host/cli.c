@@ -15,10 +15,16 @@
  * TENSOR is a coordinate text file (.tns: N 1-based indices and a value per
  * line) or synth:<1|2|3>[:draws] for the seeded synthetic workloads.
  * Duplicate coordinates are summed and empty indices removed on input.
- * The reference's reordering options (-z TYPE, -p FILE, -m MODE;
- * cmd_bench.c:84-85, 119-136) are recognised and refused: this build does
- * not reorder.  --tile is accepted and, as in the reference's AdaTM paths
- * (adatm_bench.c:535), has no effect.  The options of this build
+ * Reordering follows cmd_bench.c:119-149, 243-251: -p FILE relabels the
+ * tensor with the partition FILE by the type of -z (graph, hgraph, fib;
+ * basis mode -m, 1-based, default 1); -z without -p does nothing; -p
+ * without a known -z is an error.  The factor matrices are drawn in the
+ * original labels and follow their rows to the new ones
+ * (adatm_bench.c:40-51, 529), and -w writes every MTTKRP back in the
+ * original labels, so the outputs do not depend on the reordering.
+ * --tile gives -a csf SPLATT's dense tiling (SPLATT_DENSETILE, one tile
+ * per thread along every level below the root); the AdaTM CSFs are
+ * untiled as in the reference (adatm_bench.c:535).  The options of this build
  * (--precision, --planner, --seed) are long options only, so every short
  * option keeps the reference's meaning.
  */
@@ -170,8 +176,12 @@ struct bench_args {
   int adatm, csf;
   char *badalg;
   uint32_t niters, nthreads, rank;
-  int prec, planner, scale, write;
-  char *reorder; /* the first reordering option given (refused) */
+  int prec, planner, scale, write, tile;
+  int rtype;      /* splatt_perm_type of -z; PERM_ERROR when absent/unknown */
+  char *permerr;  /* the unknown -z argument */
+  char *pfile;    /* -p */
+  uint32_t permmode;
+  uint32_t *perm[ADATM_MAX_NMODES]; /* old -> new labels after -p, else NULL */
 };
 
 #define OPT_TILE 255
@@ -180,14 +190,14 @@ struct bench_args {
 static struct argp_option bench_opts[] = {
     {"alg", 'a', "ALG", 0, "algorithm to benchmark: adatm, csf (repeatable)", 0},
     {"iters", 'i', "NITERS", 0, "number of iterations to use (default: 5)", 0},
-    {"mode", 'm', "MODE", 0, "mode basis for hgraph reordering (not supported)", 0},
+    {"mode", 'm', "MODE", 0, "mode basis for hgraph reordering (default: 1)", 0},
     {"threads", 't', "NTHREADS", 0, "number of threads to use (default: 1)", 0},
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 16)", 0},
     {"scale", 's', 0, 0, "scale threads from 1 to NTHREADS (by 2)", 0},
-    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT (no effect here)", 0},
+    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT", 0},
     {"write", 'w', 0, 0, "write results to files ALG_mode<N>.mat (for testing)", 0},
-    {"rtype", 'z', "TYPE", 0, "designate reordering type (not supported)", 0},
-    {"pfile", 'p', "FILE", 0, "partition file for reordering (not supported)", 0},
+    {"rtype", 'z', "TYPE", 0, "designate reordering type (graph, hgraph, fib)", 0},
+    {"pfile", 'p', "FILE", 0, "partition file for reordering", 0},
     {"precision", OPT_PREC, "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},
     {"planner", OPT_PLANNER, "PLANNER", 0,
      "analytic | learned | exhaustive (default: analytic)", 0},
@@ -224,10 +234,21 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
   case 'r': a->rank = (uint32_t)atoi(arg); break;
   case 's': a->scale = 1; break;
   case 'w': a->write = 1; break;
-  case OPT_TILE: break;
-  case 'z': case 'p': case 'm':
-    if (!a->reorder) a->reorder = key == 'z' ? "-z" : (key == 'p' ? "-p" : "-m");
+  case OPT_TILE: a->tile = 1; break;
+  case 'z':
+    if (!strcmp(arg, "graph"))
+      a->rtype = PERM_GRAPH;
+    else if (!strcmp(arg, "hgraph"))
+      a->rtype = PERM_HGRAPH;
+    else if (!strcmp(arg, "fib"))
+      a->rtype = PERM_FIBSCHED;
+    else {
+      a->rtype = PERM_ERROR;
+      a->permerr = arg;
+    }
     break;
+  case 'p': a->pfile = arg; break;
+  case 'm': a->permmode = (uint32_t)(atoi(arg) - 1); break;
   case OPT_PREC: a->prec = parse_prec(arg); break;
   case OPT_PLANNER: a->planner = parse_planner(arg); break;
   case ARGP_KEY_ARG:
@@ -244,6 +265,32 @@ static error_t bench_parse(int key, char *arg, struct argp_state *st)
 
 static void write_matrix(char const *fname, double const *A, uint64_t I,
                          uint64_t J);
+
+/* rows of a relabelled matrix back to the original labels: row i of the
+ * result is row perm[i] of A (in place through a copy) */
+static void unshuffle_rows(double *A, uint32_t const *perm, uint64_t I, uint64_t J)
+{
+  double *c = (double *)malloc(I * J * sizeof(double));
+  if (!c) return;
+  memcpy(c, A, I * J * sizeof(double));
+  for (uint64_t i = 0; i < I; ++i) memcpy(A + i * J, c + (uint64_t)perm[i] * J, J * sizeof(double));
+  free(c);
+}
+
+/* the initial factor of mode m (seed 1), drawn in the original labels and
+ * moved to the new ones: row perm[i] = drawn row i (adatm_bench.c:40-51) */
+static void bench_factor(struct bench_args const *a, adatm_coo const *t, uint32_t m,
+                         double *A)
+{
+  adatm_rand_matrix(1, m, t->dims[m], a->rank, A);
+  if (!a->perm[m]) return;
+  double *c = (double *)malloc(t->dims[m] * a->rank * sizeof(double));
+  if (!c) return;
+  memcpy(c, A, t->dims[m] * a->rank * sizeof(double));
+  for (uint64_t i = 0; i < t->dims[m]; ++i)
+    memcpy(A + (uint64_t)a->perm[m][i] * a->rank, c + i * a->rank, a->rank * sizeof(double));
+  free(c);
+}
 
 /* thread counts of a run: NTHREADS, or 1, 2, 4, ..., NTHREADS with -s
  * (cmd_bench.c:174-201); the GPU path does not use host threads, so only
@@ -286,6 +333,7 @@ static int bench_loop(struct bench_args const *a, adatm_coo const *t, char const
         if (w) {
           char fn[64];
           snprintf(fn, sizeof(fn), "%s_mode%u.mat", alg, m + 1);
+          if (a->perm[m]) unshuffle_rows(out, a->perm[m], t->dims[m], a->rank);
           write_matrix(fn, out, t->dims[m], a->rank);
         }
       }
@@ -326,6 +374,8 @@ static int bench_csf(struct bench_args const *a, adatm_coo const *t)
   c.opts = splatt_default_opts();
   c.opts[SPLATT_OPTION_CSF_ALLOC] = SPLATT_CSF_TWOMODE;
   c.opts[SPLATT_OPTION_ADATM_PRECISION] = a->prec;
+  c.opts[SPLATT_OPTION_NTHREADS] = a->nthreads ? a->nthreads : 1;
+  if (a->tile) c.opts[SPLATT_OPTION_TILE] = SPLATT_DENSETILE;
   splatt_idx_t *ind[ADATM_MAX_NMODES];
   uint64_t maxd = 0;
   for (uint32_t m = 0; m < t->nmodes; ++m) {
@@ -343,7 +393,7 @@ static int bench_csf(struct bench_args const *a, adatm_coo const *t)
   c.U = (double **)calloc(t->nmodes, sizeof(double *));
   for (uint32_t m = 0; m < t->nmodes; ++m) {
     c.U[m] = (double *)malloc(t->dims[m] * a->rank * sizeof(double));
-    adatm_rand_matrix(1, m, t->dims[m], a->rank, c.U[m]);
+    bench_factor(a, t, m, c.U[m]);
   }
   c.scratch = (double *)malloc(maxd * a->rank * sizeof(double));
   rc = bench_loop(a, t, "csf", csf_mttkrp_fn, &c);
@@ -355,6 +405,39 @@ static int bench_csf(struct bench_args const *a, adatm_coo const *t)
   return rc;
 }
 
+/* -p FILE: relabel the tensor (cmd_bench.c:237-251); a->perm gets the
+ * labels so the factors and -w outputs can follow them */
+static int reorder(struct bench_args *a, adatm_coo *t)
+{
+  if (a->rtype == PERM_ERROR) {
+    fprintf(stderr, "SPLATT: reordering algorithm '%s' is not recognized.\n"
+                    "Run with '--help' for assistance.\n",
+            a->permerr ? a->permerr : "(null)");
+    return -1;
+  }
+  printf("Reordering ------------------------------------------------------\n");
+  double const t0 = now();
+  uint64_t np = 0;
+  uint32_t *parts = NULL;
+  int rc = adatm_part_read(a->pfile, &np, &parts);
+  for (uint32_t m = 0; rc == SPLATT_SUCCESS && m < t->nmodes; ++m)
+    if (!(a->perm[m] = (uint32_t *)malloc(t->dims[m] * sizeof(uint32_t))))
+      rc = SPLATT_ERROR_NOMEMORY;
+  if (rc == SPLATT_SUCCESS)
+    rc = adatm_perm_compute(t, a->rtype, a->permmode, parts, np, 0, a->perm);
+  free(parts);
+  if (rc != SPLATT_SUCCESS) {
+    fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
+    return -1;
+  }
+  adatm_coo_relabel(t, a->perm);
+  static char const *const names[] = {"graph", "hgraph", "fib"};
+  printf("TYPE=%s", names[a->rtype]);
+  if (a->rtype != PERM_GRAPH) printf(" MODE=%u", a->permmode + 1);
+  printf(" TIME=%0.3fs\n\n", now() - t0);
+  return 0;
+}
+
 static int cmd_bench(int argc, char **argv)
 {
   struct bench_args a;
@@ -364,6 +447,7 @@ static int cmd_bench(int argc, char **argv)
   a.rank = 16;
   a.prec = SPLATT_ADATM_FP64;
   a.planner = SPLATT_ADATM_PLAN_ANALYTIC;
+  a.rtype = PERM_ERROR;
   struct argp ap = {bench_opts, bench_parse, "TENSOR -a ALG",
                     "splatt-bench -- benchmark the memoized MTTKRP sequence.", 0, 0, 0};
   argp_parse(&ap, argc, argv, ARGP_IN_ORDER, 0, &a);
@@ -372,17 +456,16 @@ static int cmd_bench(int argc, char **argv)
             a.badalg ? a.badalg : "");
     return EXIT_FAILURE;
   }
-  if (a.reorder) {
-    fprintf(stderr, "SPLATT: %s: tensor reordering is not supported by this build.\n",
-            a.reorder);
-    return EXIT_FAILURE;
-  }
   adatm_coo t;
   if (load_tensor(a.tensor, &t) != 0) {
     fprintf(stderr, "SPLATT: cannot load '%s': %s\n", a.tensor, adatm_last_error());
     return EXIT_FAILURE;
   }
   print_header(&t, a.tensor);
+  if (a.pfile && reorder(&a, &t) != 0) {
+    adatm_coo_free(&t);
+    return EXIT_FAILURE;
+  }
   printf("Benchmarking ---------------------------------------------------\n");
   printf("RANK=%u ITS=%u\n", a.rank, a.niters);
   int rc = 0;
@@ -397,10 +480,20 @@ static int cmd_bench(int argc, char **argv)
       return EXIT_FAILURE;
     }
     print_storage(e, &plan, t.nmodes);
-    adatm_engine_init_factors(e, 1);
-    rc = bench_loop(&a, &t, "adatm", adatm_mttkrp_fn, e);
+    double *A = NULL;
+    uint64_t maxd = 0;
+    for (uint32_t m = 0; m < t.nmodes; ++m) maxd = t.dims[m] > maxd ? t.dims[m] : maxd;
+    A = (double *)malloc(maxd * a.rank * sizeof(double));
+    for (uint32_t m = 0; rc == 0 && m < t.nmodes; ++m) {
+      bench_factor(&a, &t, m, A);
+      if (!A || adatm_engine_set_factor(e, m, A) != SPLATT_SUCCESS) rc = -1;
+    }
+    free(A);
+    if (rc == 0) rc = bench_loop(&a, &t, "adatm", adatm_mttkrp_fn, e);
+    else fprintf(stderr, "SPLATT: %s\n", adatm_last_error());
     adatm_engine_destroy(e);
   }
+  for (uint32_t m = 0; m < ADATM_MAX_NMODES; ++m) free(a.perm[m]);
   adatm_coo_free(&t);
   return rc == 0 ? EXIT_SUCCESS : EXIT_FAILURE;
 }
@@ -423,7 +516,7 @@ static struct argp_option cpd_opts[] = {
     {"tol", OPT_TOL, "TOLERANCE", 0, "minimum change for convergence (default: 1e-5)", 0},
     {"rank", 'r', "RANK", 0, "rank of decomposition to find (default: 10)", 0},
     {"threads", 't', "NTHREADS", 0, "number of threads to use (default: #cores)", 0},
-    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT (no effect here)", 0},
+    {"tile", OPT_TILE, 0, 0, "use tiling during SPLATT (AdaTM's CSFs are untiled)", 0},
     {"nowrite", OPT_NOWRITE, 0, 0, "do not write output to file", 0},
     {"verbose", 'v', 0, 0, "turn on verbose output (default: no)", 0},
     {"precision", OPT_PREC, "P", 0, "fp64 | fp32 | bf16rows (default: fp64)", 0},

@@ -307,8 +307,10 @@ def main() -> int:
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": (f"profiles/c{args.config}_hbm_traffic.md (rocprofv3 --pmc, "
-                               f"copy-calibrated)" if traffic is not None else None),
+            "traffic_source": (f"profiles/c{args.config}_hbm_traffic.md: L2-miss (fabric) "
+                               f"bytes, rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE scaled by a "
+                               f"copy kernel's; includes Infinity Cache hits, so an upper bound "
+                               f"on HBM bytes" if traffic is not None else None),
             "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,

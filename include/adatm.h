@@ -78,6 +78,34 @@ void adatm_coo_free(adatm_coo *t);
 int adatm_coo_alloc(uint32_t nmodes, uint64_t nnz, uint64_t const *dims,
                     adatm_coo *out);
 
+/* Relabelling of a coordinate tensor's indices (the computation behind
+ * SPLATT's tt_perm).  type is splatt_perm_type (PERM_GRAPH, PERM_HGRAPH,
+ * PERM_FIBSCHED); `mode` is the basis mode of the latter two, whose CSF
+ * (root `mode`, the other modes shortest first) is built on the GPU
+ * `device` (device < 0: on the host).  parts holds one part id per vertex:
+ *   PERM_GRAPH     sum(dims) vertices, mode 0's indices first;
+ *   PERM_HGRAPH    the CSF's fibers (nodes of level N-2), in CSF order;
+ *   PERM_FIBSCHED  the CSF's slices (nodes of level 0), in CSF order.
+ * PERM_GRAPH orders each mode's indices by (part, index).  The other two
+ * visit the vertices by (part, CSF position) and give every mode's indices
+ * new labels in the order the visit first touches them (the vertex's path
+ * from the root, then its subtree in CSF order); untouched indices follow
+ * in index order.  nparts holds at least that many part ids (the first
+ * ones are used; fewer is SPLATT_ERROR_BADINPUT).  perms[m] (dims[m]
+ * entries each, caller-allocated) gets old -> new. */
+int adatm_perm_compute(adatm_coo const *t, int type, uint32_t mode,
+                       uint32_t const *parts, uint64_t nparts, int device,
+                       uint32_t **perms);
+/* number of vertices adatm_perm_compute reads for (type, mode): sum(dims),
+ * or the fiber / slice count of the basis CSF (counted on `device`) */
+int adatm_perm_nvtxs(adatm_coo const *t, int type, uint32_t mode, int device,
+                     uint64_t *nvtxs);
+/* read a partition file: whitespace-separated non-negative integers below
+ * 2^32 (anything else is SPLATT_ERROR_BADINPUT); malloc'd result */
+int adatm_part_read(char const *fname, uint64_t *count, uint32_t **parts);
+/* t->ind[m][j] = perms[m][t->ind[m][j]] for every non-NULL perms[m] */
+void adatm_coo_relabel(adatm_coo *t, uint32_t *const *perms);
+
 /* Factor matrix initialisation used by both the CPU and the GPU paths:
  * A[i*R + r] = U(0,1) from Philox stream (seed, mode). */
 void adatm_rand_matrix(uint64_t seed, uint32_t mode, uint64_t rows,

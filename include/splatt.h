@@ -219,6 +219,24 @@ typedef struct {
   int rowmajor;
 } matrix_t;
 
+/*
+ * Index relabelling of a tensor (SPLATT reorder.h, not distributed with the
+ * reference; used at cmd_bench.c:119-149, 243-251 and adatm_bench.c:26-51,
+ * 529, 654).  perms[m][old] = new, iperms[m][new] = old; a NULL perms[m]
+ * leaves mode m as it is.
+ */
+typedef enum {
+  PERM_GRAPH,    /* partition of the m-partite index graph (all modes) */
+  PERM_HGRAPH,   /* partition of the fibers of one mode's CSF */
+  PERM_FIBSCHED, /* partition of the slices of one mode's CSF */
+  PERM_ERROR
+} splatt_perm_type;
+
+typedef struct {
+  splatt_idx_t *perms[SPLATT_MAX_NMODES];
+  splatt_idx_t *iperms[SPLATT_MAX_NMODES];
+} permutation_t;
+
 #define SPLATT_ADATM_SLIMIT 90000000000UL /* reference adatm_base.h:4 */
 
 #ifdef __cplusplus
@@ -397,6 +415,26 @@ void csf_free(splatt_csf *tensors, double const *const opts);
 void splatt_csf_free(splatt_csf *tensors, double const *const opts);
 matrix_t *mat_alloc(splatt_idx_t const nrows, splatt_idx_t const ncols);
 void mat_free(matrix_t *M);
+
+/* Relabelling (SPLATT reorder.h; callers cmd_bench.c:243-251,
+ * adatm_bench.c:26-51).
+ *   tt_perm     reads the partition file PFILE (one part id per vertex, in
+ *               vertex order, whitespace separated), builds the relabelling
+ *               of `type` (basis mode `mode` for PERM_HGRAPH/PERM_FIBSCHED)
+ *               and applies it to tt.  NULL on error (adatm_last_error()).
+ *   perm_alloc  identity permutations for the first nmodes modes (nmodes 0:
+ *               every mode NULL, "no relabelling", cmd_bench.c:250).
+ *   perm_apply  tt->ind[m][j] = perm[m][tt->ind[m][j]] for every non-NULL
+ *               perm[m].
+ *   perm_matrix row perm[i] of the result is row i of mat (the rows follow
+ *               their index to its new label); retmat NULL allocates. */
+permutation_t *tt_perm(sptensor_t *const tt, splatt_perm_type const type,
+                       splatt_idx_t const mode, char const *const pfile);
+permutation_t *perm_alloc(splatt_idx_t const *const dims, splatt_idx_t const nmodes);
+void perm_free(permutation_t *perm);
+void perm_apply(sptensor_t *const tt, splatt_idx_t **perm);
+matrix_t *perm_matrix(matrix_t const *const mat, splatt_idx_t const *const perm,
+                      matrix_t *retmat);
 
 /* CP-ALS over a memoization plan (reference adatm_cpd.c:205-253). */
 int splatt_cpd_als_adaptive(splatt_csf const *const tensors,

@@ -18,6 +18,14 @@ Contents (each function cites the reference it restates):
                     1547-1682), computed level by level with numpy
   cpd_als           cpd_als_iterate_adaptive (src/adatm_cpd.c:256-374) with
                     SPLATT's Gram / inverse / normalisation, FP64
+  basis_order, perm_graph, perm_first_touch, shuffle_rows
+                    SPLATT's tt_perm / perm_apply / perm_matrix (reorder.c of
+                    SPLATT v1.1.1, absent from the reference; called at
+                    src/cmds/cmd_bench.c:243-251, src/adatm_bench.c:26-51,
+                    529, 654), stated by nonzero positions
+  densetile         SPLATT_DENSETILE (tile.c / csf.c of SPLATT v1.1.1,
+                    absent; selected at src/cmds/cmd_bench.c:151,
+                    src/cmds/cmd_cpd.c:90)
 """
 from __future__ import annotations
 
@@ -417,3 +425,86 @@ def rel_fro(a: np.ndarray, b: np.ndarray) -> float:
     d = np.linalg.norm(a - b)
     n = np.linalg.norm(b)
     return float(d / n) if n > 0 else float(d)
+
+
+# ---------------------------------------------------------------------------
+# relabelling (SPLATT reorder.c; callers src/cmds/cmd_bench.c:243-251,
+# src/adatm_bench.c:40-51)
+def basis_order(dims, mode: int) -> list[int]:
+    """Level order of a mode's own CSF: the mode, then the others by
+    (dimension, mode)."""
+    rest = sorted((m for m in range(len(dims)) if m != mode), key=lambda m: (dims[m], m))
+    return [mode] + rest
+
+
+def perm_graph(dims, parts) -> list[np.ndarray]:
+    """Each mode's indices ordered by (part of the index's graph vertex,
+    index); vertices are mode 0's indices, then mode 1's, ...  old -> new."""
+    out, off = [], 0
+    for d in dims:
+        key = np.asarray(parts[off:off + d], dtype=np.int64)
+        rank = np.empty(d, dtype=np.int64)
+        rank[np.argsort(key, kind="stable")] = np.arange(d)
+        out.append(rank)
+        off += d
+    return out
+
+
+def perm_first_touch(inds, dims, mode: int, level: int, parts) -> list[np.ndarray]:
+    """Visit the nonzeros of mode `mode`'s CSF grouped by the part of their
+    level-`level` ancestor (stable in CSF order); each mode's new labels go
+    to its indices in order of first appearance, unseen indices last in
+    index order.  old -> new."""
+    N = inds.shape[0]
+    order = basis_order(dims, mode)
+    p = lex_order(inds, order)
+    s = inds[order][:, p]
+    nnz = s.shape[1]
+    new = np.zeros(nnz, dtype=bool)
+    new[0] = True
+    for l in range(level + 1):
+        new[1:] |= s[l, 1:] != s[l, :-1]
+    vertex = np.cumsum(new) - 1
+    vparts = np.asarray(parts, dtype=np.int64)[vertex]
+    q = np.argsort(vparts, kind="stable")
+    out = [None] * N
+    for l, m in enumerate(order):
+        seq = s[l, q]
+        uniq, first = np.unique(seq, return_index=True)
+        seen = uniq[np.argsort(first, kind="stable")]
+        unseen = np.setdiff1d(np.arange(dims[m]), uniq)
+        lab = np.empty(dims[m], dtype=np.int64)
+        lab[np.concatenate([seen, unseen]).astype(np.int64)] = np.arange(dims[m])
+        out[m] = lab
+    return out
+
+
+def shuffle_rows(A: np.ndarray, perm) -> np.ndarray:
+    """Row perm[i] of the result is row i of A (perm_matrix)."""
+    B = np.empty_like(A)
+    B[np.asarray(perm, dtype=np.int64)] = A
+    return B
+
+
+# ---------------------------------------------------------------------------
+# dense tiling (SPLATT_DENSETILE)
+def densetile(inds, vals, dims, order, ntile: int, depth: int):
+    """Tiles of a CSF with levels at depth >= `depth` cut into `ntile` index
+    ranges of size max(dim // ntile, 1) (the last takes the rest).  Returns
+    (tile_dims, tiles) with tiles[t] = csf(...) of tile t's nonzeros (or
+    None when empty); tile ids are row-major over modes 0..N-1."""
+    N = inds.shape[0]
+    td = [1] * N
+    for d in range(depth, N):
+        td[order[d]] = max(ntile, 1)
+    tid = np.zeros(inds.shape[1], dtype=np.int64)
+    for m in range(N):
+        size = max(dims[m] // td[m], 1)
+        c = np.minimum(inds[m].astype(np.int64) // size, td[m] - 1)
+        tid = tid * td[m] + c
+    ntiles = int(np.prod(td))
+    tiles = []
+    for t in range(ntiles):
+        sel = np.flatnonzero(tid == t)
+        tiles.append(csf(inds[:, sel], vals[sel], order) if sel.size else None)
+    return td, tiles

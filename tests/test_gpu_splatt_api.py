@@ -228,7 +228,8 @@ def test_cli_outputs_match_the_oracle(tmp_path):
     (factors from seed 1) and `splatt cpd` its fits and factors: both against
     the oracle on the canonical form of the fixture file (the CLI sums
     duplicates and drops empty indices on input).  The reference's
-    reordering options are refused, -s scales the thread count by 2."""
+    reordering options need a partition file, -s scales the thread count
+    by 2."""
     exe = str(A.CLI_PATH)
     tns = os.path.join(os.path.dirname(__file__), "golden", "small4.tns")
     a = np.loadtxt(tns, comments="#", ndmin=2)
@@ -259,10 +260,13 @@ def test_cli_outputs_match_the_oracle(tmp_path):
         F = np.loadtxt(tmp_path / f"mode{m + 1}.mat")
         assert O.rel_fro(F, oA[m] / s_) <= 1e-6, m
     assert O.rel_fro(lam, olam) <= 1e-6
-    for opt in (["-p", "part.txt"], ["-z", "graph"], ["-m", "2"]):
-        r = subprocess.run([exe, "bench", tns, "-a", "adatm", *opt], capture_output=True,
-                           text=True, timeout=60, cwd=tmp_path)
-        assert r.returncode != 0 and "reordering is not supported" in r.stderr, opt
+    r = subprocess.run([exe, "bench", tns, "-a", "adatm", "-p", "part.txt"],
+                       capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode != 0 and "algorithm '(null)' is not recognized" in r.stderr
+    for opt in (["-z", "graph"], ["-m", "2"]):  # no partition file: no reordering
+        r = subprocess.run([exe, "bench", tns, "-a", "adatm", "-i", "1", *opt],
+                           capture_output=True, text=True, timeout=60, cwd=tmp_path)
+        assert r.returncode == 0 and "Reordering" not in r.stdout, (opt, r.stderr)
     r = subprocess.run([exe, "bench", tns, "-a", "splatt"], capture_output=True, text=True,
                        timeout=60, cwd=tmp_path)
     assert r.returncode != 0 and "not recognized" in r.stderr
@@ -329,4 +333,159 @@ def test_device_built_csf_equals_host_built(alloc):
     for a, b in zip(*built):
         assert a[0] == b[0] and a[1] == b[1]
         for x, y in zip(a[2] + a[3] + [a[4]], b[2] + b[3] + [b[4]]):
+            assert np.array_equal(x, y)
+
+
+def _canonical_fixture():
+    tns = os.path.join(os.path.dirname(__file__), "golden", "small4.tns")
+    a = np.loadtxt(tns, comments="#", ndmin=2)
+    inds0 = a[:, :4].T.astype(np.int64) - 1
+    ci, cv, cd, _ = O.canonicalize(inds0, a[:, 4].copy(),
+                                   [int(x) + 1 for x in inds0.max(axis=1)])
+    return tns, ci, cv, cd
+
+
+@pytest.mark.parametrize("kind,mode", [("graph", 1), ("hgraph", 2), ("fib", 1), ("fib", 4)])
+def test_cli_reordering_keeps_outputs(tmp_path, kind, mode):
+    """`splatt bench -z KIND -p FILE -m MODE -w`: the tensor is relabelled,
+    the factors follow their rows, and the MTTKRPs written back in the
+    original labels equal the oracle's on the unrelabelled tensor; -a csf
+    with --tile (dense tiling, 3 tiles per tiled level) as well."""
+    print(f"cli reorder kind={kind} mode={mode}", flush=True)
+    exe = str(A.CLI_PATH)
+    tns, ci, cv, cd = _canonical_fixture()
+    if kind == "graph":
+        nv = sum(cd)
+    else:
+        F = O.count_fibers(ci, O.basis_order(cd, mode - 1))
+        nv = F[len(cd) - 2] if kind == "hgraph" else F[0]
+    parts = np.random.default_rng(mode).integers(0, 7, nv)
+    np.savetxt(tmp_path / "p.part", parts, fmt="%d")
+    R = 8
+    r = subprocess.run([exe, "bench", tns, "-a", "adatm", "-a", "csf", "-w", "-i", "1",
+                        "-r", str(R), "-z", kind, "-p", "p.part", "-m", str(mode), "-t", "3",
+                        "--tile"], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert f"TYPE={kind}" in r.stdout
+    U = [A.rand_matrix(1, m, d, R) for m, d in enumerate(cd)]
+    for alg in ("adatm", "csf"):
+        for m in range(4):
+            M = np.loadtxt(tmp_path / f"{alg}_mode{m + 1}.mat").reshape(cd[m], R)
+            assert O.rel_fro(M, O.coo_mttkrp(ci, cv, U, m)) <= 1e-7, (alg, m)
+
+
+@pytest.mark.parametrize("kind", [A.PERM_HGRAPH, A.PERM_FIBSCHED])
+def test_device_relabel_matches_oracle(kind):
+    """The relabelling with the basis CSF built on the GPU equals the oracle,
+    on a tensor with duplicates and empty indices."""
+    print(f"device relabel kind={kind}", flush=True)
+    dims = [50, 40, 30, 20, 10]
+    inds, vals = _tensor(21, dims, 40000)
+    t = A.Tensor.from_arrays(inds, vals, dims)
+    rng = np.random.default_rng(kind)
+    for mode in range(len(dims)):
+        nv = A.perm_nvtxs(t, kind, mode, device=0)
+        parts = rng.integers(0, 16, nv)
+        got = A.perm_compute(t, kind, parts, mode, device=0)
+        level = len(dims) - 2 if kind == A.PERM_HGRAPH else 0
+        want = O.perm_first_touch(inds.astype(np.int64), dims, mode, level, parts)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w), mode
+
+
+class Sptensor(C.Structure):
+    _fields_ = [("nnz", C.c_uint64), ("nmodes", C.c_uint64), ("dims", C.POINTER(C.c_uint64)),
+                ("ind", C.POINTER(C.POINTER(C.c_uint64))), ("vals", C.POINTER(C.c_double)),
+                ("tiled", C.c_int), ("indmap", C.POINTER(C.c_uint64) * MAXM)]
+
+
+class Permutation(C.Structure):
+    _fields_ = [("perms", C.POINTER(C.c_uint64) * MAXM),
+                ("iperms", C.POINTER(C.c_uint64) * MAXM)]
+
+
+def test_tt_perm_relabels_in_place(tmp_path):
+    """SPLATT's tt_perm on a tensor from tt_read (duplicates kept): perms and
+    iperms are inverse, the indices are relabelled, and perm_matrix moves
+    row i to row perms[i]."""
+    L = _lib()
+    L.tt_read.restype = C.POINTER(Sptensor)
+    L.tt_read.argtypes = [C.c_char_p]
+    L.tt_free.argtypes = [C.POINTER(Sptensor)]
+    L.tt_perm.restype = C.POINTER(Permutation)
+    L.tt_perm.argtypes = [C.POINTER(Sptensor), C.c_int, C.c_uint64, C.c_char_p]
+    L.perm_free.argtypes = [C.POINTER(Permutation)]
+    tns, _, _, _ = _canonical_fixture()
+    a = np.loadtxt(tns, comments="#", ndmin=2)
+    inds = a[:, :4].T.astype(np.int64) - 1
+    dims = [int(x) + 1 for x in inds.max(axis=1)]
+    mode = 2
+    F = O.count_fibers(inds, O.basis_order(dims, mode))
+    parts = np.random.default_rng(2).integers(0, 5, F[2])
+    np.savetxt(tmp_path / "h.part", parts, fmt="%d")
+    tt = L.tt_read(tns.encode())
+    assert tt
+    p = L.tt_perm(tt, A.PERM_HGRAPH, mode, str(tmp_path / "h.part").encode())
+    assert p, A.lib().adatm_last_error()
+    want = O.perm_first_touch(inds, dims, mode, 2, parts)
+    for m in range(4):
+        perm = np.ctypeslib.as_array(p.contents.perms[m], shape=(dims[m],))
+        iperm = np.ctypeslib.as_array(p.contents.iperms[m], shape=(dims[m],))
+        assert np.array_equal(perm, want[m])
+        assert np.array_equal(iperm[perm], np.arange(dims[m]))
+        got = np.ctypeslib.as_array(tt.contents.ind[m], shape=(inds.shape[1],))
+        assert np.array_equal(got, want[m][inds[m]])
+    L.perm_free(p)
+    L.tt_free(tt)
+
+
+@pytest.mark.parametrize("alloc", [A.CSF_ONEMODE, A.CSF_ALLMODE])
+def test_densetile_device_equals_host_and_mttkrp(alloc):
+    """A DENSETILE CSF set built on the GPU equals the host build tile by
+    tile, and splatt_mttkrp on it equals the oracle (FP64, 1e-12)."""
+    print(f"densetile alloc={alloc}", flush=True)
+    L = _lib()
+    dims = [40, 30, 20, 12]
+    inds, vals = _tensor(31, dims, 20000)
+    N = len(dims)
+    built = []
+    for dev in (0, -1):
+        opts = L.splatt_default_opts()
+        opts[A.OPT_CSF_ALLOC] = alloc
+        opts[A.OPT_DEVICE] = dev
+        opts[A.OPT_TILE] = A.DENSETILE
+        opts[A.OPT_NTHREADS] = 4
+        csf = _convert(L, inds, vals, opts)
+        arr = C.cast(csf, C.POINTER(SplattCsf))
+        tiles = []
+        for g in range({A.CSF_ONEMODE: 1, A.CSF_ALLMODE: N}[alloc]):
+            c = arr[g]
+            assert c.which_tile == A.DENSETILE and c.ntiles == 4 ** (N - 1)
+            for k in range(int(c.ntiles)):
+                pt = c.pt[k]
+                F = [int(pt.nfibs[l]) for l in range(N)]
+                if F[0] == 0:
+                    tiles.append(None)
+                    continue
+                tiles.append((F, [np.ctypeslib.as_array(pt.fptr[l], shape=(F[l] + 1,)).copy()
+                                  for l in range(N - 1)],
+                              [np.ctypeslib.as_array(pt.fids[l], shape=(F[l],)).copy()
+                               for l in range(N)],
+                              np.ctypeslib.as_array(pt.vals, shape=(F[N - 1],)).copy()))
+        built.append(tiles)
+        if dev == 0:
+            rng = np.random.default_rng(5)
+            U = [rng.random((d, 8)) for d in dims]
+            for m in range(N):
+                got = _mttkrp(L, csf, m, U, opts)
+                assert O.rel_fro(got, O.coo_mttkrp(inds.astype(np.int64), vals, U, m)) <= 1e-12
+        L.splatt_free_csf(csf, opts)
+        L.splatt_free_opts(opts)
+    assert len(built[0]) == len(built[1])
+    for a, b in zip(*built):
+        assert (a is None) == (b is None)
+        if a is None:
+            continue
+        assert a[0] == b[0]
+        for x, y in zip(a[1] + a[2] + [a[3]], b[1] + b[2] + [b[3]]):
             assert np.array_equal(x, y)
