@@ -52,77 +52,103 @@ __global__ void k_convert_out(T const *__restrict__ src, double *__restrict__ ds
 /* ---- dense CP-ALS steps ------------------------------------------------- */
 #define DB 256          /* threads per dense block */
 #define NBLK_MAX 256    /* partial-sum blocks */
-#define GROWS 32        /* rows staged per Gram step */
 
-/* partial Gram of rows [lo, hi); optional column scaling 1/lambda written
- * back to A (and its bf16 copy) before accumulation */
-template <typename T>
-__device__ __forceinline__ void gram_part_body(uint32_t bid, uint32_t nblk, T *__restrict__ A,
-                                                  bf16raw *__restrict__ Abf,
-                                                  uint64_t I, uint32_t R,
-                                                  double const *__restrict__ lambda,
-                                                  double *__restrict__ part)
+/* Gram of FP64 rows on the matrix cores: v_mfma_f64_16x16x4f64 takes four
+ * rows of A per instruction.  Columns are cut into NT tiles of 16; with
+ * a = A[r0 + lane/16][16i + lane%16] and b = A[r0 + lane/16][16j + lane%16]
+ * the instruction's A operand (16 x 4, element [m][k] from lane m + 16k) is
+ * tile i of the rows transposed and its B operand (4 x 16, [k][n] from lane
+ * n + 16k) tile j, so the 16 x 16 accumulator gathers tile (i, j) of A^T A;
+ * accumulator register q of lane l holds element (row 4q + l/16, column
+ * l%16).  The tiles i <= j of the symmetric result are accumulated (1, 3, 6
+ * or 10 for R <= 16, 32, 48, 64), columns >= R are zero.  Each block sums
+ * its four waves' tiles in a fixed order and writes one FP64 partial
+ * R x R (summed across blocks by k_sum_parts).  With lambda, the rows are
+ * first divided by it and written back. */
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__device__ __forceinline__ void gram_mfma64_body(uint32_t bid, uint32_t nblk,
+                                                 double *__restrict__ A, uint64_t I,
+                                                 uint32_t R, double const *__restrict__ lambda,
+                                                 double *__restrict__ part)
 {
-  HIP_DYNAMIC_SHARED(double, sh) /* GROWS * R */
-  uint64_t const chunk = (I + nblk - 1) / nblk;
+  constexpr int P = NT * (NT + 1) / 2;
+  __shared__ double red[3][P][256];
+  uint32_t const lane = threadIdx.x & 63u;
+  uint32_t const w = threadIdx.x >> 6;
+  uint32_t const col = lane & 15u, kq = lane >> 4;
+  uint64_t const chunk = ((I + nblk - 1) / nblk + 15) & ~(uint64_t)15;
   uint64_t const lo = (uint64_t)bid * chunk;
   uint64_t const hi = lo + chunk < I ? lo + chunk : I;
-  uint32_t const RR = R * R;
-  double *myp = part + (size_t)bid * RR;
-  for (uint32_t e0 = 0; e0 < RR; e0 += DB * 16) {
-    double acc[16];
+  double lam[NT];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) acc[t] = 0.0;
-    for (uint64_t r0 = lo; r0 < hi; r0 += GROWS) {
-      uint32_t const nr = (uint32_t)((hi - r0) < GROWS ? (hi - r0) : GROWS);
-      __syncthreads();
-      for (uint32_t x = threadIdx.x; x < nr * R; x += DB) {
-        uint32_t const rr = x / R, cc = x - rr * R;
-        size_t const gi = (size_t)(r0 + rr) * R + cc;
-        double v = (double)A[gi];
+  for (int t = 0; t < NT; ++t)
+    lam[t] = (lambda && 16u * t + col < R) ? lambda[16u * t + col] : 1.0;
+  doublex4 acc[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+  for (uint64_t rp = lo + 4u * w; rp < hi; rp += 16u) {
+    uint64_t const r = rp + kq;
+    double v[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      uint32_t const c = 16u * t + col;
+      v[t] = 0.0;
+      if (r < hi && c < R) {
+        size_t const g = (size_t)r * R + c;
+        v[t] = A[g];
         if (lambda) {
-          T const sv = (T)(v / lambda[cc]);
-          if (e0 == 0) {
-            A[gi] = sv;
-            if (Abf) Abf[gi].v = to_bf16((float)sv);
-          }
-          v = (double)sv;
-        }
-        sh[x] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        uint32_t const e = e0 + threadIdx.x + DB * t;
-        if (e < RR) {
-          uint32_t const i = e / R, j = e - i * R;
-          double s = acc[t];
-          for (uint32_t rr = 0; rr < nr; ++rr) s += sh[rr * R + i] * sh[rr * R + j];
-          acc[t] = s;
+          v[t] = v[t] / lam[t];
+          A[g] = v[t];
         }
       }
     }
+    int p = 0;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      uint32_t const e = e0 + threadIdx.x + DB * t;
-      if (e < RR) myp[e] = acc[t];
-    }
-    /* a second entry chunk re-stages rows; scaling is applied only once */
-    if (lambda && e0 == 0 && RR > DB * 16) {
-      /* subsequent chunks must read the already-scaled values unscaled */
-      lambda = nullptr;
-    }
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = i; j < NT; ++j, ++p)
+        acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[i], v[j], acc[p], 0, 0, 0);
   }
+  if (w > 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red[w - 1][p][q * 64 + lane] = acc[p][q];
+  }
+  __syncthreads();
+  if (w != 0) return;
+  for (uint32_t ww = 0; ww < 3; ++ww)
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += red[ww][p][q * 64 + lane];
+  double *myp = part + (size_t)bid * R * R;
+  int p = 0;
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = i; j < NT; ++j, ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint32_t const gr = 16u * i + 4u * (uint32_t)q + kq, gc = 16u * j + col;
+        if (gr < R && gc < R) {
+          myp[(size_t)gr * R + gc] = acc[p][q];
+          if (i != j) myp[(size_t)gc * R + gr] = acc[p][q];
+        }
+      }
 }
 
-template <typename T>
-__global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
-                                                  bf16raw *__restrict__ Abf,
-                                                  uint64_t I, uint32_t R,
-                                                  double const *__restrict__ lambda,
-                                                  double *__restrict__ part)
+template <int NT>
+__global__ __launch_bounds__(DB) void k_gram_mfma64(double *__restrict__ A, uint64_t I,
+                                                    uint32_t R,
+                                                    double const *__restrict__ lambda,
+                                                    double *__restrict__ part)
 {
-  gram_part_body<T>(blockIdx.x, gridDim.x, A, Abf, I, R, lambda, part);
+  gram_mfma64_body<NT>(blockIdx.x, gridDim.x, A, I, R, lambda, part);
 }
 
 /* Gram of FP32 rows on the matrix cores: v_mfma_f32_32x32x2f32 takes two
@@ -135,7 +161,7 @@ __global__ __launch_bounds__(DB) void k_gram_part(T *__restrict__ A,
  * columns >= R are zero.  Each block sums its waves' tiles in a fixed order
  * and writes one FP64 partial R x R (summed across blocks in FP64 by
  * k_sum_parts).  With lambda, the rows are first divided by it and written
- * back (and to the bf16 copy), as k_gram_part does. */
+ * back (and to the bf16 copy). */
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ void gram_mfma_body(uint32_t bid, uint32_t nblk, float *__restrict__ A,
@@ -602,10 +628,14 @@ hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
   if constexpr (sizeof(T) == 4) {
     hipLaunchKernelGGL(k_gram_mfma, dim3(nb), dim3(DB), 0, s, (float *)A, Abf, I, R,
                        lambda, part);
+  } else if (R <= 16) {
+    hipLaunchKernelGGL((k_gram_mfma64<1>), dim3(nb), dim3(DB), 0, s, A, I, R, lambda, part);
+  } else if (R <= 32) {
+    hipLaunchKernelGGL((k_gram_mfma64<2>), dim3(nb), dim3(DB), 0, s, A, I, R, lambda, part);
+  } else if (R <= 48) {
+    hipLaunchKernelGGL((k_gram_mfma64<3>), dim3(nb), dim3(DB), 0, s, A, I, R, lambda, part);
   } else {
-    size_t const lds = (size_t)GROWS * R * sizeof(double);
-    hipLaunchKernelGGL((k_gram_part<T>), dim3(nb), dim3(DB), lds, s, A, Abf, I,
-                       R, lambda, part);
+    hipLaunchKernelGGL((k_gram_mfma64<4>), dim3(nb), dim3(DB), 0, s, A, I, R, lambda, part);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -646,13 +676,13 @@ __global__ __launch_bounds__(DB) void g_gram_mfma(adatm_kdense const *__restrict
                  x.lambda, x.part);
 }
 
-template <typename T>
-__global__ __launch_bounds__(DB) void g_gram_part(adatm_kdense const *__restrict__ d,
-                                                  uint32_t const *__restrict__ bm)
+template <int NT>
+__global__ __launch_bounds__(DB) void g_gram_mfma64(adatm_kdense const *__restrict__ d,
+                                                    uint32_t const *__restrict__ bm)
 {
   adatm_kdense const &x = d[bm[blockIdx.x]];
-  gram_part_body<T>(blockIdx.x - x.gr0, x.ngr, (T *)x.A, nullptr, x.I, x.R, x.lambda,
-                    x.part);
+  gram_mfma64_body<NT>(blockIdx.x - x.gr0, x.ngr, (double *)x.A, x.I, x.R, x.lambda,
+                       x.part);
 }
 
 __global__ __launch_bounds__(DB) void g_sum(adatm_kdense const *__restrict__ d,
@@ -782,8 +812,14 @@ int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uin
   hipLaunchKernelGGL(g_lambda, dim3(n), dim3(DB), 0, s, desc);
   if ((e = hipGetLastError()) != hipSuccess) return herr(e);
   if (prec == KP_FP64) {
-    size_t const rows = (size_t)GROWS * Rmax * sizeof(double);
-    hipLaunchKernelGGL((g_gram_part<double>), dim3(nb_gr), dim3(DB), rows, s, desc, bm_gr);
+    if (Rmax <= 16)
+      hipLaunchKernelGGL((g_gram_mfma64<1>), dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
+    else if (Rmax <= 32)
+      hipLaunchKernelGGL((g_gram_mfma64<2>), dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
+    else if (Rmax <= 48)
+      hipLaunchKernelGGL((g_gram_mfma64<3>), dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
+    else
+      hipLaunchKernelGGL((g_gram_mfma64<4>), dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
   } else {
     hipLaunchKernelGGL(g_gram_mfma, dim3(nb_gr), dim3(DB), 0, s, desc, bm_gr);
   }

@@ -171,6 +171,26 @@ def test_cpd_fit_ranks_fp32(R):
     assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= 1e-4, (fits, ref)
 
 
+@pytest.mark.parametrize("R", [10, 20, 40, 64])
+def test_cpd_fit_ranks_fp64(R):
+    """FP64 sweeps whose Grams run on the FP64 matrix cores in 1, 2, 3 and 4
+    column tiles of 16 (R = 10, 20, 40, 64) match the oracle's fits to 1e-9."""
+    print(f"cpd fp64 ranks R={R}", flush=True)
+    t = _tensor([90, 80, 70, 66], 40000, A.DIST_ZIPF, seed=24)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    U = _factors(t, R, seed=79)
+    cands, ch = A.plan_analytic(t, R)
+    for graph in (False, True):
+        e = A.Engine(t, R, cands[ch], precision=A.FP64)
+        for m in range(4):
+            e.set_factor(m, U[m])
+        fits = e.cpd(6, tol=0.0, graph=graph)
+        ref, _, _ = O.cpd_als(lambda m, F: O.coo_mttkrp(inds, vals, F, m), U, 6,
+                              float((vals ** 2).sum()))
+        assert np.max(np.abs(np.asarray(fits) - np.asarray(ref))) <= 1e-9, (graph, fits, ref)
+        e.close()
+
+
 def test_tiles_and_autotuner():
     """Work-item tiles change only the summation grouping: every tile gives
     the oracle's MTTKRP; the autotuner's choice is one of its candidates."""
