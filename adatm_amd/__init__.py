@@ -172,6 +172,8 @@ def lib() -> C.CDLL:
         "adatm_batch_destroy": (None, [C.c_void_p]),
         "adatm_batch_cpd": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "adatm_batch_sequential": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "adatm_batch_set_persistent": (C.c_int, [C.c_void_p, C.c_int]),
+        "adatm_batch_persistent": (C.c_int, [C.c_void_p]),
         "adatm_mlp_nparams": (C.c_size_t, []),
         "adatm_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(C.c_void_p)]),
         "adatm_mlp_destroy": (None, [C.c_void_p]),
@@ -784,6 +786,16 @@ class Batch:
         fits = np.zeros((len(self.engines), niters), dtype=np.float64)
         check(lib().adatm_batch_cpd(self.h, niters, C.byref(ms), _dptr(fits)), "batch_cpd")
         return ms.value, fits
+
+    def set_persistent(self, on: bool) -> None:
+        """one persistent launch per round (True) or one forked branch of
+        launches per instance (False, the default)"""
+        check(lib().adatm_batch_set_persistent(self.h, int(bool(on))), "batch_set_persistent")
+
+    @property
+    def persistent(self) -> bool:
+        """whether the last batched sweep ran in persistent mode"""
+        return bool(lib().adatm_batch_persistent(self.h))
 
     def sequential(self, niters: int) -> float:
         ms = C.c_double()

@@ -108,6 +108,48 @@ int adatm_k_fixup(hipStream_t s, int prec, adatm_kwork const *w,
 /* zero n bytes with a kernel (graph-capture safe) */
 int adatm_k_zero(hipStream_t s, void *p, size_t nbytes);
 
+/* ---- persistent launch of many MTTKRP launch sequences ----
+ * A stage is one launch of the kernels above (a reduction, a scatter, the two
+ * halves of a split-parent merge, or a zero fill), recorded with its
+ * arguments instead of launched.  adatm_k_persist runs every stage of many
+ * independent launch sequences in one launch: its workgroups take (stage,
+ * block) tasks from the queue of the XCD they run on, in queue order, and a
+ * task first waits until every block of its stage's `dep` stage is done.
+ * Stages that depend on each other are kept on one queue, so the hand-over
+ * stays inside one L2: the producer's stores complete (in that L2) before it
+ * counts its block done, and the consumer drops its L1 (an agent-scope
+ * acquire) before reading.  Every queue is
+ * ordered so that a stage's dependency precedes it, so waits end; a wait
+ * that still exceeds `spin` polls gives up and flags its queue in err[]. */
+enum { KS_ZERO = 0, KS_REDUCE = 1, KS_SCATTER = 2, KS_FIXCHUNK = 3, KS_FIXMULTI = 4 };
+#define ADATM_KS_NODEP 0xFFFFFFFFu
+#define ADATM_KS_ZERO_WORDS 4096u /* 32-bit words one zero-fill task clears */
+typedef struct {
+  uint32_t type, nblocks, dep, R, gpw;
+  uint32_t leaf, K, side;  /* reduction: leaf pass, levels fused, side outputs */
+  uint32_t d, zs;          /* scatter: level, scalar Z */
+  uint64_t nwords;         /* zero fill: 32-bit words at out */
+  adatm_kwork w;
+  adatm_kchain ch;
+  adatm_kscat sc;
+  uint32_t const *rowptr;
+  void const *in;
+  void *out, *carry, *carry2;
+} adatm_kstage;
+
+/* The persistent launch's layout (the narrow one: one lane per column) needs
+ * 8 <= R <= 32; adatm_k_persist_grid = its workgroups for the device. */
+int adatm_k_persist_grid(int prec, int device, uint32_t *grid);
+/* tasks[qoff[q] .. qoff[q+1]) = queue q, each (stage << 32 | block); xmap[x]
+ * = the queue of XCD x (x < 16; ~0: its workgroups take none); ctr =
+ * nq heads then one done counter per stage, all zero at launch; stat = nq
+ * error flags then nq served-task counts, accumulated over launches. */
+int adatm_k_persist(hipStream_t s, int prec, uint32_t grid, adatm_kstage const *stages,
+                    uint64_t const *tasks, uint32_t const *qoff, uint32_t const *xmap,
+                    uint32_t nq, uint32_t *ctr, uint32_t *stat, uint32_t spin);
+/* XCD ids (HW_REG_XCC_ID) seen by nblocks workgroups: out[b] for block b */
+int adatm_k_xcc_probe(hipStream_t s, uint32_t *out, uint32_t nblocks);
+
 /* Dense CP-ALS steps (FP64 arithmetic; factors/M in the path's precision).
  * G: N grams of R x R (FP64) contiguous.  part: scratch of
  * adatm_k_part_doubles(R) doubles. */

@@ -94,6 +94,24 @@ __device__ __forceinline__ void strow(T *p, T const (&o)[V])
     __builtin_memcpy(__builtin_assume_aligned(p, V * sizeof(T)), o, V * sizeof(T));
 }
 
+/* strow with the nontemporal hint (global_store_dwordx4 ... nt for a 16-byte
+ * piece): for rows written once and not read again by the writing pass, so
+ * that they do not push the factor rows the pass gathers out of L2 */
+template <typename T, int V>
+__device__ __forceinline__ void strow_nt(T *p, T const (&o)[V])
+{
+  if constexpr (V * sizeof(T) == 16) {
+    typedef T vec_t __attribute__((ext_vector_type(V)));
+    vec_t v;
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = o[i];
+    __builtin_nontemporal_store(v, (vec_t *)__builtin_assume_aligned(p, 16));
+  } else {
+#pragma unroll
+    for (int i = 0; i < V; ++i) __builtin_nontemporal_store(o[i], p + i);
+  }
+}
+
 /* FP32 -> BF16 with round-to-nearest-even.  Infinities keep their sign;
  * every NaN becomes a quiet NaN of its sign (its payload may sit in the low
  * 16 mantissa bits only, which truncation alone would turn into an

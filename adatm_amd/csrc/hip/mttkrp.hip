@@ -94,3 +94,57 @@ int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
 }
 
 } /* extern "C" */
+
+/* ---- persistent launch ------------------------------------------------------ */
+#define ADATM_PERSIST_DECL(NAME)                                                    \
+  hipError_t adatm_persist_##NAME(hipStream_t s, uint32_t grid, adatm_kstage const *st, \
+                                  uint64_t const *tasks, uint32_t const *qoff,     \
+                                  uint32_t const *xmap, uint32_t nq, uint32_t *ctr, \
+                                  uint32_t *stat, uint32_t spin);                  \
+  hipError_t adatm_persist_grid_##NAME(int device, uint32_t *grid);
+extern "C" {
+ADATM_PERSIST_DECL(fp64)
+ADATM_PERSIST_DECL(fp32)
+ADATM_PERSIST_DECL(bf16)
+}
+
+namespace {
+__global__ void k_xcc_probe(uint32_t *out)
+{
+  if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
+}
+} // namespace
+
+extern "C" {
+
+int adatm_k_persist_grid(int prec, int device, uint32_t *grid)
+{
+  hipError_t const e = (prec == KP_FP64)   ? adatm_persist_grid_fp64(device, grid)
+                       : (prec == KP_FP32) ? adatm_persist_grid_fp32(device, grid)
+                                           : adatm_persist_grid_bf16(device, grid);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int adatm_k_persist(hipStream_t s, int prec, uint32_t grid, adatm_kstage const *stages,
+                    uint64_t const *tasks, uint32_t const *qoff, uint32_t const *xmap,
+                    uint32_t nq, uint32_t *ctr, uint32_t *stat, uint32_t spin)
+{
+  if (grid == 0 || nq == 0 || nq > 16 || spin == 0) return (int)hipErrorInvalidValue;
+  hipError_t const e =
+      (prec == KP_FP64)
+          ? adatm_persist_fp64(s, grid, stages, tasks, qoff, xmap, nq, ctr, stat, spin)
+      : (prec == KP_FP32)
+          ? adatm_persist_fp32(s, grid, stages, tasks, qoff, xmap, nq, ctr, stat, spin)
+          : adatm_persist_bf16(s, grid, stages, tasks, qoff, xmap, nq, ctr, stat, spin);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int adatm_k_xcc_probe(hipStream_t s, uint32_t *out, uint32_t nblocks)
+{
+  if (nblocks == 0) return 0;
+  hipLaunchKernelGGL(k_xcc_probe, dim3(nblocks), dim3(64), 0, s, out);
+  hipError_t const e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+} /* extern "C" */

@@ -57,10 +57,10 @@ namespace {
  * R > 32, one group per wave, v_readlane into scalars). */
 template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V,
           bool SIDE>
-__global__ __launch_bounds__(256) void k_reduce(
-    uint32_t nitems, uint32_t const *__restrict__ istart,
+__device__ __forceinline__ void reduce_body(
+    uint32_t vb, uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
-    uint32_t const *__restrict__ ianc, adatm_kchain ch,
+    uint32_t const *__restrict__ ianc, adatm_kchain const &ch,
     T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
@@ -73,12 +73,12 @@ __global__ __launch_bounds__(256) void k_reduce(
   if constexpr (ONE) {
     if (lane >= G) return;
     grp = 0;
-    k = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) +
+    k = __builtin_amdgcn_readfirstlane(vb * (blockDim.x >> 6) +
                                        (threadIdx.x >> 6));
   } else {
     grp = lane / G;
     if (grp >= gpw) return;
-    k = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+    k = (vb * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   }
   uint32_t const gb = grp * G; /* the group's first lane */
   uint32_t const col = lane - gb;
@@ -213,7 +213,10 @@ __global__ __launch_bounds__(256) void k_reduce(
                 dst = (T *)ch.spart[j] + (size_t)stl[j] * R;
               else
                 dst = (T *)ch.sout[j] + (size_t)nd * R;
-              strow<T, V>(dst + cv, acc[j]);
+              /* memo rows are streamed out: nontemporal, so that they do
+                 not evict the gathered factor rows (-3% on a 95M-nonzero
+                 5-mode leaf-to-root pass) */
+              strow_nt<T, V>(dst + cv, acc[j]);
             }
           }
 #pragma unroll
@@ -272,6 +275,20 @@ __global__ __launch_bounds__(256) void k_reduce(
   }
 }
 
+/* one block of the reduction per workgroup of the launch */
+template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V,
+          bool SIDE>
+__global__ __launch_bounds__(256) void k_reduce(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ ianc, adatm_kchain ch,
+    T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
+    uint32_t R, uint32_t gpw)
+{
+  reduce_body<T, TU, LEAF, K, RB, DB, ONE, V, SIDE>(blockIdx.x, nitems, istart, iseg, islot,
+                                                    ianc, ch, xin, out, carry, R, gpw);
+}
+
 /* Reuse MTTKRP of level D: nodes pre-sorted by output row, taken in batches
  * of SB whose node ids and ancestor row ids are loaded one batch ahead; the
  * batch's Z rows and ancestor factor rows are then gathered together.  A
@@ -279,10 +296,10 @@ __global__ __launch_bounds__(256) void k_reduce(
 #define SB 8
 
 template <typename T, typename TU, int D, bool ZS, int V>
-__global__ __launch_bounds__(256) void k_scatter(
-    uint32_t nitems, uint32_t const *__restrict__ istart,
+__device__ __forceinline__ void scatter_body(
+    uint32_t vb, uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
-    uint32_t const *__restrict__ rowptr, adatm_kscat sc,
+    uint32_t const *__restrict__ rowptr, adatm_kscat const &sc,
     T const *__restrict__ Z, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
@@ -293,7 +310,7 @@ __global__ __launch_bounds__(256) void k_scatter(
   if (grp >= gpw) return;
   uint32_t const cv = (lane - grp * G) * V;
   uint32_t const k =
-      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+      (vb * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (k >= nitems) return;
   T const zero[V] = {};
   uint32_t const a = istart[k], b = istart[k + 1];
@@ -413,12 +430,24 @@ __global__ __launch_bounds__(256) void k_scatter(
   if (heavy) strow<T, V>(carry + ((size_t)slot * R + cv), acc);
 }
 
+template <typename T, typename TU, int D, bool ZS, int V>
+__global__ __launch_bounds__(256) void k_scatter(
+    uint32_t nitems, uint32_t const *__restrict__ istart,
+    uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
+    uint32_t const *__restrict__ rowptr, adatm_kscat sc,
+    T const *__restrict__ Z, T *__restrict__ out, T *__restrict__ carry,
+    uint32_t R, uint32_t gpw)
+{
+  scatter_body<T, TU, D, ZS, V>(blockIdx.x, nitems, istart, iseg, islot, rowptr, sc, Z,
+                                out, carry, R, gpw);
+}
+
 /* Chunk c of split parent h = cseg[c]: the sum of its <= ADATM_FIX_CHUNK
  * carry rows, eight independent partial sums combined in a fixed order; the
  * parent's output row when it has one chunk, else chunk row c of carry2. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_fix_chunk(
-    uint32_t nchunk, uint32_t const *__restrict__ cstart,
+__device__ __forceinline__ void fix_chunk_body(
+    uint32_t vb, uint32_t nchunk, uint32_t const *__restrict__ cstart,
     uint32_t const *__restrict__ cseg, uint32_t const *__restrict__ hslot,
     uint32_t const *__restrict__ hchunk, uint32_t const *__restrict__ hseg,
     T const *__restrict__ carry, T *__restrict__ carry2, T *__restrict__ out,
@@ -429,7 +458,7 @@ __global__ __launch_bounds__(256) void k_fix_chunk(
   if (grp >= gpw) return;
   uint32_t const col = lane - grp * R;
   uint32_t const c =
-      (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
+      (vb * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (c >= nchunk) return;
   uint32_t const h = cseg[c];
   uint32_t const lo = cstart[c];
@@ -453,17 +482,29 @@ __global__ __launch_bounds__(256) void k_fix_chunk(
     carry2[(size_t)c * R + col] = acc;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_fix_chunk(
+    uint32_t nchunk, uint32_t const *__restrict__ cstart,
+    uint32_t const *__restrict__ cseg, uint32_t const *__restrict__ hslot,
+    uint32_t const *__restrict__ hchunk, uint32_t const *__restrict__ hseg,
+    T const *__restrict__ carry, T *__restrict__ carry2, T *__restrict__ out,
+    uint32_t R, uint32_t gpw)
+{
+  fix_chunk_body<T>(blockIdx.x, nchunk, cstart, cseg, hslot, hchunk, hseg, carry, carry2,
+                    out, R, gpw);
+}
+
 /* Split parents with several chunks (one workgroup each): tree sum of their
  * chunk rows in a fixed order. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_fix_multi(uint32_t const *__restrict__ mlist,
-                                                   uint32_t const *__restrict__ hchunk,
-                                                   uint32_t const *__restrict__ hseg,
-                                                   T const *__restrict__ carry2,
-                                                   T *__restrict__ out, uint32_t R)
+__device__ __forceinline__ void fix_multi_body(uint32_t vb, uint32_t const *__restrict__ mlist,
+                                               uint32_t const *__restrict__ hchunk,
+                                               uint32_t const *__restrict__ hseg,
+                                               T const *__restrict__ carry2,
+                                               T *__restrict__ out, uint32_t R)
 {
   __shared__ T red[256];
-  uint32_t const h = mlist[blockIdx.x];
+  uint32_t const h = mlist[vb];
   uint32_t const rpi = 256u / R;
   uint32_t const sl = threadIdx.x / R;
   uint32_t const col = threadIdx.x - sl * R;
@@ -484,6 +525,16 @@ __global__ __launch_bounds__(256) void k_fix_multi(uint32_t const *__restrict__ 
     for (uint32_t q = 0; q < rpi; ++q) t += red[q * R + threadIdx.x];
     out[(size_t)hseg[h] * R + threadIdx.x] = t;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fix_multi(uint32_t const *__restrict__ mlist,
+                                                   uint32_t const *__restrict__ hchunk,
+                                                   uint32_t const *__restrict__ hseg,
+                                                   T const *__restrict__ carry2,
+                                                   T *__restrict__ out, uint32_t R)
+{
+  fix_multi_body<T>(blockIdx.x, mlist, hchunk, hseg, carry2, out, R);
 }
 
 /* ---- launchers ----------------------------------------------------------- */
@@ -669,6 +720,186 @@ hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr
   return hipSuccess;
 }
 
+
+/* ---- persistent launch (adatm_k_persist) ---------------------------------
+ * The stage bodies run in the narrow layout (one lane per column, 64/R items
+ * per wave) with the default batches of the single launches; every layout
+ * and batch gives bit-identical results, so a stage computes what its own
+ * launch would. */
+
+/* the stage descriptor at a wave-uniform address, so that its fields are
+   read by scalar loads */
+__device__ __forceinline__ adatm_kstage const *ps_uniform(adatm_kstage const *p)
+{
+  uint64_t const v = (uint64_t)(uintptr_t)p;
+  uint32_t const lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t const hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (adatm_kstage const *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <typename T, typename TU, bool LEAF, int K, bool SIDE>
+__device__ __forceinline__ void ps_reduce(adatm_kstage const *st, uint32_t vb)
+{
+  st = ps_uniform(st);
+  vb = __builtin_amdgcn_readfirstlane(vb);
+  constexpr int RB = (K <= 3) ? 8 : 4;
+  reduce_body<T, TU, LEAF, K, RB, false, false, 1, SIDE>(
+      vb, st->w.nitems, st->w.start, st->w.seg, st->w.slot, st->w.anc, st->ch,
+      (T const *)st->in, (T *)st->out, (T *)st->carry, st->R, st->gpw);
+}
+
+template <typename T, typename TU, int D, bool ZS>
+__device__ __forceinline__ void ps_scatter(adatm_kstage const *st, uint32_t vb)
+{
+  st = ps_uniform(st);
+  vb = __builtin_amdgcn_readfirstlane(vb);
+  scatter_body<T, TU, D, ZS, 1>(vb, st->w.nitems, st->w.start, st->w.seg, st->w.slot,
+                                st->rowptr, st->sc, (T const *)st->in, (T *)st->out,
+                                (T *)st->carry, st->R, st->gpw);
+}
+
+template <typename T>
+__device__ __forceinline__ void ps_fix_chunk(adatm_kstage const *st, uint32_t vb)
+{
+  st = ps_uniform(st);
+  vb = __builtin_amdgcn_readfirstlane(vb);
+  fix_chunk_body<T>(vb, st->w.nchunk, st->w.cstart, st->w.cseg, st->w.hslot, st->w.hchunk,
+                    st->w.hseg, (T const *)st->carry, (T *)st->carry2, (T *)st->out, st->R,
+                    st->gpw);
+}
+
+template <typename T>
+__device__ __forceinline__ void ps_fix_multi(adatm_kstage const *st, uint32_t vb)
+{
+  st = ps_uniform(st);
+  vb = __builtin_amdgcn_readfirstlane(vb);
+  fix_multi_body<T>(vb, st->w.mlist, st->w.hchunk, st->w.hseg, (T const *)st->carry2,
+                    (T *)st->out, st->R);
+}
+
+__device__ __forceinline__ void ps_zero(adatm_kstage const *st, uint32_t vb)
+{
+  st = ps_uniform(st);
+  uint32_t *p = (uint32_t *)st->out;
+  uint64_t const base = (uint64_t)__builtin_amdgcn_readfirstlane(vb) * ADATM_KS_ZERO_WORDS;
+  uint64_t const n = st->nwords;
+  for (uint32_t i = threadIdx.x; i < ADATM_KS_ZERO_WORDS; i += 256u)
+    if (base + i < n) p[base + i] = 0u;
+}
+
+template <typename T, typename TU, bool LEAF, bool SIDE>
+__device__ __forceinline__ void ps_reduce_k(adatm_kstage const *st, uint32_t K, uint32_t vb)
+{
+  switch (K) {
+  case 1:
+    if constexpr (!SIDE) ps_reduce<T, TU, LEAF, 1, SIDE>(st, vb);
+    break;
+  case 2: ps_reduce<T, TU, LEAF, 2, SIDE>(st, vb); break;
+  case 3: ps_reduce<T, TU, LEAF, 3, SIDE>(st, vb); break;
+  case 4: ps_reduce<T, TU, LEAF, 4, SIDE>(st, vb); break;
+  default: ps_reduce<T, TU, LEAF, 5, SIDE>(st, vb); break;
+  }
+}
+
+template <typename T, typename TU, bool ZS>
+__device__ __forceinline__ void ps_scatter_d(adatm_kstage const *st, uint32_t d, uint32_t vb)
+{
+  switch (d) {
+  case 1: ps_scatter<T, TU, 1, ZS>(st, vb); break;
+  case 2: ps_scatter<T, TU, 2, ZS>(st, vb); break;
+  case 3: ps_scatter<T, TU, 3, ZS>(st, vb); break;
+  case 4: ps_scatter<T, TU, 4, ZS>(st, vb); break;
+  default: ps_scatter<T, TU, 5, ZS>(st, vb); break;
+  }
+}
+
+/* XCD (L2 domain) of the calling wave: HW_REG_XCC_ID bits 3:0 */
+__device__ __forceinline__ uint32_t xcc_id()
+{
+  return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 15u;
+}
+
+template <typename T, typename TU>
+__global__ __launch_bounds__(256) void k_persist(
+    adatm_kstage const *__restrict__ stages, uint64_t const *__restrict__ tasks,
+    uint32_t const *__restrict__ qoff, uint32_t const *__restrict__ xmap, uint32_t nq,
+    uint32_t *ctr, uint32_t *stat, uint32_t spin)
+{
+  __shared__ uint32_t s_t;
+  uint32_t const q = xmap[xcc_id()];
+  if (q >= nq) return; /* an XCD without a queue: the whole workgroup leaves */
+  uint32_t const q0 = qoff[q], qn = qoff[q + 1] - q0;
+  uint32_t *const head = ctr + q;
+  uint32_t *const done = ctr + nq;
+  uint32_t served = 0;
+  uint32_t seen = ADATM_KS_NODEP; /* the last dependency this workgroup acquired */
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(head, 1u);
+    __syncthreads();
+    uint32_t const t = __builtin_amdgcn_readfirstlane(s_t);
+    if (t >= qn) break;
+    uint64_t const task = tasks[q0 + t];
+    uint32_t const si = __builtin_amdgcn_readfirstlane((uint32_t)(task >> 32));
+    uint32_t const vb = __builtin_amdgcn_readfirstlane((uint32_t)task);
+    adatm_kstage const *const st = stages + si;
+    uint32_t const dep = st->dep;
+    /* a run of tasks with one dependency waits and acquires once: nothing
+       rewrites that stage's output while its successors run */
+    if (dep != ADATM_KS_NODEP && dep != seen) {
+      seen = dep;
+      /* every block of the dependency done: the counter is polled in L2
+         (sc1 loads), a bounded number of times */
+      if (threadIdx.x == 0) {
+        uint32_t const need = stages[dep].nblocks;
+        uint32_t n = 0;
+        while (__hip_atomic_load(done + dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++n >= spin) {
+            atomicOr(stat + q, 1u);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      /* agent-scope acquire (buffer_inv sc1): drops this CU's L1, so that
+         rows of buffers a chain rewrites (carry, ping-pong temporaries)
+         come from L2, where the producer's stores completed; the
+         workgroup-scope invalidate (sc0) leaves the L1 as it is */
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    switch (st->type) {
+    case KS_REDUCE:
+      if (st->leaf) {
+        if (st->side)
+          ps_reduce_k<T, TU, true, true>(st, st->K, vb);
+        else
+          ps_reduce_k<T, TU, true, false>(st, st->K, vb);
+      } else {
+        ps_reduce_k<T, TU, false, false>(st, st->K, vb);
+      }
+      break;
+    case KS_SCATTER:
+      if (st->zs)
+        ps_scatter_d<T, TU, true>(st, st->d, vb);
+      else
+        ps_scatter_d<T, TU, false>(st, st->d, vb);
+      break;
+    case KS_FIXCHUNK: ps_fix_chunk<T>(st, vb); break;
+    case KS_FIXMULTI: ps_fix_multi<T>(st, vb); break;
+    default: ps_zero(st, vb); break;
+    }
+    /* this wave's stores have reached L2 before the block is counted */
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_fetch_add(done + si, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ++served;
+    }
+  }
+  if (threadIdx.x == 0 && served) atomicAdd(stat + nq + q, served);
+}
+
 } // namespace
 
 /* the launchers of one value type (T = value/accumulation, TU = factor rows) */
@@ -692,4 +923,24 @@ hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr
   {                                                                                \
     return scatter_d<T, TU>(s, w, rowptr, sc, Z, zs, out, carry,     \
                                           carry2, R);                              \
+  }                                                                                \
+  extern "C" hipError_t adatm_persist_##NAME(                                       \
+      hipStream_t s, uint32_t grid, adatm_kstage const *st, uint64_t const *tasks,  \
+      uint32_t const *qoff, uint32_t const *xmap, uint32_t nq, uint32_t *ctr,       \
+      uint32_t *stat, uint32_t spin)                                                \
+  {                                                                                \
+    hipLaunchKernelGGL((k_persist<T, TU>), dim3(grid), dim3(256), 0, s, st, tasks,  \
+                       qoff, xmap, nq, ctr, stat, spin);                           \
+    return hipGetLastError();                                                      \
+  }                                                                                \
+  extern "C" hipError_t adatm_persist_grid_##NAME(int device, uint32_t *grid)      \
+  {                                                                                \
+    int nb = 0, ncu = 0;                                                           \
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                   \
+        &nb, k_persist<T, TU>, 256, 0);                                            \
+    if (e == hipSuccess)                                                           \
+      e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,       \
+                                device);                                           \
+    *grid = (uint32_t)(nb > 0 ? nb : 1) * (uint32_t)(ncu > 0 ? ncu : 1);           \
+    return e;                                                                      \
   }

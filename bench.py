@@ -137,6 +137,8 @@ def main() -> int:
                          "for every engine (no autotuning)")
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="batched workload: forked branches only (no persistent launch)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched workload: skip timing the instances one after another")
     ap.add_argument("--hw-queues", type=int, default=0,
@@ -398,7 +400,9 @@ def c5_spec(k: int):
 def batched_c5(args) -> int:
     """64 uneven CP-ALS instances: analytic plans, autotuned tiles per
     instance, FP32 with FP64 solves, every instance's sweep in one hipGraph
-    (one launch per batched sweep), 50 sweeps; against the same engines run
+    (one launch per batched sweep) whose rounds run all instances' MTTKRPs
+    in one persistent kernel launch each (the value), beside the same batch
+    with one forked branch of launches per instance, the same engines run
     one instance after another, and FP64 fits of the same instances."""
     if args.hw_queues:
         # read by the HIP runtime when it starts: set before the library loads
@@ -429,10 +433,23 @@ def batched_c5(args) -> int:
     log(f"C5: {n} instances, {nnz_total} nnz in total, built and tuned in "
         f"{time.time() - t0:.1f}s")
     b = A.Batch(eng)
-    b.cpd(2)                                   # warm-up (graph capture)
-    for k, e in enumerate(eng):                # restart from the seeded factors
-        e.init_factors(c5_spec(k)[0].seed + 1)
-    ms, fits = b.cpd(iters)
+
+    def run(persistent):
+        b.set_persistent(persistent)
+        b.cpd(2)                               # warm-up (graph capture)
+        for k, e in enumerate(eng):            # restart from the seeded factors
+            e.init_factors(c5_spec(k)[0].seed + 1)
+        return b.cpd(iters), b.persistent
+
+    br_ms = br_same = None
+    if not args.no_persistent:
+        # the forked branches first, then the persistent launch (the value)
+        (br_ms, br_fits), _ = run(False)
+    (ms, fits), persistent = run(not args.no_persistent)
+    if br_ms is not None:
+        br_same = bool(np.array_equal(fits, br_fits))
+        log(f"C5: persistent {ms / iters:.3f} ms/sweep (fits equal to the forked "
+            f"branches': {br_same}), forked branches {br_ms / iters:.3f} ms/sweep")
     seq_ms = None if args.no_sequential else b.sequential(2) / 2.0
     dfit = None
     if e64:
@@ -452,6 +469,9 @@ def batched_c5(args) -> int:
                    "nnz_total": nnz_total,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "tiles": [list(x) for x in tiles]},
+        "batched_mode": "persistent" if persistent else "forked branches",
+        "forked_branches_ms_per_sweep": round(br_ms / iters, 4) if br_ms else None,
+        "persistent_fits_equal_forked": br_same,
         "sequential_ms_per_sweep": round(seq_ms, 4) if seq_ms else None,
         "batching_speedup": round(seq_ms / per, 3) if seq_ms else None,
         "aggregate_mttkrp_GBps": round(seq_bytes / (per * 1e-3) / 1e9, 1),

@@ -405,6 +405,20 @@ void adatm_batch_destroy(adatm_batch *b);
 int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *fits);
 /* each instance's niters sweeps (its own graph), one instance after another */
 int adatm_batch_sequential(adatm_batch *b, uint32_t niters, double *ms_total);
+/* Persistent mode (on = 1; it applies to batches of one precision with
+ * 8 <= R <= 32 and at most 6 modes per instance): each round's MTTKRPs of
+ * all instances run in ONE persistent kernel launch, whose workgroups take
+ * the recorded launches' blocks from per-XCD queues in dependency order,
+ * instead of one forked branch of launches per instance; the results are
+ * bit-identical.  The forked branches (on = 0) are the default: on 64
+ * uneven instances they measured about 10% faster.  adatm_batch_persistent:
+ * whether the last built batched sweep ran in persistent mode. */
+int adatm_batch_set_persistent(adatm_batch *b, int on);
+int adatm_batch_persistent(adatm_batch const *b);
+/* the persistent launch's queues (one per XCD id the device reported),
+ * xmap[16] = queue of each XCD id (~0: none) and its workgroups */
+int adatm_batch_persist_info(adatm_batch const *b, uint32_t *nq, uint32_t *xmap,
+                             uint32_t *grid);
 
 /* Last device error string (thread-local), "" when none. */
 char const *adatm_last_error(void);

@@ -282,6 +282,56 @@ def test_batched_cpd_matches_each_instance(precs):
     b.close()
 
 
+@pytest.mark.parametrize("prec", [A.FP32, A.FP64])
+def test_persistent_batch_equals_forked_branches(prec):
+    """Each round's MTTKRPs of every instance in one persistent launch (the
+    recorded launches' blocks taken from per-XCD queues in dependency order):
+    fits, factors and lambdas bit-identical to the forked-branch batch, over
+    two runs of the same captured graph; Zipf instances bring split parents
+    (merge stages) and empty root slices (zero-fill stages)."""
+    print(f"persistent batch precision={prec}", flush=True)
+    specs = [([60, 50, 40, 30], 8000, A.DIST_UNIFORM, 8),
+             ([300, 70, 60, 50, 40], 30000, A.DIST_ZIPF, 16),
+             ([40, 35, 30, 25, 20, 15], 6000, A.DIST_UNIFORM, 32),
+             ([900, 90, 80, 70], 40000, A.DIST_ZIPF, 16),
+             ([500, 400, 300, 200, 100, 50], 60000, A.DIST_ZIPF, 32),
+             ([2000, 30, 20, 10], 5000, A.DIST_ZIPF, 24)]
+    sets = []
+    for _ in range(2):
+        eng = []
+        for k, (dims, nd, dist, R) in enumerate(specs):
+            t = _tensor(dims, nd, dist, seed=90 + k)
+            cands, ch = A.plan_analytic(t, R)
+            e = A.Engine(t, R, cands[ch], precision=prec)
+            e.init_factors(17 + k)
+            eng.append(e)
+        sets.append(eng)
+    bp, bf = A.Batch(sets[0]), A.Batch(sets[1])
+    bp.set_persistent(True)
+    for n in (4, 3):
+        _, fp = bp.cpd(n)
+        _, ff = bf.cpd(n)
+        assert bp.persistent and not bf.persistent
+        assert np.array_equal(fp, ff), (fp, ff)
+        for k, (ep, ef) in enumerate(zip(*sets)):
+            assert np.array_equal(ep.get_lambda(), ef.get_lambda()), k
+            for m in range(len(specs[k][0])):
+                assert np.array_equal(ep.get_factor(m), ef.get_factor(m)), (k, m)
+    bp.close()
+    bf.close()
+    # outside its range (R > 32) the batch keeps the forked branches
+    t = _tensor([40, 30, 20, 10], 4000, A.DIST_UNIFORM, seed=99)
+    cands, ch = A.plan_analytic(t, 40)
+    e = A.Engine(t, 40, cands[ch], precision=prec)
+    e.init_factors(5)
+    b = A.Batch([e])
+    with pytest.raises(A.AdatmError):
+        b.set_persistent(True)
+    b.cpd(1)
+    assert not b.persistent
+    b.close()
+
+
 def test_device_fiber_counts_match_host():
     """Radix sort + boundary scan on the device give the host's (and the
     oracle's) fiber counts, including keys that need three 8-bit digits."""

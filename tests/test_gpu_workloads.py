@@ -125,22 +125,30 @@ def test_c5_first_instances_batched_against_the_oracle():
     import bench
     n, iters = 8, 50
     print(f"C5: first {n} instances, {iters} batched sweeps", flush=True)
-    eng, ofits = [], []
+    eng, ofits, Us = [], [], []
     for k in range(n):
         spec, R = bench.c5_spec(k)
         t = A.Tensor.synth(spec)
         t.canonicalize()
         cands, ch = A.plan_analytic(t, R, device=0)
         U = _factors(t, R, seed=spec.seed + 1)
+        Us.append(U)
         eng.append(_engine(t, R, cands[ch], A.FP32, U))
         seq = corc.OracleSeq(t.inds().astype(np.int64), t.vals, t.dims,
                              cands[ch].groups(t.nmodes), R)
         ofits.append(seq.cpd(U, iters)[0])
         seq.close()
     b = A.Batch(eng)
-    ms, fits = b.cpd(iters)
-    assert ms > 0
-    for k in range(n):
-        d = np.max(np.abs(fits[k] - ofits[k]))
-        assert d <= FIT_TOL[A.FP32], (k, d)
+    # both batched forms: one persistent launch per round, forked branches
+    for persistent in (True, False):
+        print(f"C5 batch persistent={persistent}", flush=True)
+        for e, U in zip(eng, Us):
+            for m, Um in enumerate(U):
+                e.set_factor(m, Um)
+        b.set_persistent(persistent)
+        ms, fits = b.cpd(iters)
+        assert ms > 0 and b.persistent == persistent
+        for k in range(n):
+            d = np.max(np.abs(fits[k] - ofits[k]))
+            assert d <= FIT_TOL[A.FP32], (persistent, k, d)
     b.close()
