@@ -232,18 +232,27 @@ def main() -> int:
         if args.warmup > 0:
             e.run_timed(1, args.warmup, graph=True)
         ms = e.run_timed(1, args.steps, graph=True)
-        fits = e.fits(args.steps)
         seq_ms, _ = e.time(0, 1, 5, per_kernel=False)
         _, pk = e.time(1, 0, 1, graph=False)
         launch_log = e.launches()
         F, seq_bytes, seq_flops, dev_bytes = e.stats()
-        res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=float(fits[-1]), tiles=list(tiles),
+        # the fit of a CP-ALS run of `steps` sweeps from the seeded factors and
+        # the worst status of its solves (0 clean, 1 with a ridge, 2 failed)
+        e.init_factors(spec.seed + 1)
+        try:
+            cfit = float(e.cpd(args.steps, tol=0.0, graph=True)[-1])
+        except A.AdatmError as exc:
+            log(f"{name}: CP-ALS from the seeded factors: {exc}")
+            cfit = float("nan")
+        status = e.solve_status()
+        res[name] = dict(ms_per_sweep=ms / args.steps, fit_last=cfit, solve_status=status,
+                         tiles=list(tiles),
                          root_pass=root_pass_name(tiles[3], pl.n_grp),
                          seq_ms=seq_ms, pk=pk, seq_bytes=seq_bytes,
                          seq_flops=seq_flops, dev_bytes=dev_bytes,
                          nfibs=F.tolist(), launches=launch_log)
         log(f"{name}: {ms / args.steps:.3f} ms/sweep, MTTKRP seq {seq_ms:.3f} ms, "
-            f"fit {fits[-1]:.6f}, device {dev_bytes / 2**30:.1f} GiB")
+            f"fit {cfit:.6g} (solve status {status}), device {dev_bytes / 2**30:.1f} GiB")
         # MTTKRP of every mode from the same factors, to compare precisions
         e.init_factors(spec.seed + 1)
         Mref[name] = [e.mttkrp(m) for m in range(t.nmodes)]
@@ -326,6 +335,7 @@ def main() -> int:
         "mttkrp_GBps": round(mttkrp_bytes / (mttkrp_ms * 1e-3) / 1e9, 1),
         "mttkrp_hbm_frac": round(mttkrp_bytes / (mttkrp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "fit_learned_fp32": r32["fit_last"],
+        "solve_status": {k: v["solve_status"] for k, v in res.items()},
         "fit_plain_fp64": r64["fit_last"] if r64 else None,
         "mttkrp_rel_err_fp32_vs_fp64": errs,
         "splatt_csf_convert_s": round(convert_s, 3) if convert_s is not None else None,
