@@ -173,6 +173,8 @@ def lib() -> C.CDLL:
         "adatm_batch_cpd": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "adatm_batch_sequential": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_batch_set_persistent": (C.c_int, [C.c_void_p, C.c_int]),
+        "adatm_persist_plan": (C.c_int, [C.c_uint32, P(C.c_uint32), P(C.c_double), C.c_uint32,
+                                         P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "adatm_batch_persistent": (C.c_int, [C.c_void_p]),
         "adatm_mlp_nparams": (C.c_size_t, []),
         "adatm_mlp_create": (C.c_int, [C.c_int, C.c_uint64, P(C.c_void_p)]),
@@ -767,6 +769,23 @@ class Engine:
         vals = np.zeros(nnz, dtype=np.float64)
         check(lib().adatm_engine_get_csf(self.h, g, pf, pi, _dptr(vals)), "get_csf")
         return F, fptr, fids, vals
+
+
+def persist_plan(lens, costs, nq: int):
+    """Queue plan of one round of the persistent batched form
+    (adatm_persist_plan): chains of lens[i] stages with costs stored chain
+    after chain -> (queue of each chain, stage order, start of each queue)."""
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    costs = np.ascontiguousarray(costs, dtype=np.float64)
+    n = len(lens)
+    queue = np.zeros(max(n, 1), dtype=np.uint32)
+    order = np.zeros(max(int(lens.sum()), 1), dtype=np.uint32)
+    qstart = np.zeros(nq + 1, dtype=np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    check(lib().adatm_persist_plan(n, lens.ctypes.data_as(u32), _dptr(costs), nq,
+                                   queue.ctypes.data_as(u32), order.ctypes.data_as(u32),
+                                   qstart.ctypes.data_as(u32)), "persist_plan")
+    return queue[:n], order[: int(lens.sum())], qstart
 
 
 class Batch:

@@ -2440,6 +2440,69 @@ static int batch_branch(adatm_batch *b, int i, int mode)
 
 #define PERSIST_SPIN (1u << 22) /* polls of one dependency wait (s_sleep 2 each) */
 
+int adatm_persist_plan(uint32_t n, uint32_t const *len, double const *cost, uint32_t nq,
+                       uint32_t *queue, uint32_t *order, uint32_t *qstart)
+{
+  if ((n && (!len || !cost || !queue)) || nq == 0 || !order || !qstart)
+    return SPLATT_ERROR_BADINPUT;
+  uint32_t *first = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  uint32_t *ord = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  double *work = (double *)malloc((n ? n : 1) * sizeof(double));
+  double *load = (double *)malloc(nq * sizeof(double));
+  if (!first || !ord || !work || !load) {
+    free(first);
+    free(ord);
+    free(work);
+    free(load);
+    return SPLATT_ERROR_NOMEMORY;
+  }
+  uint32_t tot = 0, maxlen = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    first[i] = tot;
+    work[i] = 0.0;
+    for (uint32_t d = 0; d < len[i]; ++d) work[i] += cost[tot + d];
+    tot += len[i];
+    if (len[i] > maxlen) maxlen = len[i];
+    ord[i] = i;
+  }
+  /* largest chains first (ties: lower index), each onto the least-loaded
+     queue (ties: lower queue) */
+  for (uint32_t i = 1; i < n; ++i) {
+    uint32_t const v = ord[i];
+    uint32_t j = i;
+    while (j > 0 && work[ord[j - 1]] < work[v]) {
+      ord[j] = ord[j - 1];
+      --j;
+    }
+    ord[j] = v;
+  }
+  for (uint32_t q = 0; q < nq; ++q) load[q] = 0.0;
+  for (uint32_t t = 0; t < n; ++t) {
+    uint32_t const i = ord[t];
+    uint32_t best = 0;
+    for (uint32_t q = 1; q < nq; ++q)
+      if (load[q] < load[best]) best = q;
+    queue[i] = best;
+    load[best] += work[i];
+  }
+  /* queue by queue, depth by depth, the chains of a depth in that order */
+  uint32_t m = 0;
+  for (uint32_t q = 0; q < nq; ++q) {
+    qstart[q] = m;
+    for (uint32_t d = 0; d < maxlen; ++d)
+      for (uint32_t t = 0; t < n; ++t) {
+        uint32_t const i = ord[t];
+        if (queue[i] == q && d < len[i]) order[m++] = first[i] + d;
+      }
+  }
+  qstart[nq] = m;
+  free(first);
+  free(ord);
+  free(work);
+  free(load);
+  return SPLATT_SUCCESS;
+}
+
 /* the XCD ids the device's workgroups report: xmap[id] = queue (~0: none) */
 static int batch_probe_xcc(adatm_batch *b)
 {
@@ -2491,14 +2554,11 @@ static int batch_persist_build(adatm_batch *b)
   uint32_t *first = (uint32_t *)malloc(n * sizeof(uint32_t));
   uint32_t *len = (uint32_t *)malloc(n * sizeof(uint32_t));
   uint32_t *queue = (uint32_t *)malloc(n * sizeof(uint32_t));
-  double *work = (double *)malloc(n * sizeof(double));
-  double *load = (double *)malloc(nq * sizeof(double));
-  uint32_t *ord = (uint32_t *)malloc(n * sizeof(uint32_t));
   b->p_stage0 = (uint32_t *)calloc(nr, sizeof(uint32_t));
   b->p_nstage = (uint32_t *)calloc(nr, sizeof(uint32_t));
   b->p_task0 = (uint32_t *)calloc(nr, sizeof(uint32_t));
   b->p_served = (uint64_t *)calloc(nq, sizeof(uint64_t));
-  if (!qoff || !first || !len || !queue || !work || !load || !ord || !b->p_stage0 ||
+  if (!qoff || !first || !len || !queue || !b->p_stage0 ||
       !b->p_nstage || !b->p_task0 || !b->p_served)
     rc = SPLATT_ERROR_NOMEMORY;
   b->ctr_words = 0;
@@ -2511,7 +2571,6 @@ static int batch_persist_build(adatm_batch *b)
       adatm_engine *e = b->e[i];
       first[i] = all.n;
       len[i] = 0;
-      work[i] = 0;
       if (e->N <= r) continue;
       e->rec = &all;
       rc = mttkrp_dev(e, r);
@@ -2524,71 +2583,25 @@ static int batch_persist_build(adatm_batch *b)
           rc = SPLATT_ERROR_BADINPUT;
         }
         x->dep = (k == first[i]) ? ADATM_KS_NODEP : k - 1 - s0; /* round-local */
-        work[i] += all.cost[k];
       }
     }
     if (rc != SPLATT_SUCCESS) break;
     b->p_nstage[r] = all.n - s0;
-    /* largest chains first, each onto the least-loaded queue */
-    for (uint32_t i = 0; i < n; ++i) ord[i] = i;
-    for (uint32_t i = 1; i < n; ++i) { /* insertion sort: work desc, index asc */
-      uint32_t const v = ord[i];
-      uint32_t j = i;
-      while (j > 0 && work[ord[j - 1]] < work[v]) {
-        ord[j] = ord[j - 1];
-        --j;
-      }
-      ord[j] = v;
-    }
-    for (uint32_t q = 0; q < nq; ++q) load[q] = 0;
-    for (uint32_t t = 0; t < n; ++t) {
-      uint32_t const i = ord[t];
-      uint32_t best = 0;
-      for (uint32_t q = 1; q < nq; ++q)
-        if (load[q] < load[best]) best = q;
-      queue[i] = best;
-      load[best] += work[i];
-    }
-    /* tasks, queue by queue, depth by depth (every chain's first stage,
-       then every second stage, ...), the chains of a depth largest first,
-       so a stage always follows the one it waits for.  (Ordering by the
-       share of its chain's cost that precedes a stage was measured slower
-       on the 64-instance workload: 5.70 against 5.52 ms per sweep.) */
+    /* chains onto queues and each queue's stage order (adatm_persist_plan:
+       depth by depth, so a stage always follows the one it waits for;
+       ordering by the share of its chain's cost before a stage measured
+       slower on the 64-instance workload, 5.70 against 5.52 ms per sweep) */
     uint32_t const ns_r = all.n - s0;
     uint32_t *sord = (uint32_t *)malloc((ns_r ? ns_r : 1) * sizeof(uint32_t));
-    uint32_t *sdep = (uint32_t *)malloc((ns_r ? ns_r : 1) * sizeof(uint32_t));
-    uint32_t *sins = (uint32_t *)malloc((ns_r ? ns_r : 1) * sizeof(uint32_t));
-    if (!sord || !sdep || !sins) rc = SPLATT_ERROR_NOMEMORY;
-    for (uint32_t i = 0; rc == SPLATT_SUCCESS && i < n; ++i)
-      for (uint32_t dd = 0; dd < len[i]; ++dd) {
-        uint32_t const k = first[i] + dd - s0;
-        sdep[k] = dd;
-        sins[k] = i;
-      }
-    uint32_t *rank = (uint32_t *)malloc(n * sizeof(uint32_t)); /* position in ord */
-    if (!rank) rc = SPLATT_ERROR_NOMEMORY;
-    for (uint32_t t = 0; rc == SPLATT_SUCCESS && t < n; ++t) rank[ord[t]] = t;
+    uint32_t *qst = (uint32_t *)malloc((nq + 1) * sizeof(uint32_t));
+    if (!sord || !qst) rc = SPLATT_ERROR_NOMEMORY;
+    if (rc == SPLATT_SUCCESS)
+      rc = adatm_persist_plan(n, len, all.cost + s0, nq, queue, sord, qst);
     uint32_t *qo = qoff + (size_t)r * (nq + 1);
     uint64_t const t0 = ntask;
     for (uint32_t q = 0; rc == SPLATT_SUCCESS && q < nq; ++q) {
       qo[q] = (uint32_t)(ntask - t0);
-      uint32_t m = 0;
-      for (uint32_t k = 0; k < ns_r; ++k)
-        if (queue[sins[k]] == q) sord[m++] = k;
-      for (uint32_t a = 1; a < m; ++a) { /* insertion sort by (depth, chain) */
-        uint32_t const v = sord[a];
-        uint32_t j = a;
-        while (j > 0) {
-          uint32_t const u = sord[j - 1];
-          int const after = sdep[u] > sdep[v] ||
-                            (sdep[u] == sdep[v] && rank[sins[u]] > rank[sins[v]]);
-          if (!after) break;
-          sord[j] = u;
-          --j;
-        }
-        sord[j] = v;
-      }
-      for (uint32_t a = 0; rc == SPLATT_SUCCESS && a < m; ++a) {
+      for (uint32_t a = qst[q]; rc == SPLATT_SUCCESS && a < qst[q + 1]; ++a) {
         adatm_kstage const *x = all.st + s0 + sord[a];
         if (ntask + x->nblocks > task_cap) {
           uint64_t cap = task_cap ? 2 * task_cap : 4096;
@@ -2608,9 +2621,7 @@ static int batch_persist_build(adatm_batch *b)
     }
     qo[nq] = (uint32_t)(ntask - t0);
     free(sord);
-    free(sdep);
-    free(sins);
-    free(rank);
+    free(qst);
     if (ntask > 0xFFFFFFFFull) {
       adatm_set_error("persistent launch: too many tasks");
       rc = SPLATT_ERROR_BADINPUT;
@@ -2655,9 +2666,6 @@ static int batch_persist_build(adatm_batch *b)
   free(first);
   free(len);
   free(queue);
-  free(work);
-  free(load);
-  free(ord);
   b->spin = PERSIST_SPIN;
   return rc;
 }

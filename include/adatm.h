@@ -415,6 +415,16 @@ int adatm_batch_sequential(adatm_batch *b, uint32_t niters, double *ms_total);
  * whether the last built batched sweep ran in persistent mode. */
 int adatm_batch_set_persistent(adatm_batch *b, int on);
 int adatm_batch_persistent(adatm_batch const *b);
+/* The queue plan of one round of the persistent form: n chains (an
+ * instance's recorded launches), chain i of len[i] stages whose costs are
+ * stored chain after chain in cost[]; queue[i] = the queue of chain i
+ * (largest total cost first onto the least-loaded of nq queues; ties to the
+ * lower chain, then the lower queue); order[qstart[q] .. qstart[q+1]) = the
+ * stages (indices into the flat arrays) of queue q, depth by depth, the
+ * chains of a depth in the order they were assigned, so every stage follows
+ * its chain's previous stage in its queue. */
+int adatm_persist_plan(uint32_t n, uint32_t const *len, double const *cost, uint32_t nq,
+                       uint32_t *queue, uint32_t *order, uint32_t *qstart);
 /* the persistent launch's queues (one per XCD id the device reported),
  * xmap[16] = queue of each XCD id (~0: none) and its workgroups */
 int adatm_batch_persist_info(adatm_batch const *b, uint32_t *nq, uint32_t *xmap,

@@ -215,3 +215,45 @@ def test_stepwise_cpd_refuses_bad_arguments(lib):
     assert L.adatm_engine_cpd_begin(None) == A.SPLATT_ERROR_BADINPUT
     st = C.c_int()
     assert L.adatm_engine_solve_status(None, C.byref(st)) == A.SPLATT_ERROR_BADINPUT
+
+
+def test_persist_plan_balances_chains_and_orders_dependencies():
+    """The persistent batched form's queue plan: chains by total cost,
+    largest first, onto the least-loaded queue; each queue's stages depth by
+    depth with the chains of a depth in assignment order, so a stage always
+    follows its chain's previous stage in the same queue."""
+    # five one-stage chains of cost 10, 9, 8, 1, 1 on two queues (hand-traced):
+    # 10 -> q0, 9 -> q1, 8 -> q1 (9 < 10), 1 -> q0 (10 < 17), 1 -> q0 (11 < 17)
+    queue, order, qstart = A.persist_plan([1, 1, 1, 1, 1], [10, 9, 8, 1, 1], 2)
+    assert queue.tolist() == [0, 1, 1, 0, 0]
+    assert qstart.tolist() == [0, 3, 5]
+    assert order.tolist() == [0, 3, 4, 1, 2]
+    # chain 0 (3 stages, cost 6) and chain 1 (1 stage, cost 2) on one queue:
+    # depth 0 of both (larger chain first), then depths 1 and 2 of chain 0
+    queue, order, qstart = A.persist_plan([3, 1], [1, 2, 3, 2], 1)
+    assert queue.tolist() == [0, 0] and order.tolist() == [0, 3, 1, 2]
+    # empty chains take no stage; equal costs go to the lower queue first
+    queue, order, qstart = A.persist_plan([0, 2, 0, 2], [1, 1, 1, 1], 3)
+    assert queue.tolist()[1] == 0 and queue.tolist()[3] == 1
+    assert qstart.tolist() == [0, 2, 4, 4]
+    # random batches: every stage once, chains whole on one queue, each
+    # stage after its chain's previous one, loads within one chain of even
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        n, nq = int(rng.integers(1, 40)), int(rng.integers(1, 9))
+        lens = rng.integers(0, 9, n)
+        costs = rng.uniform(0.1, 10.0, int(lens.sum()))
+        queue, order, qstart = A.persist_plan(lens, costs, nq)
+        first = np.concatenate([[0], np.cumsum(lens)[:-1]])
+        chain = np.repeat(np.arange(n), lens)
+        assert sorted(order.tolist()) == list(range(int(lens.sum())))
+        pos = np.empty(len(order), dtype=np.int64)
+        pos[order] = np.arange(len(order))
+        for q in range(nq):
+            for a in order[qstart[q]:qstart[q + 1]]:
+                assert queue[chain[a]] == q
+                if a > first[chain[a]]:
+                    assert pos[a - 1] < pos[a]
+        work = np.array([costs[first[i]:first[i] + lens[i]].sum() for i in range(n)])
+        load = np.bincount(queue, weights=work, minlength=nq)
+        assert load.max() - load.min() <= work.max() + 1e-9
