@@ -173,6 +173,8 @@ def lib() -> C.CDLL:
         "adatm_batch_cpd": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "adatm_batch_sequential": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "adatm_batch_set_persistent": (C.c_int, [C.c_void_p, C.c_int]),
+        "adatm_batch_set_form": (C.c_int, [C.c_void_p, C.c_int]),
+        "adatm_batch_form": (C.c_int, [C.c_void_p]),
         "adatm_persist_plan": (C.c_int, [C.c_uint32, P(C.c_uint32), P(C.c_double), C.c_uint32,
                                          P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "adatm_batch_persistent": (C.c_int, [C.c_void_p]),
@@ -788,6 +790,10 @@ def persist_plan(lens, costs, nq: int):
     return queue[:n], order[: int(lens.sum())], qstart
 
 
+# forms of a batched sweep's MTTKRPs (adatm_batch_set_form)
+FORM_FORKED, FORM_PERSISTENT, FORM_GROUPED = 0, 1, 2
+
+
 class Batch:
     """Batched CP-ALS over independent engines (wraps adatm_batch); keeps
     the engines alive while it exists."""
@@ -808,13 +814,23 @@ class Batch:
 
     def set_persistent(self, on: bool) -> None:
         """one persistent launch per round (True) or one forked branch of
-        launches per instance (False, the default)"""
+        launches per instance (False)"""
         check(lib().adatm_batch_set_persistent(self.h, int(bool(on))), "batch_set_persistent")
 
     @property
     def persistent(self) -> bool:
         """whether the last batched sweep ran in persistent mode"""
         return bool(lib().adatm_batch_persistent(self.h))
+
+    def set_form(self, form: int) -> None:
+        """FORM_GROUPED (default where it applies), FORM_PERSISTENT or
+        FORM_FORKED"""
+        check(lib().adatm_batch_set_form(self.h, int(form)), "batch_set_form")
+
+    @property
+    def form(self) -> int:
+        """the form of the last built batched sweep"""
+        return int(lib().adatm_batch_form(self.h))
 
     def sequential(self, niters: int) -> float:
         ms = C.c_double()

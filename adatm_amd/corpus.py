@@ -129,7 +129,7 @@ def split(d: dict, train: bool) -> dict:
     return out
 
 
-def train(d: dict, epochs: int = 400, batch: int = 64, lr: float = 3e-3,
+def train(d: dict, epochs: int = 800, batch: int = 64, lr: float = 3e-3,
           seed: int = 1, device: int = 0) -> "A.Mlp":
     tr = split(d, True)
     m = A.Mlp(device=device, seed=seed)
@@ -154,7 +154,9 @@ def main(argv=None) -> int:
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
-    t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--epochs", type=int, default=800,
+                   help="800 (with lr 3e-3, batch 64) had the lowest mean regret, over "
+                        "three seeds, on tensors 256..319 when training on 0..255")
     e = sub.add_parser("eval")
     e.add_argument("--data", nargs="+", required=True)
     e.add_argument("--model", required=True)

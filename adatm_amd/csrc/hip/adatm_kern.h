@@ -147,6 +147,11 @@ int adatm_k_persist_grid(int prec, int device, uint32_t *grid);
 int adatm_k_persist(hipStream_t s, int prec, uint32_t grid, adatm_kstage const *stages,
                     uint64_t const *tasks, uint32_t const *qoff, uint32_t const *xmap,
                     uint32_t nq, uint32_t *ctr, uint32_t *stat, uint32_t spin);
+/* Grouped launch: one ordinary launch of nblocks blocks runs the stages of
+ * one kind (kind->type, leaf, K, side, d, zs; K and d <= 5): block b runs
+ * block bmap[b] & 0xffffffff of stages[bmap[b] >> 32]. */
+int adatm_k_grouped(hipStream_t s, int prec, adatm_kstage const *kind,
+                    adatm_kstage const *stages, uint64_t const *bmap, uint32_t nblocks);
 /* XCD ids (HW_REG_XCC_ID) seen by nblocks workgroups: out[b] for block b */
 int adatm_k_xcc_probe(hipStream_t s, uint32_t *out, uint32_t nblocks);
 

@@ -101,7 +101,11 @@ int adatm_k_scatter(hipStream_t s, int prec, adatm_kwork const *w,
                                   uint64_t const *tasks, uint32_t const *qoff,     \
                                   uint32_t const *xmap, uint32_t nq, uint32_t *ctr, \
                                   uint32_t *stat, uint32_t spin);                  \
-  hipError_t adatm_persist_grid_##NAME(int device, uint32_t *grid);
+  hipError_t adatm_persist_grid_##NAME(int device, uint32_t *grid);                   \
+  hipError_t adatm_grouped_##NAME(hipStream_t s, uint32_t type, uint32_t leaf,        \
+                                  uint32_t K, uint32_t side, uint32_t d, uint32_t zs, \
+                                  adatm_kstage const *st, uint64_t const *bmap,       \
+                                  uint32_t nb);
 extern "C" {
 ADATM_PERSIST_DECL(fp64)
 ADATM_PERSIST_DECL(fp32)
@@ -136,6 +140,21 @@ int adatm_k_persist(hipStream_t s, int prec, uint32_t grid, adatm_kstage const *
       : (prec == KP_FP32)
           ? adatm_persist_fp32(s, grid, stages, tasks, qoff, xmap, nq, ctr, stat, spin)
           : adatm_persist_bf16(s, grid, stages, tasks, qoff, xmap, nq, ctr, stat, spin);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int adatm_k_grouped(hipStream_t s, int prec, adatm_kstage const *kind,
+                    adatm_kstage const *stages, uint64_t const *bmap, uint32_t nblocks)
+{
+  if (!kind || (kind->type == KS_REDUCE && (kind->K < 1 || kind->K > 5)) ||
+      (kind->type == KS_SCATTER && (kind->d < 1 || kind->d > 5)))
+    return (int)hipErrorInvalidValue;
+  uint32_t const t = kind->type, l = kind->leaf, K = kind->K, sd = kind->side,
+                 d = kind->d, z = kind->zs;
+  hipError_t const e =
+      (prec == KP_FP64)   ? adatm_grouped_fp64(s, t, l, K, sd, d, z, stages, bmap, nblocks)
+      : (prec == KP_FP32) ? adatm_grouped_fp32(s, t, l, K, sd, d, z, stages, bmap, nblocks)
+                          : adatm_grouped_bf16(s, t, l, K, sd, d, z, stages, bmap, nblocks);
   return e == hipSuccess ? 0 : (int)e;
 }
 

@@ -813,6 +813,115 @@ __device__ __forceinline__ void ps_scatter_d(adatm_kstage const *st, uint32_t d,
   }
 }
 
+/* ---- grouped launch (adatm_k_grouped) -----------------------------------
+ * One ordinary launch runs one kind of stage over many stages: block b runs
+ * block bmap[b] & 0xffffffff of stage bmap[b] >> 32.  Ordering between
+ * launches comes from the stream, as for the single launches. */
+template <typename T, typename TU, bool LEAF, int K, bool SIDE>
+__global__ __launch_bounds__(256) void k_g_reduce(adatm_kstage const *__restrict__ st,
+                                                  uint64_t const *__restrict__ bmap)
+{
+  uint64_t const m = bmap[blockIdx.x];
+  ps_reduce<T, TU, LEAF, K, SIDE>(st + (m >> 32), (uint32_t)m);
+}
+
+template <typename T, typename TU, int D, bool ZS>
+__global__ __launch_bounds__(256) void k_g_scatter(adatm_kstage const *__restrict__ st,
+                                                   uint64_t const *__restrict__ bmap)
+{
+  uint64_t const m = bmap[blockIdx.x];
+  ps_scatter<T, TU, D, ZS>(st + (m >> 32), (uint32_t)m);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_g_fix_chunk(adatm_kstage const *__restrict__ st,
+                                                     uint64_t const *__restrict__ bmap)
+{
+  uint64_t const m = bmap[blockIdx.x];
+  ps_fix_chunk<T>(st + (m >> 32), (uint32_t)m);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_g_fix_multi(adatm_kstage const *__restrict__ st,
+                                                     uint64_t const *__restrict__ bmap)
+{
+  uint64_t const m = bmap[blockIdx.x];
+  ps_fix_multi<T>(st + (m >> 32), (uint32_t)m);
+}
+
+__global__ __launch_bounds__(256) void k_g_zero(adatm_kstage const *__restrict__ st,
+                                                uint64_t const *__restrict__ bmap)
+{
+  uint64_t const m = bmap[blockIdx.x];
+  ps_zero(st + (m >> 32), (uint32_t)m);
+}
+
+template <typename T, typename TU, bool LEAF, bool SIDE>
+void g_reduce_k(hipStream_t s, uint32_t K, adatm_kstage const *st, uint64_t const *bmap,
+                uint32_t nb)
+{
+  dim3 const g(nb), b(256);
+  switch (K) {
+  case 1:
+    if constexpr (!SIDE) hipLaunchKernelGGL((k_g_reduce<T, TU, LEAF, 1, SIDE>), g, b, 0, s, st, bmap);
+    break;
+  case 2: hipLaunchKernelGGL((k_g_reduce<T, TU, LEAF, 2, SIDE>), g, b, 0, s, st, bmap); break;
+  case 3: hipLaunchKernelGGL((k_g_reduce<T, TU, LEAF, 3, SIDE>), g, b, 0, s, st, bmap); break;
+  case 4: hipLaunchKernelGGL((k_g_reduce<T, TU, LEAF, 4, SIDE>), g, b, 0, s, st, bmap); break;
+  default: hipLaunchKernelGGL((k_g_reduce<T, TU, LEAF, 5, SIDE>), g, b, 0, s, st, bmap); break;
+  }
+}
+
+template <typename T, typename TU, bool ZS>
+void g_scatter_d(hipStream_t s, uint32_t d, adatm_kstage const *st, uint64_t const *bmap,
+                 uint32_t nb)
+{
+  dim3 const g(nb), b(256);
+  switch (d) {
+  case 1: hipLaunchKernelGGL((k_g_scatter<T, TU, 1, ZS>), g, b, 0, s, st, bmap); break;
+  case 2: hipLaunchKernelGGL((k_g_scatter<T, TU, 2, ZS>), g, b, 0, s, st, bmap); break;
+  case 3: hipLaunchKernelGGL((k_g_scatter<T, TU, 3, ZS>), g, b, 0, s, st, bmap); break;
+  case 4: hipLaunchKernelGGL((k_g_scatter<T, TU, 4, ZS>), g, b, 0, s, st, bmap); break;
+  default: hipLaunchKernelGGL((k_g_scatter<T, TU, 5, ZS>), g, b, 0, s, st, bmap); break;
+  }
+}
+
+/* one grouped launch of the stages' kind (type, leaf, K, side, d, zs as in
+   the stage descriptor; every stage of bmap shares it) */
+template <typename T, typename TU>
+hipError_t grouped_k(hipStream_t s, uint32_t type, uint32_t leaf, uint32_t K, uint32_t side,
+                     uint32_t d, uint32_t zs, adatm_kstage const *st, uint64_t const *bmap,
+                     uint32_t nb)
+{
+  if (nb == 0) return hipSuccess;
+  switch (type) {
+  case KS_REDUCE:
+    if (leaf) {
+      if (side)
+        g_reduce_k<T, TU, true, true>(s, K, st, bmap, nb);
+      else
+        g_reduce_k<T, TU, true, false>(s, K, st, bmap, nb);
+    } else {
+      g_reduce_k<T, TU, false, false>(s, K, st, bmap, nb);
+    }
+    break;
+  case KS_SCATTER:
+    if (zs)
+      g_scatter_d<T, TU, true>(s, d, st, bmap, nb);
+    else
+      g_scatter_d<T, TU, false>(s, d, st, bmap, nb);
+    break;
+  case KS_FIXCHUNK:
+    hipLaunchKernelGGL((k_g_fix_chunk<T>), dim3(nb), dim3(256), 0, s, st, bmap);
+    break;
+  case KS_FIXMULTI:
+    hipLaunchKernelGGL((k_g_fix_multi<T>), dim3(nb), dim3(256), 0, s, st, bmap);
+    break;
+  default: hipLaunchKernelGGL(k_g_zero, dim3(nb), dim3(256), 0, s, st, bmap); break;
+  }
+  return hipGetLastError();
+}
+
 /* XCD (L2 domain) of the calling wave: HW_REG_XCC_ID bits 3:0 */
 __device__ __forceinline__ uint32_t xcc_id()
 {
@@ -932,6 +1041,13 @@ __global__ __launch_bounds__(256) void k_persist(
     hipLaunchKernelGGL((k_persist<T, TU>), dim3(grid), dim3(256), 0, s, st, tasks,  \
                        qoff, xmap, nq, ctr, stat, spin);                           \
     return hipGetLastError();                                                      \
+  }                                                                                \
+  extern "C" hipError_t adatm_grouped_##NAME(                                       \
+      hipStream_t s, uint32_t type, uint32_t leaf, uint32_t K, uint32_t side,      \
+      uint32_t d, uint32_t zs, adatm_kstage const *st, uint64_t const *bmap,      \
+      uint32_t nb)                                                                 \
+  {                                                                                \
+    return grouped_k<T, TU>(s, type, leaf, K, side, d, zs, st, bmap, nb);          \
   }                                                                                \
   extern "C" hipError_t adatm_persist_grid_##NAME(int device, uint32_t *grid)      \
   {                                                                                \

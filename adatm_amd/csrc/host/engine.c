@@ -2233,6 +2233,12 @@ typedef struct {
   uint32_t mm_off, gr_off, sp_off;    /* their block maps in dmap */
 } batch_round;
 
+typedef struct {
+  adatm_kstage kind; /* type, leaf, K, side, d, zs of the launch's stages */
+  uint64_t bmap0;    /* its first block-map entry */
+  uint32_t nb;       /* its blocks */
+} glaunch;
+
 struct adatm_batch {
   int n, device;
   adatm_engine **e;
@@ -2249,8 +2255,14 @@ struct adatm_batch {
   uint32_t fit0, nb_in, in_off; /* the fits: descriptors, grid, block map */
   adatm_kdense *ddesc; /* device descriptors of every round, then the fits */
   uint32_t *dmap;      /* device block maps of every round, then the fits' */
-  /* persistent mode: each round's MTTKRPs in one adatm_k_persist launch */
-  int want_persist, persist;
+  /* the form of the MTTKRPs of a round (ADATM_FORM_*): wanted, last built */
+  int want_form, form;
+  /* grouped form: per round, one launch per depth and kind of stage */
+  adatm_kstage *gst;   /* device stages of every round */
+  uint64_t *gbmap;     /* device block maps of every launch */
+  glaunch *gl;         /* the launches of every round, round by round */
+  uint32_t *gl0;       /* first launch of each round (nround + 1) */
+  /* persistent form: each round's MTTKRPs in one adatm_k_persist launch */
   uint32_t nq, pgrid, spin;
   uint32_t xmap[16];
   uint32_t *p_stage0, *p_nstage, *p_task0; /* per round, into the arrays below */
@@ -2290,6 +2302,10 @@ void adatm_batch_destroy(adatm_batch *b)
   free(b->p_nstage);
   free(b->p_task0);
   free(b->p_served);
+  hipFree(b->gst);
+  hipFree(b->gbmap);
+  free(b->gl);
+  free(b->gl0);
   free(b->round);
   free(b->join);
   free(b->e);
@@ -2536,6 +2552,9 @@ static int batch_persist_ok(adatm_batch const *b)
   return 1;
 }
 
+static int batch_record_round(adatm_batch *b, uint32_t r, krec *all, uint32_t s0,
+                              uint32_t *first, uint32_t *len);
+
 static int batch_persist_build(adatm_batch *b)
 {
   int rc = batch_probe_xcc(b);
@@ -2567,24 +2586,7 @@ static int batch_persist_build(adatm_batch *b)
     b->p_stage0[r] = s0;
     b->p_task0[r] = (uint32_t)ntask;
     /* each instance's chain of stages, recorded as its own launches */
-    for (uint32_t i = 0; rc == SPLATT_SUCCESS && i < n; ++i) {
-      adatm_engine *e = b->e[i];
-      first[i] = all.n;
-      len[i] = 0;
-      if (e->N <= r) continue;
-      e->rec = &all;
-      rc = mttkrp_dev(e, r);
-      e->rec = NULL;
-      len[i] = all.n - first[i];
-      for (uint32_t k = first[i]; rc == SPLATT_SUCCESS && k < all.n; ++k) {
-        adatm_kstage *x = all.st + k;
-        if ((x->type == KS_REDUCE && x->K > 5) || (x->type == KS_SCATTER && x->d > 5)) {
-          adatm_set_error("persistent launch: a stage deeper than 5 levels");
-          rc = SPLATT_ERROR_BADINPUT;
-        }
-        x->dep = (k == first[i]) ? ADATM_KS_NODEP : k - 1 - s0; /* round-local */
-      }
-    }
+    rc = batch_record_round(b, r, &all, s0, first, len);
     if (rc != SPLATT_SUCCESS) break;
     b->p_nstage[r] = all.n - s0;
     /* chains onto queues and each queue's stage order (adatm_persist_plan:
@@ -2670,6 +2672,156 @@ static int batch_persist_build(adatm_batch *b)
   return rc;
 }
 
+/* ---- grouped form ---------------------------------------------------------
+ * Each round's recorded stages run as ordinary launches, one per depth of the
+ * chains and kind of stage (type and template arguments): every instance's
+ * first stages of one kind in one launch, then the next kind, then the
+ * second stages, and so on.  The stream orders a depth after the one before,
+ * so no device-side waiting or cache maintenance is needed, and each launch
+ * spreads its instances' blocks over the whole device. */
+
+static int same_kind(adatm_kstage const *a, adatm_kstage const *b)
+{
+  return a->type == b->type && a->leaf == b->leaf && a->K == b->K && a->side == b->side &&
+         a->d == b->d && a->zs == b->zs;
+}
+
+/* every instance's chain of stages for round r appended to `all`: chain i
+   is all->st[first[i] .. first[i] + len[i]), each stage waiting for the one
+   before (round-local indices from s0) */
+static int batch_record_round(adatm_batch *b, uint32_t r, krec *all, uint32_t s0,
+                              uint32_t *first, uint32_t *len)
+{
+  int rc = SPLATT_SUCCESS;
+  for (uint32_t i = 0; rc == SPLATT_SUCCESS && i < (uint32_t)b->n; ++i) {
+    adatm_engine *e = b->e[i];
+    first[i] = all->n;
+    len[i] = 0;
+    if (e->N <= r) continue;
+    e->rec = all;
+    rc = mttkrp_dev(e, r);
+    e->rec = NULL;
+    len[i] = all->n - first[i];
+    for (uint32_t k = first[i]; rc == SPLATT_SUCCESS && k < all->n; ++k) {
+      adatm_kstage *x = all->st + k;
+      if ((x->type == KS_REDUCE && x->K > 5) || (x->type == KS_SCATTER && x->d > 5)) {
+        adatm_set_error("batched stages: a stage deeper than 5 levels");
+        rc = SPLATT_ERROR_BADINPUT;
+      }
+      x->dep = (k == first[i]) ? ADATM_KS_NODEP : k - 1 - s0;
+    }
+  }
+  return rc;
+}
+
+static int batch_grouped_build(adatm_batch *b)
+{
+  uint32_t const nr = b->nround, n = (uint32_t)b->n;
+  krec all = {0};
+  uint64_t *bmap = NULL, nbm = 0, bm_cap = 0;
+  uint32_t ngl = 0, gl_cap = 0;
+  uint32_t *first = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *len = (uint32_t *)malloc(n * sizeof(uint32_t));
+  free(b->gl); /* a failed earlier attempt */
+  free(b->gl0);
+  b->gl = NULL;
+  b->gl0 = (uint32_t *)calloc(nr + 1, sizeof(uint32_t));
+  int rc = (first && len && b->gl0) ? SPLATT_SUCCESS : SPLATT_ERROR_NOMEMORY;
+  for (uint32_t r = 0; rc == SPLATT_SUCCESS && r < nr; ++r) {
+    uint32_t const s0 = all.n;
+    b->gl0[r] = ngl;
+    rc = batch_record_round(b, r, &all, s0, first, len);
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; rc == SPLATT_SUCCESS && i < n; ++i)
+      if (len[i] > maxlen) maxlen = len[i];
+    for (uint32_t d = 0; rc == SPLATT_SUCCESS && d < maxlen; ++d) {
+      /* the kinds of depth d in the order they first occur; one launch each */
+      for (uint32_t i0 = 0; rc == SPLATT_SUCCESS && i0 < n; ++i0) {
+        if (d >= len[i0]) continue;
+        adatm_kstage const *k0 = all.st + first[i0] + d;
+        int seen = 0;
+        for (uint32_t i = 0; i < i0 && !seen; ++i)
+          seen = d < len[i] && same_kind(all.st + first[i] + d, k0);
+        if (seen) continue;
+        if (ngl == gl_cap) {
+          uint32_t const cap = gl_cap ? 2 * gl_cap : 64;
+          glaunch *ng = (glaunch *)realloc(b->gl, cap * sizeof(glaunch));
+          if (!ng) {
+            rc = SPLATT_ERROR_NOMEMORY;
+            continue;
+          }
+          b->gl = ng;
+          gl_cap = cap;
+        }
+        glaunch *L = b->gl + ngl++;
+        L->kind = *k0;
+        L->bmap0 = nbm;
+        L->nb = 0;
+        for (uint32_t i = i0; rc == SPLATT_SUCCESS && i < n; ++i) {
+          if (d >= len[i] || !same_kind(all.st + first[i] + d, k0)) continue;
+          uint32_t const si = first[i] + d, nbk = all.st[si].nblocks;
+          if (nbm + nbk > bm_cap) {
+            uint64_t cap = bm_cap ? 2 * bm_cap : 4096;
+            while (cap < nbm + nbk) cap *= 2;
+            uint64_t *nb = (uint64_t *)realloc(bmap, cap * sizeof(uint64_t));
+            if (!nb) {
+              rc = SPLATT_ERROR_NOMEMORY;
+              continue;
+            }
+            bmap = nb;
+            bm_cap = cap;
+          }
+          for (uint32_t v = 0; v < nbk; ++v) bmap[nbm++] = ((uint64_t)si << 32) | v;
+          L->nb += nbk;
+        }
+      }
+    }
+  }
+  if (rc == SPLATT_SUCCESS) {
+    b->gl0[nr] = ngl;
+    size_t const ns = all.n ? all.n : 1, nm = nbm ? nbm : 1;
+    if (hipMalloc((void **)&b->gst, ns * sizeof(adatm_kstage)) != hipSuccess ||
+        hipMalloc((void **)&b->gbmap, nm * sizeof(uint64_t)) != hipSuccess ||
+        (all.n && hipMemcpy(b->gst, all.st, all.n * sizeof(adatm_kstage),
+                            hipMemcpyHostToDevice) != hipSuccess) ||
+        (nbm && hipMemcpy(b->gbmap, bmap, nbm * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                    hipSuccess))
+      rc = hip_fail(hipGetLastError(), "grouped launch descriptors");
+  }
+  if (rc != SPLATT_SUCCESS) {
+    hipFree(b->gst);
+    hipFree(b->gbmap);
+    b->gst = NULL;
+    b->gbmap = NULL;
+  }
+  free(all.st);
+  free(all.cost);
+  free(bmap);
+  free(first);
+  free(len);
+  return rc;
+}
+
+static int batch_enqueue_grouped_form(adatm_batch *b)
+{
+  for (uint32_t r = 0; r < b->nround; ++r) {
+    batch_round const *B = b->round + r;
+    for (uint32_t l = b->gl0[r]; l < b->gl0[r + 1]; ++l) {
+      glaunch const *L = b->gl + l;
+      KCHK(adatm_k_grouped(b->s, b->e[0]->prec, &L->kind, b->gst, b->gbmap + L->bmap0, L->nb),
+           "grouped MTTKRP launch");
+    }
+    KCHK(adatm_k_dense_grouped(b->s, b->e[0]->prec, b->ddesc + B->desc0, B->n, B->Rmax,
+                                b->dmap + B->mm_off, B->nb_mm, b->dmap + B->gr_off,
+                                B->nb_gr, b->dmap + B->sp_off, B->nb_sp),
+          "grouped update");
+  }
+  KCHK(adatm_k_fit_grouped(b->s, b->e[0]->prec, b->ddesc + b->fit0, (uint32_t)b->n,
+                           b->dmap + b->in_off, b->nb_in),
+       "grouped fit");
+  return SPLATT_SUCCESS;
+}
+
 /* the persistent sweep: per round, the counters cleared, one persistent
  * launch of every instance's MTTKRP, one grouped dense update; one grouped
  * fit last */
@@ -2743,7 +2895,8 @@ int adatm_batch_create(adatm_engine *const *engines, int n, adatm_batch **out)
   b->grouped = 1;
   for (int i = 1; i < n; ++i)
     if (engines[i]->prec != engines[0]->prec) b->grouped = 0;
-  b->want_persist = 0; /* forked branches unless asked (measured faster) */
+  /* grouped launches where they apply (measured fastest), else the branches */
+  b->want_form = ADATM_FORM_GROUPED;
   int rc = SPLATT_SUCCESS;
   if (hipSetDevice(b->device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
@@ -2768,15 +2921,24 @@ static int batch_graph(adatm_batch *b)
     int const rc = batch_rounds(b);
     if (rc != SPLATT_SUCCESS) return rc;
   }
-  if (b->want_persist && !b->persist && !b->dstages && batch_persist_ok(b)) {
+  int const ok = batch_persist_ok(b);
+  if (b->want_form == ADATM_FORM_PERSISTENT && !b->dstages && ok) {
     int const rc = batch_persist_build(b);
     if (rc != SPLATT_SUCCESS) return rc;
   }
-  b->persist = b->want_persist && b->dstages != NULL;
+  if (b->want_form == ADATM_FORM_GROUPED && !b->gst && ok) {
+    int const rc = batch_grouped_build(b);
+    if (rc != SPLATT_SUCCESS) return rc;
+  }
+  b->form = (b->want_form == ADATM_FORM_PERSISTENT && b->dstages)  ? ADATM_FORM_PERSISTENT
+            : (b->want_form == ADATM_FORM_GROUPED && b->gst) ? ADATM_FORM_GROUPED
+                                                              : ADATM_FORM_FORKED;
   HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal), "begin capture");
   int rc = SPLATT_SUCCESS;
-  if (b->persist)
+  if (b->form == ADATM_FORM_PERSISTENT)
     rc = batch_enqueue_persist(b);
+  else if (b->form == ADATM_FORM_GROUPED)
+    rc = batch_enqueue_grouped_form(b);
   else if (b->grouped)
     rc = batch_enqueue_grouped(b);
   else if (hipEventRecord(b->fork, b->s) != hipSuccess)
@@ -2817,7 +2979,7 @@ int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *f
   }
   int rc = batch_graph(b);
   if (rc != SPLATT_SUCCESS) return rc;
-  if (b->persist)
+  if (b->form == ADATM_FORM_PERSISTENT)
     HIPCHK(hipMemset(b->dstat, 0, (size_t)2 * b->nq * 4), "persistent statistics");
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0), "event");
@@ -2841,7 +3003,7 @@ int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *f
   if (rc != SPLATT_SUCCESS) return rc;
   HIPCHK(es, "batch sync");
   for (int i = 0; i < b->n; ++i) b->e[i]->nsweep = niters; /* the device counters */
-  if (b->persist) {
+  if (b->form == ADATM_FORM_PERSISTENT) {
     /* every queue served all its tasks, without a wait that gave up */
     uint32_t st[32];
     HIPCHK(hipMemcpy(st, b->dstat, (size_t)2 * b->nq * 4, hipMemcpyDeviceToHost),
@@ -2874,24 +3036,34 @@ int adatm_batch_cpd(adatm_batch *b, uint32_t niters, double *ms_total, double *f
   return status_rc(worst);
 }
 
-int adatm_batch_set_persistent(adatm_batch *b, int on)
+int adatm_batch_set_form(adatm_batch *b, int form)
 {
-  if (!b) return SPLATT_ERROR_BADINPUT;
-  if (on && !batch_persist_ok(b)) {
-    adatm_set_error("batch: the persistent launch needs one precision, 8 <= R <= 32 "
-                    "and at most 6 modes per instance");
+  if (!b || form < ADATM_FORM_FORKED || form > ADATM_FORM_GROUPED) return SPLATT_ERROR_BADINPUT;
+  if (form != ADATM_FORM_FORKED && !batch_persist_ok(b)) {
+    adatm_set_error("batch: the persistent and grouped forms need one precision, "
+                    "8 <= R <= 32 and at most 6 modes per instance");
     return SPLATT_ERROR_BADINPUT;
   }
-  if ((on != 0) != (b->want_persist != 0) && b->gvalid) {
+  if (form != b->want_form && b->gvalid) {
     hipGraphExecDestroy(b->gexec);
     hipGraphDestroy(b->graph);
     b->gvalid = 0;
   }
-  b->want_persist = on != 0;
+  b->want_form = form;
   return SPLATT_SUCCESS;
 }
 
-int adatm_batch_persistent(adatm_batch const *b) { return b && b->persist; }
+int adatm_batch_form(adatm_batch const *b) { return b ? b->form : ADATM_FORM_FORKED; }
+
+int adatm_batch_set_persistent(adatm_batch *b, int on)
+{
+  return adatm_batch_set_form(b, on ? ADATM_FORM_PERSISTENT : ADATM_FORM_FORKED);
+}
+
+int adatm_batch_persistent(adatm_batch const *b)
+{
+  return b && b->form == ADATM_FORM_PERSISTENT;
+}
 
 int adatm_batch_persist_info(adatm_batch const *b, uint32_t *nq, uint32_t *xmap,
                              uint32_t *grid)

@@ -138,7 +138,7 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=64, help="batched workload (--config 5)")
     ap.add_argument("--iters", type=int, default=50, help="batched workload sweeps")
     ap.add_argument("--no-persistent", action="store_true",
-                    help="batched workload: forked branches only (no persistent launch)")
+                    help="batched workload: forked branches only (no grouped or persistent form)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched workload: skip timing the instances one after another")
     ap.add_argument("--hw-queues", type=int, default=0,
@@ -411,9 +411,10 @@ def batched_c5(args) -> int:
     """64 uneven CP-ALS instances: analytic plans, autotuned tiles per
     instance, FP32 with FP64 solves, every instance's sweep in one hipGraph
     (one launch per batched sweep) whose rounds run all instances' MTTKRPs
-    in one persistent kernel launch each (the value), beside the same batch
-    with one forked branch of launches per instance, the same engines run
-    one instance after another, and FP64 fits of the same instances."""
+    as grouped launches (one per depth and kind of stage: the value), beside
+    the same batch with one persistent kernel launch per round and with one
+    forked branch of launches per instance, the same engines run one
+    instance after another, and FP64 fits of the same instances."""
     if args.hw_queues:
         # read by the HIP runtime when it starts: set before the library loads
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.hw_queues, 1), 32))
@@ -444,22 +445,26 @@ def batched_c5(args) -> int:
         f"{time.time() - t0:.1f}s")
     b = A.Batch(eng)
 
-    def run(persistent):
-        b.set_persistent(persistent)
+    def run(form):
+        b.set_form(form)
         b.cpd(2)                               # warm-up (graph capture)
         for k, e in enumerate(eng):            # restart from the seeded factors
             e.init_factors(c5_spec(k)[0].seed + 1)
-        return b.cpd(iters), b.persistent
+        return b.cpd(iters), b.form
 
-    br_ms = br_same = None
+    br_ms = br_same = pe_ms = pe_same = None
     if not args.no_persistent:
-        # the forked branches first, then the persistent launch (the value)
-        (br_ms, br_fits), _ = run(False)
-    (ms, fits), persistent = run(not args.no_persistent)
+        # the forked branches and the persistent launch first, then the
+        # grouped launches (the default form: the value)
+        (br_ms, br_fits), _ = run(A.FORM_FORKED)
+        (pe_ms, pe_fits), _ = run(A.FORM_PERSISTENT)
+    (ms, fits), form = run(A.FORM_FORKED if args.no_persistent else A.FORM_GROUPED)
     if br_ms is not None:
         br_same = bool(np.array_equal(fits, br_fits))
-        log(f"C5: persistent {ms / iters:.3f} ms/sweep (fits equal to the forked "
-            f"branches': {br_same}), forked branches {br_ms / iters:.3f} ms/sweep")
+        pe_same = bool(np.array_equal(fits, pe_fits))
+        log(f"C5: grouped launches {ms / iters:.3f} ms/sweep (fits equal to the forked "
+            f"branches': {br_same}, to the persistent launch's: {pe_same}), forked "
+            f"branches {br_ms / iters:.3f}, persistent launch {pe_ms / iters:.3f} ms/sweep")
     seq_ms = None if args.no_sequential else b.sequential(2) / 2.0
     dfit = None
     if e64:
@@ -479,9 +484,12 @@ def batched_c5(args) -> int:
                    "nnz_total": nnz_total,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "tiles": [list(x) for x in tiles]},
-        "batched_mode": "persistent" if persistent else "forked branches",
+        "batched_form": {A.FORM_FORKED: "forked branches", A.FORM_PERSISTENT: "persistent launch",
+                         A.FORM_GROUPED: "grouped launches"}[form],
         "forked_branches_ms_per_sweep": round(br_ms / iters, 4) if br_ms else None,
-        "persistent_fits_equal_forked": br_same,
+        "persistent_launch_ms_per_sweep": round(pe_ms / iters, 4) if pe_ms else None,
+        "fits_equal_forked": br_same,
+        "fits_equal_persistent": pe_same,
         "sequential_ms_per_sweep": round(seq_ms, 4) if seq_ms else None,
         "batching_speedup": round(seq_ms / per, 3) if seq_ms else None,
         "aggregate_mttkrp_GBps": round(seq_bytes / (per * 1e-3) / 1e9, 1),

@@ -410,11 +410,24 @@ int adatm_batch_sequential(adatm_batch *b, uint32_t niters, double *ms_total);
  * all instances run in ONE persistent kernel launch, whose workgroups take
  * the recorded launches' blocks from per-XCD queues in dependency order,
  * instead of one forked branch of launches per instance; the results are
- * bit-identical.  The forked branches (on = 0) are the default: on 64
- * uneven instances they measured about 10% faster.  adatm_batch_persistent:
- * whether the last built batched sweep ran in persistent mode. */
+ * bit-identical.  on = 0 selects the forked branches (see also
+ * adatm_batch_set_form).  adatm_batch_persistent: whether the last built
+ * batched sweep ran in persistent mode. */
 int adatm_batch_set_persistent(adatm_batch *b, int on);
 int adatm_batch_persistent(adatm_batch const *b);
+/* The form of each round's MTTKRPs: ADATM_FORM_FORKED (one forked branch of
+ * launches per instance), ADATM_FORM_PERSISTENT (above), or
+ * ADATM_FORM_GROUPED (the default where it applies): the instances' launches
+ * recorded as stages and run as ordinary launches, one per depth of the
+ * instances' launch sequences and kind of stage, each covering every
+ * instance at that depth (same eligibility as the persistent form;
+ * bit-identical results; on 64 uneven instances 3.9 ms per sweep against 5.1
+ * for the branches and 5.7 for the persistent launch).  A batch outside the
+ * eligibility runs the branches.  adatm_batch_form: the form of the last
+ * built batched sweep. */
+enum { ADATM_FORM_FORKED = 0, ADATM_FORM_PERSISTENT = 1, ADATM_FORM_GROUPED = 2 };
+int adatm_batch_set_form(adatm_batch *b, int form);
+int adatm_batch_form(adatm_batch const *b);
 /* The queue plan of one round of the persistent form: n chains (an
  * instance's recorded launches), chain i of len[i] stages whose costs are
  * stored chain after chain in cost[]; queue[i] = the queue of chain i

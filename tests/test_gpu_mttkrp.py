@@ -282,14 +282,16 @@ def test_batched_cpd_matches_each_instance(precs):
     b.close()
 
 
+@pytest.mark.parametrize("form", [A.FORM_PERSISTENT, A.FORM_GROUPED])
 @pytest.mark.parametrize("prec", [A.FP32, A.FP64])
-def test_persistent_batch_equals_forked_branches(prec):
-    """Each round's MTTKRPs of every instance in one persistent launch (the
-    recorded launches' blocks taken from per-XCD queues in dependency order):
-    fits, factors and lambdas bit-identical to the forked-branch batch, over
-    two runs of the same captured graph; Zipf instances bring split parents
+def test_persistent_batch_equals_forked_branches(prec, form):
+    """Each round's MTTKRPs of every instance recorded as stages and run in
+    one persistent launch (blocks taken from per-XCD queues in dependency
+    order) or as grouped launches (one per depth and kind of stage): fits,
+    factors and lambdas bit-identical to the forked-branch batch, over two
+    runs of the same captured graph; Zipf instances bring split parents
     (merge stages) and empty root slices (zero-fill stages)."""
-    print(f"persistent batch precision={prec}", flush=True)
+    print(f"batch form={form} precision={prec}", flush=True)
     specs = [([60, 50, 40, 30], 8000, A.DIST_UNIFORM, 8),
              ([300, 70, 60, 50, 40], 30000, A.DIST_ZIPF, 16),
              ([40, 35, 30, 25, 20, 15], 6000, A.DIST_UNIFORM, 32),
@@ -307,11 +309,13 @@ def test_persistent_batch_equals_forked_branches(prec):
             eng.append(e)
         sets.append(eng)
     bp, bf = A.Batch(sets[0]), A.Batch(sets[1])
-    bp.set_persistent(True)
+    bp.set_form(form)
+    bf.set_form(A.FORM_FORKED)
     for n in (4, 3):
         _, fp = bp.cpd(n)
         _, ff = bf.cpd(n)
-        assert bp.persistent and not bf.persistent
+        assert bp.form == form and bf.form == A.FORM_FORKED
+        assert bp.persistent == (form == A.FORM_PERSISTENT) and not bf.persistent
         assert np.array_equal(fp, ff), (fp, ff)
         for k, (ep, ef) in enumerate(zip(*sets)):
             assert np.array_equal(ep.get_lambda(), ef.get_lambda()), k
@@ -326,9 +330,9 @@ def test_persistent_batch_equals_forked_branches(prec):
     e.init_factors(5)
     b = A.Batch([e])
     with pytest.raises(A.AdatmError):
-        b.set_persistent(True)
+        b.set_form(form)
     b.cpd(1)
-    assert not b.persistent
+    assert b.form == A.FORM_FORKED
     b.close()
 
 

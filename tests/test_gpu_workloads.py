@@ -139,16 +139,17 @@ def test_c5_first_instances_batched_against_the_oracle():
         ofits.append(seq.cpd(U, iters)[0])
         seq.close()
     b = A.Batch(eng)
-    # both batched forms: one persistent launch per round, forked branches
-    for persistent in (True, False):
-        print(f"C5 batch persistent={persistent}", flush=True)
+    # every batched form: one persistent launch per round, grouped launches,
+    # forked branches
+    for form in (A.FORM_PERSISTENT, A.FORM_GROUPED, A.FORM_FORKED):
+        print(f"C5 batch form={form}", flush=True)
         for e, U in zip(eng, Us):
             for m, Um in enumerate(U):
                 e.set_factor(m, Um)
-        b.set_persistent(persistent)
+        b.set_form(form)
         ms, fits = b.cpd(iters)
-        assert ms > 0 and b.persistent == persistent
+        assert ms > 0 and b.form == form
         for k in range(n):
             d = np.max(np.abs(fits[k] - ofits[k]))
-            assert d <= FIT_TOL[A.FP32], (persistent, k, d)
+            assert d <= FIT_TOL[A.FP32], (form, k, d)
     b.close()
