@@ -52,6 +52,10 @@ __global__ void k_convert_out(T const *__restrict__ src, double *__restrict__ ds
 /* ---- dense CP-ALS steps ------------------------------------------------- */
 #define DB 256          /* threads per dense block */
 #define NBLK_MAX 256    /* partial-sum blocks */
+/* rows per Gram block at least: a block writes an R x R FP64 partial, so
+   with fewer rows than ~4R its partial outweighs the rows it reads (the
+   batched sweeps' many small factors) */
+#define GRAM_ROWS 128u
 
 /* Gram of FP64 rows on the matrix cores: v_mfma_f64_16x16x4f64 takes four
  * rows of A per instruction.  Columns are cut into NT tiles of 16; with
@@ -624,7 +628,7 @@ template <typename T>
 hipError_t gram_impl(hipStream_t s, T *A, bf16raw *Abf, uint64_t I, uint32_t R,
                      double const *lambda, double *part, double *G_out)
 {
-  uint32_t const nb = rows_blocks(I, 32);
+  uint32_t const nb = rows_blocks(I, GRAM_ROWS);
   if constexpr (sizeof(T) == 4) {
     hipLaunchKernelGGL(k_gram_mfma, dim3(nb), dim3(DB), 0, s, (float *)A, Abf, I, R,
                        lambda, part);
@@ -782,7 +786,7 @@ void adatm_k_dense_blocks(uint64_t I, uint32_t R, uint32_t *nmm, uint32_t *ngr,
                           uint32_t *nsp)
 {
   *nmm = rows_blocks(I, DB / R);
-  *ngr = rows_blocks(I, 32);
+  *ngr = rows_blocks(I, GRAM_ROWS);
   *nsp = (R * R + 31) / 32;
 }
 
