@@ -94,28 +94,44 @@ __device__ __forceinline__ void gram_mfma64_body(uint32_t bid, uint32_t nblk,
   for (int p = 0; p < P; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
-  for (uint64_t rp = lo + 4u * w; rp < hi; rp += 16u) {
-    uint64_t const r = rp + kq;
-    double v[NT];
+  /* the wave's row quads rp, rp + 16, ... taken GU at a time: their loads
+     are issued together, then their products accumulated in row order */
+  constexpr int GU = 2;
+  for (uint64_t rp = lo + 4u * w; rp < hi; rp += 16u * GU) {
+    double v[GU][NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      uint32_t const c = 16u * t + col;
-      v[t] = 0.0;
-      if (r < hi && c < R) {
-        size_t const g = (size_t)r * R + c;
-        v[t] = A[g];
-        if (lambda) {
-          v[t] = v[t] / lam[t];
-          A[g] = v[t];
+    for (int u = 0; u < GU; ++u) {
+      uint64_t const r = rp + 16u * (uint32_t)u + kq;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        uint32_t const c = 16u * t + col;
+        v[u][t] = (r < hi && c < R) ? A[(size_t)r * R + c] : 0.0;
+      }
+    }
+    if (lambda) {
+#pragma unroll
+      for (int u = 0; u < GU; ++u) {
+        uint64_t const r = rp + 16u * (uint32_t)u + kq;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          uint32_t const c = 16u * t + col;
+          if (r < hi && c < R) {
+            v[u][t] = v[u][t] / lam[t];
+            A[(size_t)r * R + c] = v[u][t];
+          }
         }
       }
     }
-    int p = 0;
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
+    for (int u = 0; u < GU; ++u) {
+      if (rp + 16u * (uint32_t)u >= hi) break; /* wave-uniform */
+      int p = 0;
 #pragma unroll
-      for (int j = i; j < NT; ++j, ++p)
-        acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[i], v[j], acc[p], 0, 0, 0);
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int j = i; j < NT; ++j, ++p)
+          acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u][i], v[u][j], acc[p], 0, 0, 0);
+    }
   }
   if (w > 0) {
 #pragma unroll
@@ -191,32 +207,49 @@ __device__ __forceinline__ void gram_mfma_body(uint32_t bid, uint32_t nblk, floa
     a01[i] = 0.f;
     a11[i] = 0.f;
   }
-  for (uint64_t rp = lo + 2u * w; rp < hi; rp += 8u) {
-    uint64_t const r = rp + (lane >> 5);
-    float v0 = 0.f, v1 = 0.f;
-    if (r < hi) {
-      size_t const g0 = (size_t)r * R + c0, g1 = (size_t)r * R + c1;
-      if (c0 < R) {
-        v0 = A[g0];
-        if (lambda) {
-          v0 = (float)((double)v0 / l0);
-          A[g0] = v0;
-          if (Abf) Abf[g0].v = to_bf16(v0);
-        }
+  /* the wave's row pairs rp, rp + 8, ... taken GU at a time: their loads
+     are issued together, then their products accumulated in row order */
+  constexpr int GU = 4;
+  for (uint64_t rp = lo + 2u * w; rp < hi; rp += 8u * GU) {
+    float v0[GU], v1[GU];
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      uint64_t const r = rp + 8u * (uint32_t)u + (lane >> 5);
+      v0[u] = 0.f;
+      v1[u] = 0.f;
+      if (r < hi) {
+        size_t const g0 = (size_t)r * R + c0, g1 = (size_t)r * R + c1;
+        if (c0 < R) v0[u] = A[g0];
+        if (two && c1 < R) v1[u] = A[g1];
       }
-      if (two && c1 < R) {
-        v1 = A[g1];
-        if (lambda) {
-          v1 = (float)((double)v1 / l1);
-          A[g1] = v1;
-          if (Abf) Abf[g1].v = to_bf16(v1);
+    }
+    if (lambda) {
+#pragma unroll
+      for (int u = 0; u < GU; ++u) {
+        uint64_t const r = rp + 8u * (uint32_t)u + (lane >> 5);
+        if (r < hi) {
+          size_t const g0 = (size_t)r * R + c0, g1 = (size_t)r * R + c1;
+          if (c0 < R) {
+            v0[u] = (float)((double)v0[u] / l0);
+            A[g0] = v0[u];
+            if (Abf) Abf[g0].v = to_bf16(v0[u]);
+          }
+          if (two && c1 < R) {
+            v1[u] = (float)((double)v1[u] / l1);
+            A[g1] = v1[u];
+            if (Abf) Abf[g1].v = to_bf16(v1[u]);
+          }
         }
       }
     }
-    a00 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0, v0, a00, 0, 0, 0);
-    if (two) {
-      a01 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0, v1, a01, 0, 0, 0);
-      a11 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1, v1, a11, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      if (rp + 8u * (uint32_t)u >= hi) break; /* wave-uniform */
+      a00 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], v0[u], a00, 0, 0, 0);
+      if (two) {
+        a01 = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[u], v1[u], a01, 0, 0, 0);
+        a11 = __builtin_amdgcn_mfma_f32_32x32x2f32(v1[u], v1[u], a11, 0, 0, 0);
+      }
     }
   }
   if (w > 0) {
@@ -308,19 +341,47 @@ __device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
   __shared__ double rowk[2][64], colk[2][4 * Q];
   uint32_t const s = threadIdx.x >> 6, j = threadIdx.x & 63u;
   double a[Q];
-  /* the matrix (plus ridge on the diagonal) into registers */
+  /* the matrix (plus ridge on the diagonal) into registers.  The Grams are
+     read four modes at a time with every load issued before the first
+     product (addresses clamped into the buffer, a factor of exactly 1 for
+     the skipped mode m and the modes past N), so a build costs ceil(N/4)
+     memory round trips rather than one per mode and row; the products are
+     taken in mode order as before. */
   auto build = [&](double ridge) {
+    uint32_t xo[Q];
+    double v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       uint32_t const i = s + 4u * (uint32_t)q;
-      double v = 1.0;
-      if (i < R && j < R) {
-        size_t const x = (size_t)i * R + j;
-        for (uint32_t k = 0; k < N; ++k)
-          if (k != m) v *= G[(size_t)k * R * R + x];
-        if (i == j) v += ridge;
+      xo[q] = (i < R && j < R) ? i * R + j : 0u;
+      v[q] = 1.0;
+    }
+    for (uint32_t k0 = 0; k0 < N; k0 += 4) {
+      double g[4][Q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint32_t const k = k0 + (uint32_t)u;
+        size_t const kb = (size_t)(k < N ? k : 0u) * R * R;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[u][q] = G[kb + xo[q]];
       }
-      a[q] = v;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint32_t const k = k0 + (uint32_t)u;
+        bool const use = k < N && k != m;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] *= use ? g[u][q] : 1.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint32_t const i = s + 4u * (uint32_t)q;
+      double x = 1.0;
+      if (i < R && j < R) {
+        x = v[q];
+        if (i == j) x += ridge;
+      }
+      a[q] = x;
     }
   };
   build(0.0);
@@ -415,7 +476,28 @@ __device__ __forceinline__ void matmul_stats_body(uint32_t bid, uint32_t nblk, T
     uint64_t const chunk = (I + nblk - 1) / nblk;
     uint64_t const lo = (uint64_t)bid * chunk;
     uint64_t const hi = lo + chunk < I ? lo + chunk : I;
-    for (uint64_t i = lo + rr; i < hi; i += rpi) {
+    /* four rows at a time (independent product chains, each over j in
+       order), their statistics taken in row order */
+    uint64_t i = lo + rr;
+    for (; i + 3u * rpi < hi; i += 4u * rpi) {
+      T const *mrow = M + (size_t)i * R;
+      size_t const st = (size_t)rpi * R;
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      for (uint32_t j = 0; j < R; ++j) {
+        double const vj = sv[j * R + cc];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] += (double)mrow[u * st + j] * vj;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        T const o = (T)s[u];
+        A[(size_t)(i + (uint64_t)u * rpi) * R + cc] = o;
+        double const od = (double)o;
+        ssq += od * od;
+        mx = od > mx ? od : mx;
+      }
+    }
+    for (; i < hi; i += rpi) {
       T const *mrow = M + (size_t)i * R;
       double s = 0.0;
       for (uint32_t j = 0; j < R; ++j) s += (double)mrow[j] * sv[j * R + cc];

@@ -444,8 +444,11 @@ __global__ __launch_bounds__(256) void k_scatter(
 
 /* Chunk c of split parent h = cseg[c]: the sum of its <= ADATM_FIX_CHUNK
  * carry rows, eight independent partial sums combined in a fixed order; the
- * parent's output row when it has one chunk, else chunk row c of carry2. */
-template <typename T>
+ * parent's output row when it has one chunk, else chunk row c of carry2.  A
+ * lane group of G = R/V lanes per chunk, lane j holding columns [jV, jV+V):
+ * every column sees the same additions whatever V is, so the layouts agree
+ * bit for bit. */
+template <typename T, int V>
 __device__ __forceinline__ void fix_chunk_body(
     uint32_t vb, uint32_t nchunk, uint32_t const *__restrict__ cstart,
     uint32_t const *__restrict__ cseg, uint32_t const *__restrict__ hslot,
@@ -453,10 +456,11 @@ __device__ __forceinline__ void fix_chunk_body(
     T const *__restrict__ carry, T *__restrict__ carry2, T *__restrict__ out,
     uint32_t R, uint32_t gpw)
 {
+  uint32_t const G = R / V;
   uint32_t const lane = threadIdx.x & 63u;
-  uint32_t const grp = lane / R;
+  uint32_t const grp = lane / G;
   if (grp >= gpw) return;
-  uint32_t const col = lane - grp * R;
+  uint32_t const cv = (lane - grp * G) * V;
   uint32_t const c =
       (vb * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gpw + grp;
   if (c >= nchunk) return;
@@ -464,23 +468,42 @@ __device__ __forceinline__ void fix_chunk_body(
   uint32_t const lo = cstart[c];
   uint32_t const end = hslot[h + 1];
   uint32_t const hi = (lo + ADATM_FIX_CHUNK < end) ? lo + ADATM_FIX_CHUNK : end;
-  T p[8];
+  T p[8][V];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) p[i] = (T)0;
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int v = 0; v < V; ++v) p[i][v] = (T)0;
+  T x[8][V];
   uint32_t s = lo;
   for (; s + 8 <= hi; s += 8) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) p[i] += carry[(size_t)(s + i) * R + col];
+    for (int i = 0; i < 8; ++i) ldrow(carry + ((size_t)(s + i) * R + cv), x[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int v = 0; v < V; ++v) p[i][v] += x[i][v];
   }
 #pragma unroll
   for (int i = 0; i < 7; ++i)
-    if (s + i < hi) p[i] += carry[(size_t)(s + i) * R + col];
-  T const acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+    if (s + i < hi) {
+      ldrow(carry + ((size_t)(s + i) * R + cv), x[i]);
+#pragma unroll
+      for (int v = 0; v < V; ++v) p[i][v] += x[i][v];
+    }
+  T acc[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    acc[v] = ((p[0][v] + p[1][v]) + (p[2][v] + p[3][v])) +
+             ((p[4][v] + p[5][v]) + (p[6][v] + p[7][v]));
   if (hchunk[h + 1] - hchunk[h] == 1)
-    out[(size_t)hseg[h] * R + col] = acc;
+    strow<T, V>(out + ((size_t)hseg[h] * R + cv), acc);
   else
-    carry2[(size_t)c * R + col] = acc;
+    strow<T, V>(carry2 + ((size_t)c * R + cv), acc);
 }
+
+/* columns per lane of the merges: 16 bytes when R is a multiple */
+template <typename T>
+constexpr int fix_v() { return sizeof(T) == 8 ? 2 : 4; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fix_chunk(
@@ -490,41 +513,83 @@ __global__ __launch_bounds__(256) void k_fix_chunk(
     T const *__restrict__ carry, T *__restrict__ carry2, T *__restrict__ out,
     uint32_t R, uint32_t gpw)
 {
-  fix_chunk_body<T>(blockIdx.x, nchunk, cstart, cseg, hslot, hchunk, hseg, carry, carry2,
-                    out, R, gpw);
+  if (R % fix_v<T>() == 0)
+    fix_chunk_body<T, fix_v<T>()>(blockIdx.x, nchunk, cstart, cseg, hslot, hchunk, hseg,
+                                  carry, carry2, out, R, gpw);
+  else
+    fix_chunk_body<T, 1>(blockIdx.x, nchunk, cstart, cseg, hslot, hchunk, hseg, carry,
+                         carry2, out, R, gpw);
 }
 
-/* Split parents with several chunks (one workgroup each): tree sum of their
- * chunk rows in a fixed order. */
-template <typename T>
+/* Split parents with several chunks (one workgroup each): slice q of the
+ * workgroup (G = R/V lanes, lane j holding columns [jV, jV+V)) sums the
+ * parent's chunk rows q, q + S, q + 2S, ... (S = 256/G slices) into four
+ * running sums taken in turn, and the slices are then added in order.  V
+ * depends on R only, so every launch form sums in the same order. */
+template <typename T, int V>
 __device__ __forceinline__ void fix_multi_body(uint32_t vb, uint32_t const *__restrict__ mlist,
                                                uint32_t const *__restrict__ hchunk,
                                                uint32_t const *__restrict__ hseg,
                                                T const *__restrict__ carry2,
                                                T *__restrict__ out, uint32_t R)
 {
-  __shared__ T red[256];
+  __shared__ T red[256 * V];
   uint32_t const h = mlist[vb];
-  uint32_t const rpi = 256u / R;
-  uint32_t const sl = threadIdx.x / R;
-  uint32_t const col = threadIdx.x - sl * R;
-  T a0 = (T)0, a1 = (T)0;
+  uint32_t const G = R / V;
+  uint32_t const rpi = 256u / G;
+  uint32_t const sl = threadIdx.x / G;
+  uint32_t const cv = (threadIdx.x - sl * G) * V;
+  T a[4][V];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < V; ++v) a[u][v] = (T)0;
   if (sl < rpi) {
     uint32_t const c1 = hchunk[h + 1];
     uint32_t c = hchunk[h] + sl;
-    for (; c + rpi < c1; c += 2 * rpi) {
-      a0 += carry2[(size_t)c * R + col];
-      a1 += carry2[(size_t)(c + rpi) * R + col];
+    T x[4][V];
+    for (; c + 3 * rpi < c1; c += 4 * rpi) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ldrow(carry2 + ((size_t)(c + u * rpi) * R + cv), x[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < V; ++v) a[u][v] += x[u][v];
     }
-    if (c < c1) a0 += carry2[(size_t)c * R + col];
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (c + u * rpi < c1) {
+        ldrow(carry2 + ((size_t)(c + u * rpi) * R + cv), x[u]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) a[u][v] += x[u][v];
+      }
   }
-  red[threadIdx.x] = a0 + a1;
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    red[threadIdx.x * V + v] = (a[0][v] + a[1][v]) + (a[2][v] + a[3][v]);
   __syncthreads();
-  if (threadIdx.x < R) {
-    T t = (T)0;
-    for (uint32_t q = 0; q < rpi; ++q) t += red[q * R + threadIdx.x];
-    out[(size_t)hseg[h] * R + threadIdx.x] = t;
+  if (threadIdx.x < G) {
+    T t[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) t[v] = (T)0;
+    for (uint32_t q = 0; q < rpi; ++q)
+#pragma unroll
+      for (int v = 0; v < V; ++v) t[v] += red[(q * G + threadIdx.x) * V + v];
+    strow<T, V>(out + ((size_t)hseg[h] * R + cv), t);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void fix_multi_any(uint32_t vb, uint32_t const *__restrict__ mlist,
+                                              uint32_t const *__restrict__ hchunk,
+                                              uint32_t const *__restrict__ hseg,
+                                              T const *__restrict__ carry2,
+                                              T *__restrict__ out, uint32_t R)
+{
+  if (R % fix_v<T>() == 0)
+    fix_multi_body<T, fix_v<T>()>(vb, mlist, hchunk, hseg, carry2, out, R);
+  else
+    fix_multi_body<T, 1>(vb, mlist, hchunk, hseg, carry2, out, R);
 }
 
 template <typename T>
@@ -534,7 +599,7 @@ __global__ __launch_bounds__(256) void k_fix_multi(uint32_t const *__restrict__ 
                                                    T const *__restrict__ carry2,
                                                    T *__restrict__ out, uint32_t R)
 {
-  fix_multi_body<T>(blockIdx.x, mlist, hchunk, hseg, carry2, out, R);
+  fix_multi_any<T>(blockIdx.x, mlist, hchunk, hseg, carry2, out, R);
 }
 
 /* ---- launchers ----------------------------------------------------------- */
@@ -543,7 +608,7 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
                         void *carry2, void *out, uint32_t R)
 {
   if (w->nchunk == 0) return hipSuccess;
-  uint32_t const gpw = 64u / R;
+  uint32_t const gpw = 64u / ((R % fix_v<T>() == 0) ? R / fix_v<T>() : R);
   uint32_t const per_block = 4u * gpw;
   hipLaunchKernelGGL((k_fix_chunk<T>), dim3((w->nchunk + per_block - 1) / per_block),
                      dim3(256), 0, s, w->nchunk, w->cstart, w->cseg, w->hslot,
@@ -763,7 +828,7 @@ __device__ __forceinline__ void ps_fix_chunk(adatm_kstage const *st, uint32_t vb
 {
   st = ps_uniform(st);
   vb = __builtin_amdgcn_readfirstlane(vb);
-  fix_chunk_body<T>(vb, st->w.nchunk, st->w.cstart, st->w.cseg, st->w.hslot, st->w.hchunk,
+  fix_chunk_body<T, 1>(vb, st->w.nchunk, st->w.cstart, st->w.cseg, st->w.hslot, st->w.hchunk,
                     st->w.hseg, (T const *)st->carry, (T *)st->carry2, (T *)st->out, st->R,
                     st->gpw);
 }
@@ -773,7 +838,7 @@ __device__ __forceinline__ void ps_fix_multi(adatm_kstage const *st, uint32_t vb
 {
   st = ps_uniform(st);
   vb = __builtin_amdgcn_readfirstlane(vb);
-  fix_multi_body<T>(vb, st->w.mlist, st->w.hchunk, st->w.hseg, (T const *)st->carry2,
+  fix_multi_any<T>(vb, st->w.mlist, st->w.hchunk, st->w.hseg, (T const *)st->carry2,
                     (T *)st->out, st->R);
 }
 
