@@ -306,8 +306,18 @@ __device__ __forceinline__ void sum_parts_body(uint32_t bid, double const *__res
   uint32_t const c = threadIdx.x >> 5, el = threadIdx.x & 31u;
   uint32_t const e = bid * 32u + el;
   double s = 0.0;
-  if (e < len)
-    for (uint32_t b = c; b < nblk; b += 8) s += part[(size_t)b * len + e];
+  if (e < len) {
+    /* four partial rows' loads in flight, added in row order */
+    uint32_t b = c;
+    for (; b + 24u < nblk; b += 32u) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] = part[(size_t)(b + 8u * u) * len + e];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s += x[u];
+    }
+    for (; b < nblk; b += 8) s += part[(size_t)b * len + e];
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   if (c == 0 && e < len) {
@@ -447,6 +457,117 @@ __device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
   }
 }
 
+/* R <= 32: the same elimination in one wave, so a pivot needs no barrier.
+ * Lane l = 32h + j holds column j of rows h, h+2, ..., h+30; the pivot row
+ * and column go through LDS, whose accesses one wave performs in program
+ * order.  Every element is updated by the same expression as in
+ * solve_inv_body, from the same operands, so both give identical bits. */
+__device__ __forceinline__ void solve_inv32_body(double const *__restrict__ G, uint32_t N,
+                                                 uint32_t m, uint32_t R,
+                                                 double *__restrict__ Vinv,
+                                                 int *__restrict__ status)
+{
+  constexpr int Q = 16;
+  __shared__ double rowk[32], colk[32];
+  if (threadIdx.x >= 64u) return;
+  uint32_t const h = threadIdx.x >> 5, j = threadIdx.x & 31u;
+  double a[Q];
+  auto build = [&](double ridge) {
+    uint32_t xo[Q];
+    double v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint32_t const i = h + 2u * (uint32_t)q;
+      xo[q] = (i < R && j < R) ? i * R + j : 0u;
+      v[q] = 1.0;
+    }
+    for (uint32_t k0 = 0; k0 < N; k0 += 4) {
+      double g[4][Q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint32_t const k = k0 + (uint32_t)u;
+        size_t const kb = (size_t)(k < N ? k : 0u) * R * R;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[u][q] = G[kb + xo[q]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint32_t const k = k0 + (uint32_t)u;
+        bool const use = k < N && k != m;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] *= use ? g[u][q] : 1.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint32_t const i = h + 2u * (uint32_t)q;
+      double x = 1.0;
+      if (i < R && j < R) {
+        x = v[q];
+        if (i == j) x += ridge;
+      }
+      a[q] = x;
+    }
+  };
+  build(0.0);
+  int bad = 0, reg = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (attempt > 0) {
+      /* the diagonal through LDS, its maximum read by every lane */
+      build(0.0);
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (h + 2u * (uint32_t)q == j) colk[j] = a[q];
+      __builtin_amdgcn_wave_barrier();
+      double dm = 0.0;
+      for (uint32_t i = 0; i < R; ++i) dm = colk[i] > dm ? colk[i] : dm;
+      __builtin_amdgcn_wave_barrier();
+      build(dm * (attempt == 1 ? 1e-10 : 1e-6));
+    }
+    bad = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if ((uint32_t)k >= R) continue; /* wave-uniform */
+      if (h == (uint32_t)(k & 1)) rowk[j] = a[k >> 1];
+      if (j == (uint32_t)k) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) colk[h + 2u * (uint32_t)q] = a[q];
+      }
+      __builtin_amdgcn_wave_barrier();
+      double const p = rowk[k];
+      bad |= !(p > 0.0);
+      double const pinv = 1.0 / p;
+      double const rk = ((j == (uint32_t)k) ? 1.0 : rowk[j]) * pinv;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        uint32_t const i = h + 2u * (uint32_t)q;
+        double const f = colk[i];
+        a[q] = (i == (uint32_t)k) ? rk : ((j == (uint32_t)k) ? 0.0 : a[q]) - f * rk;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!bad) break;
+    reg = 1;
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint32_t const i = h + 2u * (uint32_t)q;
+    if (i < R && j < R) Vinv[(size_t)i * R + j] = a[q];
+  }
+  if (threadIdx.x == 0 && status) {
+    int const v = bad ? 2 : reg;
+    if (v > *status) *status = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_solve_inv32(double const *__restrict__ G, uint32_t N,
+                                                    uint32_t m, uint32_t R,
+                                                    double *__restrict__ Vinv,
+                                                    int *__restrict__ status)
+{
+  solve_inv32_body(G, N, m, R, Vinv, status);
+}
+
 template <int Q>
 __global__ __launch_bounds__(DB) void k_solve_inv(double const *__restrict__ G,
                                                   uint32_t N, uint32_t m,
@@ -545,12 +666,28 @@ __device__ __forceinline__ void lambda_body(double const *__restrict__ part,
   uint32_t const C = DB / R;
   uint32_t const c = threadIdx.x / R, r = threadIdx.x - c * R;
   double s = 0.0, m = -1.0e300;
-  if (c < C)
-    for (uint32_t b = c; b < nblk; b += C) {
+  if (c < C) {
+    /* four partial rows' loads in flight, combined in row order */
+    uint32_t b = c;
+    for (; b + 3u * C < nblk; b += 4u * C) {
+      double x[4], y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        x[u] = part[(size_t)(b + u * C) * 2 * R + r];
+        y[u] = part[(size_t)(b + u * C) * 2 * R + R + r];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s += x[u];
+        m = y[u] > m ? y[u] : m;
+      }
+    }
+    for (; b < nblk; b += C) {
       s += part[(size_t)b * 2 * R + r];
       double const v = part[(size_t)b * 2 * R + R + r];
       m = v > m ? v : m;
     }
+  }
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = m;
   __syncthreads();
@@ -588,7 +725,18 @@ __device__ __forceinline__ void inner_part_body(uint32_t bid, uint32_t nblk, T c
     uint64_t const chunk = (I + nblk - 1) / nblk;
     uint64_t const lo = (uint64_t)bid * chunk;
     uint64_t const hi = lo + chunk < I ? lo + chunk : I;
-    for (uint64_t i = lo + rr; i < hi; i += rpi)
+    uint64_t i = lo + rr;
+    for (; i + 3u * rpi < hi; i += 4u * rpi) {
+      double x[4], y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        x[u] = (double)A[(size_t)(i + (uint64_t)u * rpi) * R + cc];
+        y[u] = (double)M[(size_t)(i + (uint64_t)u * rpi) * R + cc];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s += x[u] * y[u];
+    }
+    for (; i < hi; i += rpi)
       s += (double)A[(size_t)i * R + cc] * (double)M[(size_t)i * R + cc];
   }
   red[threadIdx.x] = s;
@@ -650,7 +798,15 @@ __device__ __forceinline__ void fit_final_body(double const *__restrict__ part,
   double inner = 0.0;
   for (uint32_t r = threadIdx.x; r < R; r += DB) {
     double s = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) s += part[(size_t)b * R + r];
+    uint32_t b = 0;
+    for (; b + 4u <= nblk; b += 4u) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] = part[(size_t)(b + u) * R + r];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s += x[u];
+    }
+    for (; b < nblk; ++b) s += part[(size_t)b * R + r];
     inner += s * lambda[r];
   }
   double nrm = 0.0;
@@ -737,6 +893,12 @@ __global__ __launch_bounds__(DB) void g_solve(adatm_kdense const *__restrict__ d
 {
   adatm_kdense const &x = d[blockIdx.x];
   solve_inv_body<Q>(x.G, x.N, x.m, x.R, x.Vinv, x.status);
+}
+
+__global__ __launch_bounds__(64) void g_solve32(adatm_kdense const *__restrict__ d)
+{
+  adatm_kdense const &x = d[blockIdx.x];
+  solve_inv32_body(x.G, x.N, x.m, x.R, x.Vinv, x.status);
 }
 
 template <typename T>
@@ -853,12 +1015,8 @@ int adatm_k_solve_inv(hipStream_t s, double const *G, uint32_t N, uint32_t m,
                       uint32_t R, double *Vinv, int *status)
 {
   if (R > 64) return herr(hipErrorInvalidValue);
-  if (R <= 8)
-    hipLaunchKernelGGL((k_solve_inv<2>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
-  else if (R <= 16)
-    hipLaunchKernelGGL((k_solve_inv<4>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
-  else if (R <= 32)
-    hipLaunchKernelGGL((k_solve_inv<8>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
+  if (R <= 32)
+    hipLaunchKernelGGL(k_solve_inv32, dim3(1), dim3(64), 0, s, G, N, m, R, Vinv, status);
   else
     hipLaunchKernelGGL((k_solve_inv<16>), dim3(1), dim3(DB), 0, s, G, N, m, R, Vinv, status);
   return herr(hipGetLastError());
@@ -879,12 +1037,8 @@ int adatm_k_dense_grouped(hipStream_t s, int prec, adatm_kdense const *desc, uin
 {
   if (n == 0) return 0;
   if (Rmax > 64) return herr(hipErrorInvalidValue);
-  if (Rmax <= 8)
-    hipLaunchKernelGGL((g_solve<2>), dim3(n), dim3(DB), 0, s, desc);
-  else if (Rmax <= 16)
-    hipLaunchKernelGGL((g_solve<4>), dim3(n), dim3(DB), 0, s, desc);
-  else if (Rmax <= 32)
-    hipLaunchKernelGGL((g_solve<8>), dim3(n), dim3(DB), 0, s, desc);
+  if (Rmax <= 32)
+    hipLaunchKernelGGL(g_solve32, dim3(n), dim3(64), 0, s, desc);
   else
     hipLaunchKernelGGL((g_solve<16>), dim3(n), dim3(DB), 0, s, desc);
   hipError_t e = hipGetLastError();
