@@ -421,8 +421,8 @@ int adatm_batch_persistent(adatm_batch const *b);
  * recorded as stages and run as ordinary launches, one per depth of the
  * instances' launch sequences and kind of stage, each covering every
  * instance at that depth (same eligibility as the persistent form;
- * bit-identical results; on 64 uneven instances 3.9 ms per sweep against 5.1
- * for the branches and 5.7 for the persistent launch).  A batch outside the
+ * bit-identical results; on 64 uneven instances 3.5 ms per sweep against 5.0
+ * for the branches and 5.3 for the persistent launch).  A batch outside the
  * eligibility runs the branches.  adatm_batch_form: the form of the last
  * built batched sweep. */
 enum { ADATM_FORM_FORKED = 0, ADATM_FORM_PERSISTENT = 1, ADATM_FORM_GROUPED = 2 };
