@@ -421,13 +421,15 @@ def test_degenerate_tensors(case, prec):
         e.close()
 
 
-def test_solve_status_reports_a_ridge():
+@pytest.mark.parametrize("R", [4, 40])
+def test_solve_status_reports_a_ridge(R):
     """Factors whose columns are all equal make every hadamard of Grams
     exactly singular (powers of two keep the elimination exact), so the first
     solve finds a zero pivot and solves with a ridge: the host reads status 1
-    and the fits stay finite.  A well-posed run reads status 0."""
-    print("solve status", flush=True)
-    I, J, K, R = 8, 4, 16, 4
+    and the fits stay finite (R = 4: the one-wave elimination; R = 40: the
+    four-wave one).  A well-posed run reads status 0."""
+    print(f"solve status R={R}", flush=True)
+    I, J, K = 8, 4, 16
     g = np.stack(np.meshgrid(np.arange(I), np.arange(J), np.arange(K), indexing="ij"))
     inds = g.reshape(3, -1)
     vals = np.random.default_rng(1).random(inds.shape[1]) + 0.5
