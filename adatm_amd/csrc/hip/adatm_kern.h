@@ -61,6 +61,11 @@ typedef struct {
   uint32_t wide;        /* nonzero: 16-byte row pieces per lane (R/4 or R/2 lanes
                            per item; R a multiple, >= 8 lanes) */
   uint32_t const *orow; /* output row of each top node (NULL: the node) */
+  /* nonzero: every U[j], 1 <= j < K, spans less than 2^31 bytes, so the
+     wide-layout reductions read the node rows by raw buffer loads whose
+     offsets fit in 32 bits (a child that does not close its node then loads
+     nothing) */
+  uint32_t nbuf;
   uint8_t const *cfd;
   uint32_t const *ids[ADATM_KMAXFUSE + 1];
   void const *U[ADATM_KMAXFUSE + 1];

@@ -94,6 +94,43 @@ __device__ __forceinline__ void strow(T *p, T const (&o)[V])
     __builtin_memcpy(__builtin_assume_aligned(p, V * sizeof(T)), o, V * sizeof(T));
 }
 
+/* Raw buffer access to a factor of less than 2^31 bytes (the descriptor's
+ * record count; gfx9 buffer format word): a load at an offset at or past
+ * the count returns zeros without touching memory. */
+#define BUF_RECORDS 0x7fffffffu
+#define BUF_NONE 0x80000000u /* an offset that loads nothing */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(void const *p)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)BUF_RECORDS,
+                                           0x00020000);
+}
+
+/* ldrow from a buffer at byte offset off (V * sizeof(TU) bytes, one load) */
+template <typename TU, typename T, int V>
+__device__ __forceinline__ void ldrow_buf(__amdgpu_buffer_rsrc_t rs, uint32_t off, T (&o)[V])
+{
+  constexpr int B = V * (int)sizeof(TU);
+  static_assert(B == 2 || B == 4 || B == 8 || B == 16, "row piece size");
+  TU r[V];
+  if constexpr (B == 16) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    u4 const t = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+    __builtin_memcpy(r, &t, 16);
+  } else if constexpr (B == 8) {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    u2 const t = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+    __builtin_memcpy(r, &t, 8);
+  } else if constexpr (B == 4) {
+    uint32_t const t = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0);
+    __builtin_memcpy(r, &t, 4);
+  } else {
+    uint16_t const t = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)off, 0, 0);
+    __builtin_memcpy(r, &t, 2);
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v) o[v] = ldu(r + v);
+}
+
 /* strow with the nontemporal hint (global_store_dwordx4 ... nt for a 16-byte
  * piece): for rows written once and not read again by the writing pass, so
  * that they do not push the factor rows the pass gathers out of L2 */

@@ -56,7 +56,7 @@ namespace {
  * with the per-child metadata read from the holding lane (ds_bpermute; for
  * R > 32, one group per wave, v_readlane into scalars). */
 template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V,
-          bool SIDE>
+          bool SIDE, bool NB = false>
 __device__ __forceinline__ void reduce_body(
     uint32_t vb, uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -107,6 +107,12 @@ __device__ __forceinline__ void reduce_body(
     for (int v = 0; v < V; ++v) acc[j][v] = (T)0;
   uint32_t const *__restrict__ idb = ch.ids[K];
   TU const *__restrict__ Ub = (TU const *)ch.U[K];
+  /* NB: buffer descriptors of the intermediate levels' factors */
+  __amdgpu_buffer_rsrc_t nrs[K];
+  if constexpr (NB) {
+#pragma unroll
+    for (int j = 1; j < K; ++j) nrs[j] = buf_rsrc(ch.U[j]);
+  }
   /* SIDE: the partial-row slots of the nodes this item's ends cut, and the
      item's first node, at every relative level written beside the top */
   uint32_t shd[K], stl[K], nfst[K];
@@ -177,13 +183,21 @@ __device__ __forceinline__ void reduce_body(
           ldrow(xin + ((size_t)(w0 + src) * R + cv), xv[i]);
 #pragma unroll
         for (int j = 1; j < K; ++j) {
-          /* a level-j node's row is used once, when the node closes: a
-             child that does not close it loads row 0 instead (the group's
-             close bit; no branch, and the line stays cached), so the node's
-             row is fetched once per node rather than once per child */
+          /* a level-j node's row is used once, when the node closes (the
+             group's close bit), so it is fetched once per node rather than
+             once per child, with no branch among the batch's loads: NB
+             (every intermediate factor under 2^31 bytes), a child that does
+             not close its node loads through a buffer at an offset past the
+             records, which returns zeros without a memory access; else it
+             loads row 0, whose line stays cached */
           uint32_t const nid_j = getl(wn[j], src);
-          uint32_t const row_j = ((cm[j] >> (s0 + (uint32_t)i)) & 1u) ? nid_j : 0u;
-          ldrow((TU const *)ch.U[j] + (row_j * R + cv), uc[j][i]);
+          bool const cl = (cm[j] >> (s0 + (uint32_t)i)) & 1u;
+          if constexpr (NB) {
+            ldrow_buf<TU>(nrs[j], cl ? (nid_j * R + cv) * (uint32_t)sizeof(TU) : BUF_NONE,
+                          uc[j][i]);
+          } else {
+            ldrow((TU const *)ch.U[j] + ((cl ? nid_j : 0u) * R + cv), uc[j][i]);
+          }
         }
       }
     };
@@ -277,7 +291,7 @@ __device__ __forceinline__ void reduce_body(
 
 /* one block of the reduction per workgroup of the launch */
 template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, bool ONE, int V,
-          bool SIDE>
+          bool SIDE, bool NB = false>
 __global__ __launch_bounds__(256) void k_reduce(
     uint32_t nitems, uint32_t const *__restrict__ istart,
     uint32_t const *__restrict__ iseg, uint32_t const *__restrict__ islot,
@@ -285,7 +299,7 @@ __global__ __launch_bounds__(256) void k_reduce(
     T const *__restrict__ xin, T *__restrict__ out, T *__restrict__ carry,
     uint32_t R, uint32_t gpw)
 {
-  reduce_body<T, TU, LEAF, K, RB, DB, ONE, V, SIDE>(blockIdx.x, nitems, istart, iseg, islot,
+  reduce_body<T, TU, LEAF, K, RB, DB, ONE, V, SIDE, NB>(blockIdx.x, nitems, istart, iseg, islot,
                                                     ianc, ch, xin, out, carry, R, gpw);
 }
 
@@ -621,7 +635,7 @@ hipError_t launch_fixup(hipStream_t s, adatm_kwork const *w, void const *carry,
 }
 
 template <typename T, typename TU, bool LEAF, int K, int RB, bool DB, int V,
-          bool SIDE = false>
+          bool SIDE = false, bool NB = false>
 void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
                   void const *xin, void *out, void *carry, uint32_t R)
 {
@@ -637,7 +651,7 @@ void go_reduce_rb(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
       return;
     }
   }
-  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false, V, SIDE>), grid, dim3(256),
+  hipLaunchKernelGGL((k_reduce<T, TU, LEAF, K, RB, DB, false, V, SIDE, NB>), grid, dim3(256),
                      0, s, w->nitems, w->start, w->seg, w->slot, w->anc, *ch,
                      (T const *)xin, (T *)out, (T *)carry, R, gpw);
 }
@@ -664,10 +678,16 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
     if (ch->side) {
       bool const r8 = K <= 3 && rb != 4 && G >= 8;
       if (wide) {
-        if (r8)
-          go_reduce_rb<T, TU, LEAF, K, 8, false, VW, true>(s, w, ch, xin, out, carry, R);
-        else
+        if (r8) {
+          if (ch->nbuf)
+            go_reduce_rb<T, TU, LEAF, K, 8, false, VW, true, true>(s, w, ch, xin, out, carry, R);
+          else
+            go_reduce_rb<T, TU, LEAF, K, 8, false, VW, true>(s, w, ch, xin, out, carry, R);
+        } else if (ch->nbuf) {
+          go_reduce_rb<T, TU, LEAF, K, 4, false, VW, true, true>(s, w, ch, xin, out, carry, R);
+        } else {
           go_reduce_rb<T, TU, LEAF, K, 4, false, VW, true>(s, w, ch, xin, out, carry, R);
+        }
       } else if (r8) {
         go_reduce_rb<T, TU, LEAF, K, 8, false, 1, true>(s, w, ch, xin, out, carry, R);
       } else {
@@ -677,11 +697,22 @@ void go_reduce(hipStream_t s, adatm_kwork const *w, adatm_kchain const *ch,
     }
   }
   if (wide) {
+    /* node rows through buffers where the factors allow it (passes of
+       >= 2 levels; one level has no node rows) */
+    bool const nbuf = K >= 2 && ch->nbuf;
     if constexpr (K <= 3) {
-      if (rb == 4)
-        go_reduce_rb<T, TU, LEAF, K, 4, false, VW>(s, w, ch, xin, out, carry, R);
-      else
+      if (rb == 4) {
+        if (nbuf)
+          go_reduce_rb<T, TU, LEAF, K, 4, false, VW, false, true>(s, w, ch, xin, out, carry, R);
+        else
+          go_reduce_rb<T, TU, LEAF, K, 4, false, VW>(s, w, ch, xin, out, carry, R);
+      } else if (nbuf) {
+        go_reduce_rb<T, TU, LEAF, K, 8, false, VW, false, true>(s, w, ch, xin, out, carry, R);
+      } else {
         go_reduce_rb<T, TU, LEAF, K, 8, false, VW>(s, w, ch, xin, out, carry, R);
+      }
+    } else if (nbuf) {
+      go_reduce_rb<T, TU, LEAF, K, 4, false, VW, false, true>(s, w, ch, xin, out, carry, R);
     } else {
       go_reduce_rb<T, TU, LEAF, K, 4, false, VW>(s, w, ch, xin, out, carry, R);
     }

@@ -420,9 +420,15 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
   ch.wide = (e->batch & ADATM_BATCH_WIDE) ? 1u : 0u;
   ch.orow = (lt == 0) ? g->orow : NULL;
   ch.cfd = g->cfd[lb];
+  /* node rows through raw buffer loads when every intermediate factor
+     spans less than 2^31 bytes (their offsets then fit in 32 bits) */
+  ch.nbuf = 1u;
   for (uint32_t j = 1; j <= ch.K; ++j) {
     ch.ids[j] = g->ids[lt + j];
     ch.U[j] = gather_factor(e, g->order[lt + j]);
+    if (j < ch.K &&
+        (double)e->dims[g->order[lt + j]] * e->R * e->wsz >= (double)0x7fffff00u)
+      ch.nbuf = 0u;
   }
   for (uint32_t j = 1; j < ch.K; ++j)
     if (P->side[j]) {
