@@ -430,6 +430,10 @@ __device__ __forceinline__ void solve_inv_body(double const *__restrict__ G,
       __syncthreads();
       double const p = rowk[b][k];
       bad |= !(p > 0.0);
+      /* an attempt that will be retried stops at its first bad pivot (every
+         thread reads the same p, so all leave together); the last attempt
+         runs to the end */
+      if (bad && attempt < 2) break;
       double const pinv = 1.0 / p;
       double const rk = ((j == k) ? 1.0 : rowk[b][j]) * pinv;
 #pragma unroll
