@@ -25,6 +25,7 @@ SPLATT_ERROR_NUMERIC = 5
 
 FP64, FP32, BF16ROWS = 0, 1, 2
 ENGINE_STAGED = 0x1
+ENGINE_GLOBAL_NODE_ROWS = 0x2  # node rows never through buffer loads
 
 
 def ENGINE_STAGED_GROUP(g: int) -> int:

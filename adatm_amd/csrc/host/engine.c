@@ -422,7 +422,7 @@ static int run_pass(adatm_engine *e, dgroup *g, uint32_t lb, uint32_t lt,
   ch.cfd = g->cfd[lb];
   /* node rows through raw buffer loads when every intermediate factor
      spans less than 2^31 bytes (their offsets then fit in 32 bits) */
-  ch.nbuf = 1u;
+  ch.nbuf = (e->flags & ADATM_ENGINE_GLOBAL_NODE_ROWS) ? 0u : 1u;
   for (uint32_t j = 1; j <= ch.K; ++j) {
     ch.ids[j] = g->ids[lt + j];
     ch.U[j] = gather_factor(e, g->order[lt + j]);
@@ -1403,7 +1403,7 @@ int adatm_engine_create_ex(adatm_coo const *t, uint64_t rank, adatm_plan const *
                            int precision, int device, uint32_t red_tile,
                            uint32_t scat_tile, uint32_t flags, adatm_engine **out)
 {
-  uint32_t const known = ADATM_ENGINE_STAGED |
+  uint32_t const known = ADATM_ENGINE_STAGED | ADATM_ENGINE_GLOBAL_NODE_ROWS |
                          (ADATM_ENGINE_STAGED_GROUP(ADATM_MAX_NMODES) -
                           ADATM_ENGINE_STAGED_GROUP(0));
   if (flags & ~known) {

@@ -260,6 +260,11 @@ int adatm_engine_create_tiled(adatm_coo const *t, uint64_t rank,
  * summation grouping. */
 #define ADATM_ENGINE_STAGED 0x1u
 #define ADATM_ENGINE_STAGED_GROUP(g) (0x100u << (g))
+/* The fused reductions read intermediate nodes' factor rows by raw buffer
+ * loads when every such factor spans less than 2^31 bytes (a child that does
+ * not close its node then loads nothing); with this flag they always use
+ * the path of larger factors (a stand-in row).  Results are the same bits. */
+#define ADATM_ENGINE_GLOBAL_NODE_ROWS 0x2u
 int adatm_engine_create_ex(adatm_coo const *t, uint64_t rank, adatm_plan const *plan,
                            int precision, int device, uint32_t red_tile,
                            uint32_t scat_tile, uint32_t flags, adatm_engine **out);

@@ -247,6 +247,39 @@ def test_every_batch_every_plan(prec, R):
         e.close()
 
 
+@pytest.mark.parametrize("prec", [A.FP32, A.FP64, A.BF16ROWS])
+def test_node_rows_buffer_and_global_paths_agree(prec):
+    """The fused reductions read intermediate nodes' rows through buffer
+    loads (a non-closing child loads nothing) when every factor spans less
+    than 2^31 bytes, else through a stand-in row (ENGINE_GLOBAL_NODE_ROWS
+    forces that path): every plan's MTTKRPs, leaf to root and staged, in the
+    wide and narrow layouts, agree bit for bit, and with the oracle."""
+    print(f"node rows prec={prec}", flush=True)
+    t = _tensor([150, 100, 60, 40, 20], 40000, A.DIST_ZIPF, seed=43)
+    R = 32
+    U = _factors(t, R, seed=6)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(t.nmodes)]
+    for plan in A.plan_candidates(t.nmodes, t.dims):
+        for staged in (False, True):
+            outs = []
+            for fl in (0, A.ENGINE_GLOBAL_NODE_ROWS):
+                e = A.Engine(t, R, plan, precision=prec, tiles=(32, 16, 0), staged=staged,
+                             flags=fl)
+                for m in range(t.nmodes):
+                    e.set_factor(m, U[m])
+                o = []
+                for batch in (0, 8 | A.BATCH_WIDE, 4):
+                    e.set_batch(batch)
+                    o += [e.mttkrp(m) for m in range(t.nmodes)]
+                outs.append(o)
+                e.close()
+            for m in range(t.nmodes):
+                assert O.rel_fro(outs[0][m], ref[m]) <= TOL[prec], (plan.groups(t.nmodes), m)
+            for a, b in zip(outs[0], outs[1]):
+                assert np.array_equal(a, b), (plan.groups(t.nmodes), staged)
+
+
 @pytest.mark.parametrize("precs", [(A.FP32,) * 4, (A.FP64,) * 4,
                                    (A.FP32, A.FP64, A.FP32, A.FP64)])
 def test_batched_cpd_matches_each_instance(precs):
