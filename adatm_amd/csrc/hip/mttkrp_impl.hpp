@@ -308,6 +308,17 @@ __global__ __launch_bounds__(256) void k_reduce(
  * batch's Z rows and ancestor factor rows are then gathered together.  A
  * batch in which no output row ends is summed without per-node checks. */
 #define SB 8
+static_assert(SB >= 2 && (SB & (SB - 1)) == 0, "the batch sum is a pairwise tree");
+
+/* v[O][c] + ... + v[O+N-1][c] as a pairwise tree: ((v0+v1)+(v2+v3))+... */
+template <int N, int O, typename T, int V>
+__device__ __forceinline__ T tree_sum(T const (&v)[SB][V], int c)
+{
+  if constexpr (N == 1)
+    return v[O][c];
+  else
+    return tree_sum<N / 2, O>(v, c) + tree_sum<N / 2, O + N / 2>(v, c);
+}
 
 template <typename T, typename TU, int D, bool ZS, int V>
 __device__ __forceinline__ void scatter_body(
@@ -400,11 +411,7 @@ __device__ __forceinline__ void scatter_body(
       if (nb == (uint32_t)SB && (heavy || e >= c0 + SB)) {
         /* no row ends before the batch's last node (a piece has no row end) */
 #pragma unroll
-        for (int c = 0; c < V; ++c) {
-          T const s01 = v[0][c] + v[1][c], s23 = v[2][c] + v[3][c];
-          T const s45 = v[4][c] + v[5][c], s67 = v[6][c] + v[7][c];
-          acc[c] += (s01 + s23) + (s45 + s67);
-        }
+        for (int c = 0; c < V; ++c) acc[c] += tree_sum<SB, 0>(v, c);
         if (!heavy && e == c0 + SB) { /* the row ends with the batch */
           strow<T, V>(out + ((size_t)p * R + cv), acc);
 #pragma unroll
