@@ -19,7 +19,7 @@ enum { KP_FP64 = 0, KP_FP32 = 1, KP_BF16ROWS = 2 };
 
 #define ADATM_KMAXL 16
 #define ADATM_KMAXFUSE 8 /* levels one k_reduce launch folds */
-#define ADATM_KMAXSCAT 8 /* deepest level k_scatter handles */
+#define ADATM_KMAXSCAT 16 /* deepest level k_scatter handles (the engine's mode limit) */
 
 /* Work list of a segmented reduction (see adatm_segwork in the host code):
  * item k walks bottom positions [start[k], start[k+1]); a light item covers

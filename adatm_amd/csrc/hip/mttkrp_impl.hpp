@@ -361,12 +361,15 @@ __device__ __forceinline__ void scatter_body(
   uint32_t const gb = grp * G;
   uint32_t const col = lane - gb;
   uint32_t const W = (G >= SB) ? (G / SB) * SB : G;
+  /* D > 8: one instance bounds the levels 9..16, the depth sc.d runs */
+  constexpr bool RT = D > 8;
+  uint32_t const dd = RT ? sc.d : (uint32_t)D;
   auto load_win = [&](uint32_t w0, uint32_t &mn, uint32_t (&mr)[D]) {
     uint32_t q = w0 + (col < W ? col : 0u);
     q = (q < b) ? q : b - 1;
     mn = sc.node[q];
 #pragma unroll
-    for (int l = 0; l < D; ++l) mr[l] = sc.rid[l][q];
+    for (int l = 0; l < D; ++l) mr[l] = (!RT || (uint32_t)l < dd) ? sc.rid[l][q] : 0u;
   };
   uint32_t wnd, wrr[D];
   load_win(a, wnd, wrr);
@@ -385,7 +388,8 @@ __device__ __forceinline__ void scatter_body(
         uint32_t const src = gb + (c0 - w0) + ((uint32_t)i < nb ? (uint32_t)i : nb - 1);
         nd[i] = lane_get(wnd, src);
 #pragma unroll
-        for (int l = 0; l < D; ++l) rr[l][i] = lane_get(wrr[l], src);
+        for (int l = 0; l < D; ++l)
+          rr[l][i] = (!RT || (uint32_t)l < dd) ? lane_get(wrr[l], src) : 0u;
       }
       T v[SB][V];
 #pragma unroll
@@ -394,6 +398,7 @@ __device__ __forceinline__ void scatter_body(
         ldrow((TU const *)sc.U[0] + (rr[0][i] * R + cv), y);
 #pragma unroll
         for (int l = 1; l < D; ++l) {
+          if (RT && (uint32_t)l >= dd) continue; /* uniform */
           ldrow((TU const *)sc.U[l] + (rr[l][i] * R + cv), u);
 #pragma unroll
           for (int c = 0; c < V; ++c) y[c] = y[c] * u[c];
@@ -816,7 +821,10 @@ hipError_t scatter_d(hipStream_t s, adatm_kwork const *w, uint32_t const *rowptr
     case 5: go_scatter<T, TU, 5>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
     case 6: go_scatter<T, TU, 6>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
     case 7: go_scatter<T, TU, 7>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
-    default: go_scatter<T, TU, 8>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    case 8: go_scatter<T, TU, 8>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
+    /* deeper levels (tensors of more than 10 modes): one instance, the
+       depth a runtime bound */
+    default: go_scatter<T, TU, ADATM_KMAXSCAT>(s, w, rowptr, sc, Z, zs, out, carry, R); break;
     }
     return hipGetLastError();
   }

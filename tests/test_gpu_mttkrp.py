@@ -64,6 +64,30 @@ def test_every_plan_every_mode(case, prec, staged):
         e.close()
 
 
+@pytest.mark.parametrize("N,prec,R", [(10, A.FP64, 16), (10, A.FP32, 32), (12, A.FP64, 16),
+                                      (16, A.FP64, 8), (16, A.FP32, 32)])
+def test_many_modes_every_plan(N, prec, R):
+    """Tensors of 10, 12 and 16 modes (the engine's limit): leaf-to-root
+    passes are cut at 8 fused levels and reuse scatters reach 8, 10 and 14
+    levels deep (levels 9..16 share one kernel instance whose depth is a
+    runtime bound); every candidate plan and mode against the COO brute
+    force."""
+    print(f"many modes N={N} prec={prec} R={R}", flush=True)
+    dims = [6 + (m % 3) for m in range(N)]
+    t = _tensor(dims, 20000, A.DIST_CLUSTER, seed=200 + N)
+    U = _factors(t, R)
+    inds, vals = t.inds().astype(np.int64), t.vals.copy()
+    ref = [O.coo_mttkrp(inds, vals, U, m) for m in range(N)]
+    for plan in A.plan_candidates(N, t.dims):
+        e = A.Engine(t, R, plan, precision=prec)
+        for m in range(N):
+            e.set_factor(m, U[m])
+        for m in range(N):
+            err = O.rel_fro(e.mttkrp(m), ref[m])
+            assert err <= TOL[prec], (plan.groups(N), m, err)
+        e.close()
+
+
 def test_bf16_rows():
     dims, nd, dist, R = CASES[2]
     t = _tensor(dims, nd, dist, seed=7)
