@@ -119,6 +119,14 @@ def test_engine_rejects_bad_arguments_before_any_device_call(lib):
     tb = A.Tensor.synth(big)
     rc = L.adatm_engine_create(C.byref(tb.c), 64, C.byref(plan), A.FP32, 0, C.byref(h))
     assert rc == 2 and b"2^32" in L.adatm_last_error()
+    # no nonzeros
+    te = A.Tensor.from_arrays(np.zeros((4, 0), dtype=np.uint32), np.zeros(0), [30, 20, 10, 8])
+    rc = L.adatm_engine_create(C.byref(te.c), 8, C.byref(plan), A.FP32, 0, C.byref(h))
+    assert rc == 2 and not h.value
+    # a creation flag this build does not know
+    rc = L.adatm_engine_create_ex(C.byref(t.c), 8, C.byref(plan), A.FP32, 0, 0, 0, 0x4,
+                                  C.byref(h))
+    assert rc == 2 and b"unknown creation flags" in L.adatm_last_error()
 
 
 def _bf16_rne(u: int) -> int:
